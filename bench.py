@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: fused extract_pore_network on a 512^3 fp32 SDF on one MI355X.
+
+One step = one pass of the fused pipeline (pores, watershed segmentation, topology) over the
+device-resident SDF, exactly what pnm_extract_pore_network_d does: the SDF is in HBM before the
+timed region, the segmentation stays in HBM, the pore list and the unique connection list are
+written to device buffers. Prints ONE JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--workload polydisperse|boolean] [--size 512]
+
+`value` is voxels per second of the whole step (Gvox/s). `roofline` is the dominant kernel's
+algorithmic bytes per launch over its average duration, timed live with HIP events on the
+launch stream. `cpu_baseline` times the serial C oracle (1 core) on a bounded z-slab of the same
+volume.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "peclet-pnm_amd"), os.path.join(ROOT, "oracle")]
+
+METRIC = "extract_pore_network Gvox/s @512³ (% HBM BW); batched & surrogate speedup"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory, one streaming pass) bytes per voxel of each per-voxel kernel.
+ALGO_BYTES_PER_VOXEL = {
+    "k_pre": 12,          # read sdf 4; write marker label 4, ascent successor 4
+    "k_pre_peaks": 4,     # read sdf 4
+    "k_ccl_union": 4,     # read marker labels 4 (union-find writes are sparse)
+    "k_ccl_flatten": 8,   # read + write labels
+    "k_roots_min": 16,    # read sdf 4, labels 4, successor 4; write roots 4
+    "k_seg": 16,          # read sdf 4, roots 4, labels 4; write seg 4
+    "k_topo_cand": 4,     # read seg 4
+}
+PIPELINE_BYTES_PER_VOXEL = 64  # S1..S6 compulsory traffic of extract_pore_network
+
+
+def make_sdf(workload, size, seed):
+    from peclet.pnm import synth
+
+    shape = (size, size, size)
+    t = time.time()
+    if workload == "polydisperse":
+        sdf, phi, info = synth.polydisperse_sdf(shape, 9.0, 0.3, 0.35, seed=seed)
+        desc = {"workload": "polydisperse grain pack, lognormal radii (median 9 vox, sigma 0.3)"
+                            ", RSA + uniform overgrowth",
+                "porosity": round(phi, 4), "spheres": info["spheres"]}
+    else:
+        sdf = synth.boolean_sdf(shape, 9.0, 0.3, 0.5, seed=seed)
+        desc = {"workload": "Boolean model of overlapping spheres, lognormal radii "
+                            "(median 9 vox, sigma 0.3)",
+                "porosity": round(float((sdf > 0).mean()), 4)}
+    desc["gen_s"] = round(time.time() - t, 2)
+    return sdf, desc
+
+
+def cpu_baseline(sdf, slab):
+    """Serial C oracle (1 core) on the first `slab` z-slices, as their own periodic volume."""
+    import numpy as np
+    import pnm_oracle as orc
+
+    crop = np.ascontiguousarray(sdf[:slab])
+    t = time.time()
+    orc.extract_pore_network(crop, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dt = time.time() - t
+    return {"value": round(crop.size / dt / 1e9, 6), "unit": "Gvox/s", "cores": 1,
+            "kind": "port",
+            "sample": f"serial C oracle, extract_pore_network on z-slab {crop.shape} "
+                      f"({crop.size} voxels) of the same SDF, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--workload", default="polydisperse", choices=["polydisperse", "boolean"])
+    ap.add_argument("--seed", type=int, default=2001)
+    ap.add_argument("--cpu-slab", type=int, default=96)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU only (the 512^3 volume is one GPU's block); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from peclet.pnm import capi
+
+    sdf, desc = make_sdf(args.workload, args.size, args.seed)
+    shape = sdf.shape
+    n = sdf.size
+    dims = (shape[2], shape[1], shape[0])
+    dev = torch.device("cuda:0")
+    s = torch.from_numpy(sdf).to(dev)
+    seg = torch.empty(n, dtype=torch.int32, device=dev)
+    pores = torch.empty(4 * 1_000_000, dtype=torch.float32, device=dev)
+    pairs = torch.empty(2 * 40_000_000, dtype=torch.int32, device=dev)
+    ctx = capi.Context(0)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream.cuda_stream)
+
+    def step():
+        return ctx.extract_pore_network_d(s, dims, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), pores, seg,
+                                          pairs)
+
+    for _ in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        res = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    npores, total_pores, npairs = res
+
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4),
+                   "avg_ms": round(v[0] / v[1], 5), "launches_per_step": v[1] / args.steps}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    dom = next((k for k in kernels if k in ALGO_BYTES_PER_VOXEL), None)
+    roof = None
+    if dom is not None:
+        bpl = ALGO_BYTES_PER_VOXEL[dom] * n
+        avg_s = kernels[dom]["avg_ms"] / 1e3
+        ach = bpl / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": bpl,
+                "share_of_step": round(kernels[dom]["ms_per_step"] / ms, 3)}
+    gvox = n / (ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(gvox, 3),
+        "unit": "Gvox/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": dict(desc, shape=list(shape), spacing=[1.0, 1.0, 1.0],
+                       pores=npores, pore_labels=int(seg.max().item()),
+                       connections=npairs),
+        "hbm_frac_pipeline": round(PIPELINE_BYTES_PER_VOXEL * n / (ms / 1e3) / 1e9
+                                   / HBM_PEAK_GBS, 4),
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sdf, min(args.cpu_slab, shape[0]))
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * pnm_capi.h — C-ABI of the MI355X (gfx950) pore-network extraction engine, libpnm_hip.so.
+ *
+ * Plain pointers and sizes only (no C++ or torch types). Every entry point replaces one host
+ * wrapper of the reference's src/pore_extraction.hpp (cited per function); the Python module
+ * `peclet.pnm` (built over this header) mirrors src/pnm_bindings.cpp.
+ *
+ * Conventions (as the reference's core, pore_extraction.hpp:59-64):
+ *   - dims/origin/spacing are X,Y,Z. The Python layer reorders the zyx arguments and casts
+ *     origin/spacing to float exactly as pnm_bindings.cpp:116-118 does.
+ *   - volumes are flat, x fastest: i = z*Ny*Nx + y*Nx + x (numpy C-order of (Nz,Ny,Nx));
+ *     every stencil wraps periodically.
+ *   - n = Nx*Ny*Nz must be < 2^31 (labels are int32 voxel indices, as in the reference).
+ *
+ * Two families of entry points:
+ *   - host-pointer calls (`pnm_*`): inputs in host memory, results copied back to host, as the
+ *     reference's `*_k` wrappers do;
+ *   - device-pointer calls (`pnm_*_d`): inputs and the per-voxel outputs stay in HBM (pointers
+ *     from hipMalloc/torch); only small lists come back. They run on the context's stream.
+ *
+ * Errors: every call returns a pnm_status; pnm_last_error() gives the message of the most recent
+ * failure on the calling thread. Arguments are validated before any device work, so a bad call
+ * fails the same way with or without a GPU.
+ *
+ * Threading: a context owns one HIP stream and one scratch arena; it is not re-entrant. Separate
+ * contexts may be used from separate threads.
+ */
+#ifndef PNM_CAPI_H
+#define PNM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pore record, layout-identical to `struct pnm::Pore {float x,y,z,radius;}`
+ * (pore_extraction.hpp:24-26). */
+typedef struct {
+  float x, y, z, radius;
+} pnm_pore_t;
+
+typedef enum {
+  PNM_OK = 0,
+  PNM_EINVAL = 1,      /* bad argument (shape, length, null pointer, n >= 2^31) */
+  PNM_ENOMEM = 2,      /* device or host allocation failed */
+  PNM_EHIP = 3,        /* HIP runtime error, or no GPU */
+  PNM_ECAPACITY = 4,   /* a caller-provided output buffer is too small (needed size reported) */
+  PNM_ENOCONVERGE = 5  /* a fixpoint loop exceeded its iteration guard */
+} pnm_status;
+
+typedef struct pnm_ctx_s* pnm_ctx_t;
+
+/* Maximum pores returned by pnm_extract_pores / pnm_extract_pore_network
+ * (max_pores, pore_extraction.hpp:77). The reference truncates silently to an arbitrary subset;
+ * this library keeps the lowest-index peaks and reports the untruncated count. */
+#define PNM_MAX_PORES 1000000
+
+/* ---- library / context ---- */
+const char* pnm_version(void);
+/* "HIP" — the value of the module attribute `execution_space` (pnm_bindings.cpp:84). */
+const char* pnm_execution_space(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* pnm_last_error(void);
+/* Number of visible HIP devices (0 when there is no GPU); never fails. */
+int pnm_device_count(void);
+
+/* Create a context on HIP device `device` (its own stream + arena). Replaces Kokkos::initialize
+ * (pnm_bindings.cpp:75-76). */
+pnm_status pnm_ctx_create(pnm_ctx_t* out, int device);
+void pnm_ctx_destroy(pnm_ctx_t ctx);
+/* Run this context's work on `stream` (a hipStream_t, e.g. torch's current stream); NULL
+ * restores the context's own stream. */
+pnm_status pnm_ctx_set_stream(pnm_ctx_t ctx, void* stream);
+void* pnm_ctx_stream(pnm_ctx_t ctx);
+/* Release the context's scratch arena (it regrows on the next call). */
+pnm_status pnm_ctx_trim(pnm_ctx_t ctx);
+/* Bytes currently held by the context's scratch arena. */
+int64_t pnm_ctx_arena_bytes(pnm_ctx_t ctx);
+
+/* Kernel timing (no reference counterpart; the reference labels kernels for Kokkos Tools).
+ * enable != 0: every kernel launch of this context is bracketed by two HIP events on the launch
+ * stream. pnm_ctx_profile_read aggregates the recorded launches per kernel name since the last
+ * read: names (newline-separated into `names`), total milliseconds and launch counts.
+ * *count = number of distinct kernels (PNM_ECAPACITY when it exceeds cap). */
+pnm_status pnm_ctx_profile(pnm_ctx_t ctx, int enable);
+pnm_status pnm_ctx_profile_read(pnm_ctx_t ctx, char* names, int64_t names_len, double* total_ms,
+                                int64_t* launches, int64_t cap, int64_t* count);
+
+/* Free a list returned by this library. */
+void pnm_free(void* p);
+
+/* ---- host-pointer entry points ---- */
+
+/* extract_pores_k (pore_extraction.hpp:143-151) -> extractPoresView (:69-140).
+ * Pores in ascending peak-voxel index; *pores is malloc'ed (pnm_free), *count <= PNM_MAX_PORES,
+ * *total = untruncated number of peaks (may be NULL). n == 0 gives an empty list. */
+pnm_status pnm_extract_pores(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                             const float origin[3], const float spacing[3], pnm_pore_t** pores,
+                             int64_t* count, int64_t* total);
+
+/* segment_volume_k (pore_extraction.hpp:389-397) -> segmentVolumeView (:157-386).
+ * seg_out: caller-allocated int32[n]. */
+pnm_status pnm_segment_volume(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                              const float spacing[3], int32_t* seg_out);
+
+/* extract_topology_k (pore_extraction.hpp:454-461) -> extractTopologyView (:401-451).
+ * *pairs: malloc'ed int32[2*count] of (min,max) pairs in signed lexicographic order. */
+pnm_status pnm_extract_topology(pnm_ctx_t ctx, const int32_t* seg, const int32_t dims[3],
+                                int32_t** pairs, int64_t* count);
+
+/* extract_pore_network_k (pore_extraction.hpp:1011-1025): pores, seg, connections from one
+ * upload of the SDF. seg_out: caller int32[n]; *pores and *pairs malloc'ed. */
+pnm_status pnm_extract_pore_network(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                                    const float origin[3], const float spacing[3],
+                                    pnm_pore_t** pores, int64_t* npores, int64_t* total_pores,
+                                    int32_t* seg_out, int32_t** pairs, int64_t* npairs);
+
+/* Result of pnm_extract_network_flow: struct NetworkFlow (pore_extraction.hpp:482-491).
+ * All arrays are malloc'ed by the library; release with pnm_flow_result_free. */
+typedef struct {
+  int64_t np;             /* pore labels 1..np */
+  int64_t nt;             /* throats (per connected face patch) */
+  pnm_pore_t* pores;      /* [np]  pores[k] <-> label k+1 */
+  double* pore_pressure;  /* [np]  trilinear p at the pore centre */
+  double* pore_residual;  /* [np]  signed net outflow over the pore's boundary */
+  int32_t* throats;       /* [2*nt] (li, lj), li < lj, sorted by (pair, patch root face) */
+  double* throat_flow;    /* [nt]  positive li -> lj */
+  double* throat_area;    /* [nt]  sum of openness * face area */
+  double* throat_dp;      /* [nt]  P_i - P_j including the macroscopic gradient */
+} pnm_flow_result_t;
+
+/* extract_network_flow_k (pore_extraction.hpp:493-997). u,v,w,p: float64[n] MAC fields
+ * (u(i,j,k) on the -x face of cell (i,j,k)); ox,oy,oz: all three or all NULL (fully open).
+ * grad_p is X,Y,Z. */
+pnm_status pnm_extract_network_flow(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                                    const float origin[3], const float spacing[3],
+                                    const double* u, const double* v, const double* w,
+                                    const double* p, const double* ox, const double* oy,
+                                    const double* oz, const double grad_p[3],
+                                    pnm_flow_result_t* out);
+void pnm_flow_result_free(pnm_flow_result_t* r);
+
+/* ---- device-pointer entry points (inputs and per-voxel outputs resident in HBM) ---- */
+
+/* extractPoresView (pore_extraction.hpp:69-140) on a device SDF. pores_d: device buffer of
+ * `cap` records; *count = number written (<= min(cap, PNM_MAX_PORES)), *total = peaks found. */
+pnm_status pnm_extract_pores_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                               const float origin[3], const float spacing[3],
+                               pnm_pore_t* pores_d, int64_t cap, int64_t* count,
+                               int64_t* total);
+
+/* segmentVolumeView (pore_extraction.hpp:157-386) on a device SDF; seg_d: device int32[n].
+ * roots_d (nullable): device int32[n] receiving the gradient-path roots of sdf>0 voxels
+ * (rootsOut, :383-384; other voxels get their own index). */
+pnm_status pnm_segment_volume_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                const float spacing[3], int32_t* seg_d, int32_t* roots_d);
+
+/* extractTopologyView (pore_extraction.hpp:401-451) on a device segmentation.
+ * pairs_d: device int32[2*cap]; *count = unique pairs. PNM_ECAPACITY if count > cap (nothing
+ * written, *count holds the needed size). */
+pnm_status pnm_extract_topology_d(pnm_ctx_t ctx, const int32_t* seg_d, const int32_t dims[3],
+                                  int32_t* pairs_d, int64_t cap, int64_t* count);
+
+/* The fused pipeline (pore_extraction.hpp:1011-1025) fully device-resident: pores_d
+ * (cap_pores records), seg_d (int32[n]) and pairs_d (int32[2*cap_pairs]) are device buffers.
+ * Per-buffer PNM_ECAPACITY as above. */
+pnm_status pnm_extract_pore_network_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                      const float origin[3], const float spacing[3],
+                                      pnm_pore_t* pores_d, int64_t cap_pores, int64_t* npores,
+                                      int64_t* total_pores, int32_t* seg_d, int32_t* pairs_d,
+                                      int64_t cap_pairs, int64_t* npairs);
+
+/* extract_network_flow_k on device-resident SDF and fields; results come back to host. */
+pnm_status pnm_extract_network_flow_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                      const float origin[3], const float spacing[3],
+                                      const double* u_d, const double* v_d, const double* w_d,
+                                      const double* p_d, const double* ox_d, const double* oy_d,
+                                      const double* oz_d, const double grad_p[3],
+                                      pnm_flow_result_t* out);
+
+/* ---- stage-level device entry points (parity tests of single kernels; no reference
+ * counterpart as separate calls: each is one step of segmentVolumeView) ---- */
+
+/* Steps 1-3 (pore_extraction.hpp:173-269): marker labels after CCL and flood. */
+pnm_status pnm_stage_labels_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                              const float spacing[3], int32_t* labels_d, int32_t* ccl_only_d);
+/* Ascent roots: all_voxels = 0 -> step 4 (:271-311) for sdf>0 voxels (others: own index);
+ * all_voxels = 1 -> nf::flow_basins walk (:621-652) from every voxel. */
+pnm_status pnm_stage_roots_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                             int all_voxels, int32_t* roots_d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PNM_CAPI_H */

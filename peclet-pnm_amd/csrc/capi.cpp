@@ -1,0 +1,644 @@
+// capi.cpp — the extern "C" boundary of libpnm_hip.so (declared in include/pnm_capi.h).
+//
+// Every entry point validates its arguments before touching the device, converts C++
+// exceptions into a pnm_status plus a thread-local message, and runs on the context's stream.
+#include <algorithm>
+#include <cstdlib>
+#include <new>
+#include <string>
+
+#include "pnm_internal.hpp"
+
+namespace pnm {
+
+void fail(pnm_status code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e == hipSuccess) return;
+  const pnm_status code =
+      (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PNM_ENOMEM : PNM_EHIP;
+  fail(code, std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +
+                 std::to_string(line) + ")");
+}
+
+void* Arena::get(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  if (bytes == 0) bytes = 256;
+  if (next_ < slots_.size()) {
+    Slot& s = slots_[next_];
+    if (s.size >= bytes) {
+      ++next_;
+      return s.p;
+    }
+    PNM_HIP(hipFree(s.p));
+    s.p = nullptr;
+    s.size = 0;
+    void* p = nullptr;
+    PNM_HIP(hipMalloc(&p, bytes));
+    s.p = p;
+    s.size = bytes;
+    ++next_;
+    return p;
+  }
+  void* p = nullptr;
+  PNM_HIP(hipMalloc(&p, bytes));
+  slots_.push_back({p, bytes});
+  ++next_;
+  return p;
+}
+
+void Arena::release() {
+  for (auto& s : slots_)
+    if (s.p) (void)hipFree(s.p);
+  slots_.clear();
+  next_ = 0;
+}
+
+int64_t Arena::bytes() const {
+  int64_t b = 0;
+  for (const auto& s : slots_) b += (int64_t)s.size;
+  return b;
+}
+
+hipEvent_t Ctx::take_event() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  PNM_HIP(hipEventCreate(&e));
+  return e;
+}
+
+KTimer::KTimer(Ctx* ctx, const char* nm) : c(ctx), name(nm) {
+  if (!c->prof) return;
+  a = c->take_event();
+  PNM_HIP(hipEventRecord(a, c->stream));
+}
+
+KTimer::~KTimer() {
+  if (!a) return;
+  hipEvent_t b = nullptr;
+  try {
+    b = c->take_event();
+  } catch (...) {
+    c->pool.push_back(a);
+    return;
+  }
+  if (hipEventRecord(b, c->stream) != hipSuccess) {
+    c->pool.push_back(a);
+    c->pool.push_back(b);
+    return;
+  }
+  c->recs.push_back({name, a, b});
+}
+
+CallScope::CallScope(Ctx* c) : ctx(c), prev_device(-1) {
+  if (hipGetDevice(&prev_device) != hipSuccess) prev_device = -1;
+  if (prev_device != c->device) PNM_HIP(hipSetDevice(c->device));
+  c->arena.reset();
+}
+
+CallScope::~CallScope() {
+  if (prev_device >= 0 && prev_device != ctx->device) (void)hipSetDevice(prev_device);
+}
+
+}  // namespace pnm
+
+using namespace pnm;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+pnm_status guarded(F&& f) {
+  try {
+    f();
+    return PNM_OK;
+  } catch (const Error& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return PNM_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PNM_EHIP;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) fail(PNM_EINVAL, msg);
+}
+
+Vol check_dims(const int32_t* dims) {
+  need(dims != nullptr, "dims must not be NULL");
+  need(dims[0] >= 0 && dims[1] >= 0 && dims[2] >= 0, "dims must be non-negative");
+  const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+  need(n < ((int64_t)1 << 31), "volume too large: Nx*Ny*Nz must be < 2^31 (int32 labels)");
+  return Vol{dims[0], dims[1], dims[2], n};
+}
+
+Ctx* check_ctx(pnm_ctx_t ctx) {
+  need(ctx != nullptr, "context is NULL (create one with pnm_ctx_create)");
+  return reinterpret_cast<Ctx*>(ctx);
+}
+
+template <class T>
+T* host_copy(const T* dev, int64_t count, hipStream_t s) {
+  T* h = static_cast<T*>(std::malloc(sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
+  if (!h) throw std::bad_alloc();
+  if (count > 0) {
+    const hipError_t e = hipMemcpyAsync(h, dev, sizeof(T) * count, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) {
+      PNM_HIP(hipStreamSynchronize(s));
+    } else {
+      std::free(h);
+      PNM_HIP(e);
+    }
+  }
+  return h;
+}
+
+void upload(void* dev, const void* host, size_t bytes, hipStream_t s) {
+  if (bytes) PNM_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pnm_version(void) { return "0.1.0"; }
+const char* pnm_execution_space(void) { return "HIP"; }
+const char* pnm_last_error(void) { return g_err.c_str(); }
+
+int pnm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+pnm_status pnm_ctx_create(pnm_ctx_t* out, int device) {
+  return guarded([&] {
+    need(out != nullptr, "out must not be NULL");
+    *out = nullptr;
+    const int nd = pnm_device_count();
+    if (nd <= 0) fail(PNM_EHIP, "no HIP device visible (libpnm_hip needs an MI355X GPU)");
+    need(device >= 0 && device < nd, "device index out of range");
+    Ctx* c = new Ctx();
+    c->device = device;
+    try {
+      int prev = -1;
+      (void)hipGetDevice(&prev);
+      PNM_HIP(hipSetDevice(device));
+      PNM_HIP(hipStreamCreate(&c->own_stream));
+      PNM_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->pinned),
+                            sizeof(uint64_t) * Ctx::kPinnedWords, hipHostMallocDefault));
+      if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
+    } catch (...) {
+      if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+      delete c;
+      throw;
+    }
+    c->stream = c->own_stream;
+    *out = reinterpret_cast<pnm_ctx_t>(c);
+  });
+}
+
+void pnm_ctx_destroy(pnm_ctx_t ctx) {
+  if (!ctx) return;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->arena.release();
+  for (auto& r : c->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (prev >= 0) (void)hipSetDevice(prev);
+  delete c;
+}
+
+pnm_status pnm_ctx_set_stream(pnm_ctx_t ctx, void* stream) {
+  return guarded([&] {
+    Ctx* c = check_ctx(ctx);
+    c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  });
+}
+
+void* pnm_ctx_stream(pnm_ctx_t ctx) {
+  return ctx ? static_cast<void*>(reinterpret_cast<Ctx*>(ctx)->stream) : nullptr;
+}
+
+pnm_status pnm_ctx_trim(pnm_ctx_t ctx) {
+  return guarded([&] {
+    Ctx* c = check_ctx(ctx);
+    CallScope cs(c);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    c->arena.release();
+  });
+}
+
+int64_t pnm_ctx_arena_bytes(pnm_ctx_t ctx) {
+  return ctx ? reinterpret_cast<Ctx*>(ctx)->arena.bytes() : 0;
+}
+
+void pnm_free(void* p) { std::free(p); }
+
+pnm_status pnm_ctx_profile(pnm_ctx_t ctx, int enable) {
+  return guarded([&] {
+    Ctx* c = check_ctx(ctx);
+    c->prof = enable != 0;
+  });
+}
+
+pnm_status pnm_ctx_profile_read(pnm_ctx_t ctx, char* names, int64_t names_len, double* total_ms,
+                                int64_t* launches, int64_t cap, int64_t* count) {
+  return guarded([&] {
+    Ctx* c = check_ctx(ctx);
+    need(count != nullptr, "count must not be NULL");
+    need(cap == 0 || (names && total_ms && launches), "output buffers required when cap > 0");
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    PNM_HIP(hipSetDevice(c->device));
+    std::vector<std::string> nm;
+    std::vector<double> ms;
+    std::vector<int64_t> cnt;
+    for (auto& r : c->recs) {
+      PNM_HIP(hipEventSynchronize(r.b));
+      float t = 0.0f;
+      PNM_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      size_t k = 0;
+      while (k < nm.size() && nm[k] != r.name) ++k;
+      if (k == nm.size()) {
+        nm.emplace_back(r.name);
+        ms.push_back(0.0);
+        cnt.push_back(0);
+      }
+      ms[k] += t;
+      cnt[k] += 1;
+      c->pool.push_back(r.a);
+      c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    if (prev >= 0) (void)hipSetDevice(prev);
+    *count = (int64_t)nm.size();
+    std::string joined;
+    for (size_t k = 0; k < nm.size(); ++k) {
+      if ((int64_t)k < cap) {
+        total_ms[k] = ms[k];
+        launches[k] = cnt[k];
+      }
+      joined += nm[k];
+      joined.push_back('\n');
+    }
+    if (names && names_len > 0) {
+      const size_t m = std::min<size_t>(joined.size(), (size_t)names_len - 1);
+      std::memcpy(names, joined.data(), m);
+      names[m] = 0;
+    }
+    if ((int64_t)nm.size() > cap) fail(PNM_ECAPACITY, "profile buffers too small");
+  });
+}
+
+// ------------------------- device-pointer entry points -------------------------
+
+pnm_status pnm_extract_pores_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                               const float origin[3], const float spacing[3],
+                               pnm_pore_t* pores_d, int64_t cap, int64_t* count,
+                               int64_t* total) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(origin && spacing && count, "origin, spacing and count must not be NULL");
+    need(v.n == 0 || sdf_d, "sdf must not be NULL");
+    need(cap == 0 || pores_d, "pores buffer must not be NULL when cap > 0");
+    Ctx* c = check_ctx(ctx);
+    *count = 0;
+    if (total) *total = 0;
+    if (v.n == 0) return;
+    CallScope cs(c);
+    int64_t tot = 0;
+    const int64_t k = pores_device(c, sdf_d, v, origin, spacing, pores_d, cap, &tot);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    if (total) *total = tot;
+    if (k < 0) {
+      *count = -k;
+      fail(PNM_ECAPACITY, "pores buffer too small: need " + std::to_string(-k) + " records");
+    }
+    *count = k;
+  });
+}
+
+pnm_status pnm_segment_volume_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                const float spacing[3], int32_t* seg_d, int32_t* roots_d) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(spacing != nullptr, "spacing must not be NULL");
+    need(v.n == 0 || (sdf_d && seg_d), "sdf and seg must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    if (v.n == 0) return;
+    CallScope cs(c);
+    segment_device(c, sdf_d, v, spacing, seg_d, roots_d);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+pnm_status pnm_extract_topology_d(pnm_ctx_t ctx, const int32_t* seg_d, const int32_t dims[3],
+                                  int32_t* pairs_d, int64_t cap, int64_t* count) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(count != nullptr, "count must not be NULL");
+    need(v.n == 0 || seg_d, "seg must not be NULL");
+    need(cap == 0 || pairs_d, "pairs buffer must not be NULL when cap > 0");
+    Ctx* c = check_ctx(ctx);
+    *count = 0;
+    if (v.n == 0) return;
+    CallScope cs(c);
+    int64_t needed = 0;
+    const int64_t k = topology(c, seg_d, v, 0, 0, false, pairs_d, cap, &needed);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    *count = needed;
+    if (k < 0)
+      fail(PNM_ECAPACITY, "pairs buffer too small: need " + std::to_string(needed) + " pairs");
+  });
+}
+
+pnm_status pnm_extract_pore_network_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                      const float origin[3], const float spacing[3],
+                                      pnm_pore_t* pores_d, int64_t cap_pores, int64_t* npores,
+                                      int64_t* total_pores, int32_t* seg_d, int32_t* pairs_d,
+                                      int64_t cap_pairs, int64_t* npairs) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(origin && spacing && npores && npairs, "origin, spacing, npores, npairs required");
+    need(v.n == 0 || (sdf_d && seg_d), "sdf and seg must not be NULL");
+    need(cap_pores == 0 || pores_d, "pores buffer must not be NULL when cap_pores > 0");
+    need(cap_pairs == 0 || pairs_d, "pairs buffer must not be NULL when cap_pairs > 0");
+    Ctx* c = check_ctx(ctx);
+    *npores = 0;
+    *npairs = 0;
+    if (total_pores) *total_pores = 0;
+    if (v.n == 0) return;
+    CallScope cs(c);
+    int64_t tot = 0, np2 = 0;
+    bool povf = false;
+    const int64_t k = network_device(c, sdf_d, v, origin, spacing, pores_d, cap_pores, &tot,
+                                     seg_d, pairs_d, cap_pairs, &np2, &povf);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    if (total_pores) *total_pores = tot;
+    *npores = k < 0 ? -k : k;
+    *npairs = np2;
+    if (k < 0) fail(PNM_ECAPACITY, "pores buffer too small: need " + std::to_string(-k));
+    if (povf) fail(PNM_ECAPACITY, "pairs buffer too small: need " + std::to_string(np2));
+  });
+}
+
+pnm_status pnm_stage_labels_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                              const float spacing[3], int32_t* labels_d, int32_t* ccl_only_d) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(spacing != nullptr, "spacing must not be NULL");
+    need(v.n == 0 || (sdf_d && labels_d), "sdf and labels must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    if (v.n == 0) return;
+    CallScope cs(c);
+    const float mn = std::min(spacing[0], std::min(spacing[1], spacing[2]));
+    int32_t* next = c->arena.get_n<int32_t>((size_t)v.n);
+    int32_t* min_pore = c->arena.get_n<int32_t>((size_t)v.n);
+    int32_t* min_solid = c->arena.get_n<int32_t>((size_t)v.n);
+    uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)v.n);
+    uint32_t* band = c->arena.get_n<uint32_t>((size_t)v.n);
+    uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)v.n);
+    int32_t* newlab = c->arena.get_n<int32_t>((size_t)v.n);
+    uint32_t* skeys = c->arena.get_n<uint32_t>((size_t)v.n);
+    uint64_t* ctr = c->arena.get_n<uint64_t>(8);
+    fill_u64(c, ctr, 0, 8);
+    PreOut o{};
+    o.labels = labels_d;
+    o.next = next;
+    o.min_pore = min_pore;
+    o.peak_list = peaks;
+    o.band_list = band;
+    o.counters = ctr;
+    o.cap_peaks = v.n;
+    launch_pre(c, sdf_d, v, -1.5f * mn, o);
+    launch_ccl(c, v, labels_d, min_solid, skeys, ctr);
+    if (ccl_only_d)
+      PNM_HIP(hipMemcpyAsync(ccl_only_d, labels_d, sizeof(int32_t) * v.n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    uint64_t h[2];
+    readback(c, ctr, 2, h);
+    uint32_t* cur = band;
+    uint32_t* nxt = band2;
+    int64_t nb = (int64_t)h[1];
+    while (nb > 0) {
+      const int64_t nb2 = flood_sweep(c, v, labels_d, cur, nb, nxt, newlab, ctr + 5);
+      if (nb2 == nb) break;
+      std::swap(cur, nxt);
+      nb = nb2;
+    }
+    PNM_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+pnm_status pnm_stage_roots_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                             int all_voxels, int32_t* roots_d) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(v.n == 0 || (sdf_d && roots_d), "sdf and roots must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    if (v.n == 0) return;
+    CallScope cs(c);
+    int32_t* next = c->arena.get_n<int32_t>((size_t)v.n);
+    launch_next(c, sdf_d, v, next, all_voxels != 0);
+    launch_chase(c, sdf_d, v, next, roots_d, all_voxels != 0);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+pnm_status pnm_extract_network_flow_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                      const float origin[3], const float spacing[3],
+                                      const double* u_d, const double* v_d, const double* w_d,
+                                      const double* p_d, const double* ox_d, const double* oy_d,
+                                      const double* oz_d, const double grad_p[3],
+                                      pnm_flow_result_t* out) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(origin && spacing && grad_p && out, "origin, spacing, grad_p and out are required");
+    need(v.n == 0 || (sdf_d && u_d && v_d && w_d && p_d), "sdf, u, v, w, p must not be NULL");
+    need((ox_d == nullptr) == (oy_d == nullptr) && (ox_d == nullptr) == (oz_d == nullptr),
+         "pass all three openness arrays (ox,oy,oz) or none");
+    Ctx* c = check_ctx(ctx);
+    std::memset(out, 0, sizeof(*out));
+    if (v.n == 0) return;
+    CallScope cs(c);
+    flow_device(c, sdf_d, v, origin, spacing, u_d, v_d, w_d, p_d, ox_d, oy_d, oz_d, grad_p, out);
+  });
+}
+
+// ------------------------- host-pointer entry points -------------------------
+
+pnm_status pnm_extract_pores(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                             const float origin[3], const float spacing[3], pnm_pore_t** pores,
+                             int64_t* count, int64_t* total) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(origin && spacing && pores && count, "origin, spacing, pores and count required");
+    need(v.n == 0 || sdf, "sdf must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    *pores = nullptr;
+    *count = 0;
+    if (total) *total = 0;
+    if (v.n == 0) {
+      *pores = static_cast<pnm_pore_t*>(std::malloc(sizeof(pnm_pore_t)));
+      return;
+    }
+    CallScope cs(c);
+    float* sdf_d = c->arena.get_n<float>((size_t)v.n);
+    pnm_pore_t* pd = c->arena.get_n<pnm_pore_t>(PNM_MAX_PORES);
+    upload(sdf_d, sdf, sizeof(float) * v.n, c->stream);
+    int64_t tot = 0;
+    const int64_t k = pores_device(c, sdf_d, v, origin, spacing, pd, PNM_MAX_PORES, &tot);
+    *pores = host_copy(pd, k, c->stream);
+    *count = k;
+    if (total) *total = tot;
+  });
+}
+
+pnm_status pnm_segment_volume(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                              const float spacing[3], int32_t* seg_out) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(spacing != nullptr, "spacing must not be NULL");
+    need(v.n == 0 || (sdf && seg_out), "sdf and seg must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    if (v.n == 0) return;
+    CallScope cs(c);
+    float* sdf_d = c->arena.get_n<float>((size_t)v.n);
+    int32_t* seg_d = c->arena.get_n<int32_t>((size_t)v.n);
+    upload(sdf_d, sdf, sizeof(float) * v.n, c->stream);
+    segment_device(c, sdf_d, v, spacing, seg_d, nullptr);
+    PNM_HIP(hipMemcpyAsync(seg_out, seg_d, sizeof(int32_t) * v.n, hipMemcpyDeviceToHost,
+                           c->stream));
+    PNM_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+pnm_status pnm_extract_topology(pnm_ctx_t ctx, const int32_t* seg, const int32_t dims[3],
+                                int32_t** pairs, int64_t* count) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(pairs && count, "pairs and count required");
+    need(v.n == 0 || seg, "seg must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    *pairs = nullptr;
+    *count = 0;
+    if (v.n == 0) {
+      *pairs = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * 2));
+      return;
+    }
+    CallScope cs(c);
+    int32_t* seg_d = c->arena.get_n<int32_t>((size_t)v.n);
+    int32_t* pd = c->arena.get_n<int32_t>((size_t)(6 * v.n));
+    upload(seg_d, seg, sizeof(int32_t) * v.n, c->stream);
+    int64_t needed = 0;
+    const int64_t k = topology(c, seg_d, v, 0, 0, false, pd, 3 * v.n, &needed);
+    *pairs = host_copy(pd, 2 * k, c->stream);
+    *count = k;
+  });
+}
+
+pnm_status pnm_extract_pore_network(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                                    const float origin[3], const float spacing[3],
+                                    pnm_pore_t** pores, int64_t* npores, int64_t* total_pores,
+                                    int32_t* seg_out, int32_t** pairs, int64_t* npairs) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(origin && spacing && pores && npores && pairs && npairs, "output pointers required");
+    need(v.n == 0 || (sdf && seg_out), "sdf and seg must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    *pores = nullptr;
+    *pairs = nullptr;
+    *npores = 0;
+    *npairs = 0;
+    if (total_pores) *total_pores = 0;
+    if (v.n == 0) {
+      *pores = static_cast<pnm_pore_t*>(std::malloc(sizeof(pnm_pore_t)));
+      *pairs = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * 2));
+      return;
+    }
+    CallScope cs(c);
+    float* sdf_d = c->arena.get_n<float>((size_t)v.n);
+    int32_t* seg_d = c->arena.get_n<int32_t>((size_t)v.n);
+    pnm_pore_t* pd = c->arena.get_n<pnm_pore_t>(PNM_MAX_PORES);
+    int32_t* prd = c->arena.get_n<int32_t>((size_t)(6 * v.n));
+    upload(sdf_d, sdf, sizeof(float) * v.n, c->stream);
+    int64_t tot = 0, np2 = 0;
+    bool povf = false;
+    const int64_t k = network_device(c, sdf_d, v, origin, spacing, pd, PNM_MAX_PORES, &tot,
+                                     seg_d, prd, 3 * v.n, &np2, &povf);
+    PNM_HIP(hipMemcpyAsync(seg_out, seg_d, sizeof(int32_t) * v.n, hipMemcpyDeviceToHost,
+                           c->stream));
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    *pores = host_copy(pd, k, c->stream);
+    *npores = k;
+    if (total_pores) *total_pores = tot;
+    *pairs = host_copy(prd, 2 * np2, c->stream);
+    *npairs = np2;
+  });
+}
+
+pnm_status pnm_extract_network_flow(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
+                                    const float origin[3], const float spacing[3],
+                                    const double* u, const double* v, const double* w,
+                                    const double* p, const double* ox, const double* oy,
+                                    const double* oz, const double grad_p[3],
+                                    pnm_flow_result_t* out) {
+  return guarded([&] {
+    const Vol vol = check_dims(dims);
+    need(origin && spacing && grad_p && out, "origin, spacing, grad_p and out are required");
+    need(vol.n == 0 || (sdf && u && v && w && p), "sdf, u, v, w, p must not be NULL");
+    need((ox == nullptr) == (oy == nullptr) && (ox == nullptr) == (oz == nullptr),
+         "pass all three openness arrays (ox,oy,oz) or none");
+    Ctx* c = check_ctx(ctx);
+    std::memset(out, 0, sizeof(*out));
+    if (vol.n == 0) return;
+    CallScope cs(c);
+    const size_t nd = (size_t)vol.n;
+    float* sdf_d = c->arena.get_n<float>(nd);
+    double* f[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const double* hf[7] = {u, v, w, p, ox, oy, oz};
+    upload(sdf_d, sdf, sizeof(float) * nd, c->stream);
+    for (int k = 0; k < 7; ++k)
+      if (hf[k]) {
+        f[k] = c->arena.get_n<double>(nd);
+        upload(f[k], hf[k], sizeof(double) * nd, c->stream);
+      }
+    flow_device(c, sdf_d, vol, origin, spacing, f[0], f[1], f[2], f[3], f[4], f[5], f[6], grad_p,
+                out);
+  });
+}
+
+void pnm_flow_result_free(pnm_flow_result_t* r) {
+  if (!r) return;
+  std::free(r->pores);
+  std::free(r->pore_pressure);
+  std::free(r->pore_residual);
+  std::free(r->throats);
+  std::free(r->throat_flow);
+  std::free(r->throat_area);
+  std::free(r->throat_dp);
+  std::memset(r, 0, sizeof(*r));
+}
+
+}  // extern "C"

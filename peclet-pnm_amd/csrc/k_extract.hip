@@ -1,0 +1,185 @@
+// k_extract.hip — one streaming pass over the SDF: 26-neighbour peak test (extract_pores
+// predicate), ascent successor, watershed markers and the flood band; plus the FP32 centroid
+// refinement of the peaks.
+//
+// FP32 exactness: contraction is off for this file, so `px += dx*w` and `o + (i+f)*s` round
+// exactly as the reference's written expressions (pore_extraction.hpp:107-127).
+#pragma clang fp contract(off)
+
+#include "k_common.hpp"
+
+namespace pnm {
+
+// Voxel-kernel launch shape: a wave is one 64-voxel x-row; 4 rows per block; grid.z = z-slice.
+static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz); }
+static const dim3 kVoxBlock(64, 4, 1);
+
+// Lexicographic (sdf, flat index) maximum over the voxel and its 26 periodic neighbours — the
+// ascent step of pore_extraction.hpp:280-304 (ties to the larger index; NaN never wins).
+__device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x, int y, int z,
+                                           int nx, int ny, int nz, int ci, float cv) {
+  const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
+  const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
+  const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
+  int best = ci;
+  float bv = cv;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int row = (zs[a] * ny + ys[b]) * nx;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (a == 1 && b == 1 && c == 1) continue;
+        const int ni = row + xs[c];
+        const float nv = sdf[ni];
+        if (nv > bv) {
+          bv = nv;
+          best = ni;
+        } else if (nv == bv && ni > best) {
+          best = ni;
+        }
+      }
+    }
+  return best;
+}
+
+// SEG = false: peaks only (extract_pores). SEG = true: also markers, next, band, min_pore init.
+template <bool SEG>
+__global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int nx, int ny,
+                                             int nz, float thr, PreOut o) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  const int lane = threadIdx.x;
+  const bool in = (x < nx) && (y < ny);
+  bool is_peak = false, pos_peak = false, band = false;
+  int i = 0;
+  if (in) {
+    i = (z * ny + y) * nx + x;
+    const float cv = sdf[i];
+    int nxt = i;
+    if (!(cv <= 0.0f)) {  // extract_pores: `if (cv <= 0.0f) return;` (:90)
+      const int best = ascent_best(sdf, x, y, z, nx, ny, nz, i, cv);
+      is_peak = (best == i);
+      if (cv > 0.0f) nxt = best;
+      pos_peak = is_peak && (cv > 0.0f);
+    }
+    if (SEG) {
+      const bool mk = cv < thr;  // init_markers (:177)
+      o.labels[i] = mk ? i : -1;
+      o.next[i] = nxt;
+      band = !(cv >= 0.0f) && !mk;  // flood candidates (:245-248)
+      if (pos_peak) o.min_pore[i] = INT_MAX;
+    }
+  }
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(o.counters);
+  const uint64_t slot = wave_append(&ctr[0], is_peak, lane);
+  if (is_peak && (int64_t)slot < o.cap_peaks) o.peak_list[slot] = (uint32_t)i;
+  if (SEG) {
+    wave_append(&ctr[4], pos_peak, lane);
+    const uint64_t bs = wave_append(&ctr[1], band, lane);
+    if (band) o.band_list[bs] = (uint32_t)i;
+  }
+}
+
+void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o) {
+  if (o.labels)
+    PNM_LAUNCH(c, "k_pre", k_pre<true>, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz,
+                       thr, o);
+  else
+    PNM_LAUNCH(c, "k_pre_peaks", k_pre<false>, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz,
+               thr, o);
+}
+
+// Squared-positive-SDF weighted centroid, dz -> dy -> dx order, FP32
+// (pore_extraction.hpp:107-127, identical block at :543-562).
+__global__ void k_centroids(const float* __restrict__ sdf, int nx, int ny, int nz, float ox,
+                            float oy, float oz, float sx, float sy, float sz,
+                            const uint32_t* __restrict__ peaks, int64_t count,
+                            pnm_pore_t* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int ci = (int)peaks[k];
+  const int ix = ci % nx, iy = (ci / nx) % ny, iz = ci / (nx * ny);
+  const int xs[3] = {wm1(ix, nx), ix, wp1(ix, nx)};
+  const int ys[3] = {wm1(iy, ny), iy, wp1(iy, ny)};
+  const int zs[3] = {wm1(iz, nz), iz, wp1(iz, nz)};
+  float sw = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int cc = 0; cc < 3; ++cc) {
+        const float val = sdf[(zs[a] * ny + ys[b]) * nx + xs[cc]];
+        float w = val > 0.0f ? val : 0.0f;
+        w = w * w;
+        sw += w;
+        px += (float)(cc - 1) * w;
+        py += (float)(b - 1) * w;
+        pz += (float)(a - 1) * w;
+      }
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (sw > 1e-6f) {
+    fx = px / sw;
+    fy = py / sw;
+    fz = pz / sw;
+  }
+  pnm_pore_t P;
+  P.x = ox + ((float)ix + fx) * sx;
+  P.y = oy + ((float)iy + fy) * sy;
+  P.z = oz + ((float)iz + fz) * sz;
+  P.radius = sdf[ci];
+  out[k] = P;
+}
+
+void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
+                      const uint32_t* peaks, int64_t count, pnm_pore_t* out) {
+  if (count <= 0) return;
+  PNM_LAUNCH(c, "k_centroids", k_centroids, dim3(cdiv(count, 256)), dim3(256), 0, sdf, v.nx,
+                     v.ny, v.nz, org[0], org[1], org[2], spc[0], spc[1], spc[2], peaks, count,
+                     out);
+}
+
+// Ascent successor of every voxel (all_voxels) or of sdf > 0 voxels (others: themselves).
+__global__ __launch_bounds__(256) void k_next(const float* __restrict__ sdf, int nx, int ny,
+                                              int nz, int32_t* __restrict__ next,
+                                              bool all_voxels) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  const int i = (z * ny + y) * nx + x;
+  const float cv = sdf[i];
+  int nxt = i;
+  if (all_voxels || cv > 0.0f) nxt = ascent_best(sdf, x, y, z, nx, ny, nz, i, cv);
+  next[i] = nxt;
+}
+
+void launch_next(Ctx* c, const float* sdf, Vol v, int32_t* next, bool all_voxels) {
+  PNM_LAUNCH(c, "k_next", k_next, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz, next,
+                     all_voxels);
+}
+
+// Root = the walker after min(depth, 512) successor steps (pore_extraction.hpp:279-309).
+__global__ __launch_bounds__(256) void k_chase(const float* __restrict__ sdf, int64_t n,
+                                               const int32_t* __restrict__ next,
+                                               int32_t* __restrict__ roots, bool all_voxels) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int w = (int)i;
+  if (all_voxels || sdf[i] > 0.0f) {
+    for (int s = 0; s < kMaxSteps; ++s) {
+      const int nx = next[w];
+      if (nx == w) break;
+      w = nx;
+    }
+  }
+  roots[i] = w;
+}
+
+void launch_chase(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
+                  bool all_voxels) {
+  PNM_LAUNCH(c, "k_chase", k_chase, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, next,
+                     roots, all_voxels);
+}
+
+}  // namespace pnm

@@ -1,0 +1,355 @@
+// k_segment.hip — marker-controlled watershed on gfx950 (segmentVolumeView,
+// pore_extraction.hpp:157-386), reformulated around the order-free fixpoints:
+//   CCL      : label = minimum flat index of the periodic 26-connected marker component;
+//              lock-free union-find with atomicMin hooking (larger root under smaller).
+//   flood    : Jacobi sweeps restricted to a compacted band list (the only voxels a sweep can
+//              change), double-buffered through a per-band-entry label array.
+//   roots    : chase the ascent successor array up to 512 steps (exactly the capped walk).
+//   renumber : per-key first-appearance minimum by atomicMin, then a rank by radix sort of the
+//              (few) keys instead of two full-volume scans.
+#include "k_common.hpp"
+
+namespace pnm {
+
+static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz); }
+static const dim3 kVoxBlock(64, 4, 1);
+
+// ---------------- CCL ----------------
+__device__ __forceinline__ int uf_find(const int* L, int x) {
+  int p = ld_agent(L + x);
+  while (p != x) {
+    x = p;
+    p = ld_agent(L + x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_unite(int* L, int a, int b) {
+  while (true) {
+    a = uf_find(L, a);
+    b = uf_find(L, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(L + b, a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+// merge_markers (pore_extraction.hpp:198-224): 13 forward periodic neighbours (:204-206).
+__global__ __launch_bounds__(256) void k_ccl_union(int* L, int nx, int ny, int nz) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  const int i = (z * ny + y) * nx + x;
+  if (L[i] == -1) return;
+  const int xm = wm1(x, nx), xp = wp1(x, nx);
+  const int yp = wp1(y, ny), ym = wm1(y, ny);
+  const int zp = wp1(z, nz);
+  const int xs[3] = {xm, x, xp};
+  // (dz=1: dy=-1,0,1 x dx=-1,0,1), (dz=0, dy=1: dx=-1,0,1), (dz=0, dy=0, dx=1)
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int yy = b == 0 ? ym : (b == 1 ? y : yp);
+    const int row = (zp * ny + yy) * nx;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ni = row + xs[c];
+      if (L[ni] != -1) uf_unite(L, i, ni);
+    }
+  }
+  {
+    const int row = (z * ny + yp) * nx;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ni = row + xs[c];
+      if (L[ni] != -1) uf_unite(L, i, ni);
+    }
+  }
+  {
+    const int ni = (z * ny + y) * nx + xp;
+    if (L[ni] != -1) uf_unite(L, i, ni);
+  }
+}
+
+// flatten (pore_extraction.hpp:183-194). Component roots (= component minima) are appended to
+// the solid key list and get min_solid[root] = INT_MAX for the renumbering.
+__global__ __launch_bounds__(256) void k_ccl_flatten(int* L, int64_t n, int32_t* min_solid,
+                                                     uint32_t* keys,
+                                                     unsigned long long* counter) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool root = false;
+  if (i < n) {
+    int l = L[i];
+    if (l != -1) {
+      int p = L[l];
+      while (p != l) {
+        l = p;
+        p = L[l];
+      }
+      L[i] = l;
+      root = (l == (int)i);
+      if (root) min_solid[i] = INT_MAX;
+    }
+  }
+  const uint64_t s = wave_append(counter, root, lane);
+  if (root) keys[s] = (uint32_t)i;
+}
+
+void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_list,
+                uint64_t* counters) {
+  PNM_LAUNCH(c, "k_ccl_union", k_ccl_union, vox_grid(v), kVoxBlock, 0, labels, v.nx, v.ny,
+                     v.nz);
+  PNM_LAUNCH(c, "k_ccl_flatten", k_ccl_flatten, dim3(cdiv(v.n, 256)), dim3(256), 0, labels, v.n,
+                     min_solid, key_list, reinterpret_cast<unsigned long long*>(counters + 2));
+}
+
+// ---------------- flood (pore_extraction.hpp:237-269) ----------------
+__global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__ L, int nx,
+                                                     int ny, int nz,
+                                                     const uint32_t* __restrict__ band,
+                                                     int64_t nband,
+                                                     uint32_t* __restrict__ band_next,
+                                                     int32_t* __restrict__ newlab,
+                                                     unsigned long long* counter) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool keep = false;
+  uint32_t i = 0;
+  if (j < nband) {
+    i = band[j];
+    const int x = (int)(i % (uint32_t)nx);
+    const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
+    const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+    const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
+    const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
+    const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
+    int best = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const int row = (zs[a] * ny + ys[b]) * nx;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          if (a == 1 && b == 1 && cc == 1) continue;
+          const int nl = L[row + xs[cc]];
+          if (nl != -1 && (best == -1 || nl < best)) best = nl;
+        }
+      }
+    newlab[j] = best;
+    keep = (best == -1);
+  }
+  const uint64_t s = wave_append(counter, keep, lane);
+  if (keep) band_next[s] = i;
+}
+
+__global__ void k_flood_apply(int32_t* __restrict__ L, const uint32_t* __restrict__ band,
+                              const int32_t* __restrict__ newlab, int64_t nband) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nband) return;
+  const int32_t b = newlab[j];
+  if (b != -1) L[band[j]] = b;
+}
+
+int64_t flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, int64_t nband,
+                    uint32_t* band_next, int32_t* newlab, uint64_t* counter) {
+  fill_u64(c, counter, 0, 1);
+  PNM_LAUNCH(c, "k_flood_sweep", k_flood_sweep, dim3(cdiv(nband, 256)), dim3(256), 0, labels,
+                     v.nx, v.ny, v.nz, band, nband, band_next, newlab,
+                     reinterpret_cast<unsigned long long*>(counter));
+  PNM_LAUNCH(c, "k_flood_apply", k_flood_apply, dim3(cdiv(nband, 256)), dim3(256), 0, labels, band,
+                     newlab, nband);
+  uint64_t h;
+  readback(c, counter, 1, &h);
+  return (int64_t)h;
+}
+
+// ---------------- roots + first appearance ----------------
+// relabel_min (pore_extraction.hpp:331-337) fused with the root walk. Lanes are consecutive
+// voxels, so a lane whose key equals its left neighbour's key skips its atomicMin (the left lane
+// holds a smaller index for the same key).
+__global__ __launch_bounds__(256) void k_roots_min(const float* __restrict__ sdf, int64_t n,
+                                                   const int32_t* __restrict__ next,
+                                                   const int32_t* __restrict__ labels,
+                                                   int32_t* __restrict__ roots,
+                                                   int32_t* __restrict__ min_pore,
+                                                   int32_t* __restrict__ min_solid,
+                                                   unsigned long long* flags, bool do_min) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int pk = -1, sk = -1;
+  if (i < n) {
+    const float cv = sdf[i];
+    if (cv > 0.0f) {
+      int w = (int)i;
+      int s = 0;
+      for (; s < kMaxSteps; ++s) {
+        const int nx = next[w];
+        if (nx == w) break;
+        w = nx;
+      }
+      roots[i] = w;
+      if (s == kMaxSteps && next[w] != w) atomicOr(flags, 1ull);
+      pk = w;
+    } else {
+      roots[i] = (int)i;
+      sk = labels[i];
+    }
+  }
+  if (!do_min) return;
+  const int ppk = __shfl_up(pk, 1, 64);
+  const int psk = __shfl_up(sk, 1, 64);
+  if (pk >= 0 && (lane == 0 || ppk != pk)) atomicMin(min_pore + pk, (int)i);
+  if (sk >= 0 && (lane == 0 || psk != sk)) atomicMin(min_solid + sk, (int)i);
+}
+
+void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next,
+                      const int32_t* labels, int32_t* roots, int32_t* min_pore,
+                      int32_t* min_solid, uint64_t* counters, bool do_min) {
+  PNM_LAUNCH(c, "k_roots_min", k_roots_min, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, next,
+                     labels, roots, min_pore, min_solid,
+                     reinterpret_cast<unsigned long long*>(counters + 3), do_min);
+}
+
+// Capped-walk fallback: the root set is no longer the peak set, so redo the pore minima from a
+// fully initialised array and collect the keys by a full scan.
+__global__ void k_min_pore_full(const float* __restrict__ sdf, int64_t n,
+                                const int32_t* __restrict__ roots, int32_t* min_pore) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && sdf[i] > 0.0f) atomicMin(min_pore + roots[i], (int)i);
+}
+
+void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots,
+                     const int32_t* labels, int32_t* min_pore, int32_t* min_solid) {
+  (void)labels;
+  (void)min_solid;
+  fill_u32(c, reinterpret_cast<uint32_t*>(min_pore), (uint32_t)INT_MAX, v.n);
+  PNM_LAUNCH(c, "k_min_pore_full", k_min_pore_full, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
+                     roots, min_pore);
+}
+
+__global__ void k_collect_keys(const float* __restrict__ sdf, int64_t n,
+                               const int32_t* __restrict__ min_pore, uint32_t* keys,
+                               unsigned long long* counter) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool key = (i < n) && sdf[i] > 0.0f && min_pore[i] != INT_MAX;
+  const uint64_t s = wave_append(counter, key, lane);
+  if (key) keys[s] = (uint32_t)i;
+}
+
+int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_pore,
+                          uint32_t* key_list, uint64_t* counter) {
+  fill_u64(c, counter, 0, 1);
+  PNM_LAUNCH(c, "k_collect_keys", k_collect_keys, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
+                     min_pore, key_list, reinterpret_cast<unsigned long long*>(counter));
+  uint64_t h;
+  readback(c, counter, 1, &h);
+  return (int64_t)h;
+}
+
+// ---------------- ids (relabel_first/scan/assign, pore_extraction.hpp:340-374) ----------------
+// sortkey = the key's first-appearance voxel; pore keys that are not sdf>0 (NaN peaks of the
+// extract_pores predicate) sort last with 0xFFFFFFFF and get no id.
+__global__ void k_gather_first(const float* __restrict__ sdf, const uint32_t* __restrict__ keys,
+                               int64_t nkeys, const int32_t* __restrict__ mins, bool solid,
+                               uint32_t* __restrict__ sk, uint32_t* __restrict__ sv) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nkeys) return;
+  const uint32_t r = keys[j];
+  const bool valid = solid || sdf[r] > 0.0f;
+  sk[j] = valid ? (uint32_t)mins[r] : 0xFFFFFFFFu;
+  sv[j] = r;
+}
+
+// Solid ranks only count labels whose first voxel has sdf <= 0 (the relabel_first predicate,
+// :343-345; a NaN first voxel leaves the label's id at 0).
+__global__ void k_solid_flags(const float* __restrict__ sdf, const uint32_t* __restrict__ sk,
+                              int64_t nkeys, uint32_t* __restrict__ flag) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nkeys) return;
+  flag[j] = (sdf[sk[j]] <= 0.0f) ? 1u : 0u;
+}
+
+__global__ void k_assign(const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv,
+                         const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank,
+                         int64_t nkeys, bool solid, int32_t* __restrict__ ids,
+                         unsigned long long* __restrict__ count) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nkeys) return;
+  const uint32_t r = sv[j];
+  if (solid) {
+    const bool v = flag[j] != 0;
+    ids[r] = v ? -(int32_t)(rank[j] + 1) : 0;
+    if (j == nkeys - 1) *count = rank[j] + (v ? 1u : 0u);
+  } else {
+    if (sk[j] != 0xFFFFFFFFu) ids[r] = (int32_t)(j + 1);
+    if (j == nkeys - 1) *count = 0;
+  }
+}
+
+static int bits_for(uint64_t maxval) {
+  int b = 0;
+  while (b < 64 && (maxval >> b) != 0) ++b;
+  return b;
+}
+
+int64_t assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys,
+                   int32_t* id_arr, bool solid) {
+  if (nkeys <= 0) return 0;
+  uint32_t* sk = c->arena.get_n<uint32_t>((size_t)nkeys);
+  uint32_t* sv = c->arena.get_n<uint32_t>((size_t)nkeys);
+  uint32_t* flag = c->arena.get_n<uint32_t>((size_t)nkeys);
+  uint32_t* rank = c->arena.get_n<uint32_t>((size_t)nkeys);
+  uint64_t* cnt = c->arena.get_n<uint64_t>(1);
+  const unsigned nb = cdiv(nkeys, 256);
+  PNM_LAUNCH(c, "k_gather_first", k_gather_first, dim3(nb), dim3(256), 0, sdf, keys, nkeys,
+                     (const int32_t*)id_arr, solid, sk, sv);
+  radix_sort_u32(c, sk, sv, nkeys, 0, 32);
+  if (solid) {
+    PNM_LAUNCH(c, "k_solid_flags", k_solid_flags, dim3(nb), dim3(256), 0, sdf, sk, nkeys, flag);
+    scan_exclusive_u32(c, flag, rank, nkeys);
+  }
+  PNM_LAUNCH(c, "k_assign", k_assign, dim3(nb), dim3(256), 0, sk, sv, flag, rank, nkeys,
+                     solid, id_arr, reinterpret_cast<unsigned long long*>(cnt));
+  if (!solid) return -1;  // caller knows the pore count
+  uint64_t h;
+  readback(c, cnt, 1, &h);
+  (void)bits_for;
+  return (int64_t)h;
+}
+
+// relabel_seg (pore_extraction.hpp:378-381).
+__global__ __launch_bounds__(256) void k_seg(const float* __restrict__ sdf, int64_t n,
+                                             const int32_t* __restrict__ roots,
+                                             const int32_t* __restrict__ labels,
+                                             const int32_t* __restrict__ pore_id,
+                                             const int32_t* __restrict__ solid_id,
+                                             int32_t* __restrict__ seg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float cv = sdf[i];
+  int32_t s;
+  if (cv > 0.0f) {
+    s = pore_id[roots[i]];
+  } else {
+    const int32_t l = labels[i];
+    s = (l == -1) ? 0 : solid_id[l];
+  }
+  seg[i] = s;
+}
+
+void launch_seg(Ctx* c, const float* sdf, Vol v, const int32_t* roots, const int32_t* labels,
+                const int32_t* pore_id, const int32_t* solid_id, int32_t* seg) {
+  PNM_LAUNCH(c, "k_seg", k_seg, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, roots,
+                     labels, pore_id, solid_id, seg);
+}
+
+}  // namespace pnm

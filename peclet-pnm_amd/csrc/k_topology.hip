@@ -1,0 +1,128 @@
+// k_topology.hip — boundary-pair topology (extractTopologyView, pore_extraction.hpp:401-451).
+//
+// The reference appends one (min,max) pair per differing +x/+y/+z face (tens of millions at
+// 512^3) and sorts them on the host. Here each block deduplicates the pairs of a 64x4x4 brick in
+// an LDS hash set and appends only its distinct keys; the short candidate list is then radix
+// sorted and made unique on the device. The result is the same sorted unique list.
+//
+// Key encoding: key = (uint64)(l1 - lo) << bits | (l2 - lo), with lo the smallest possible
+// label, so unsigned key order is signed lexicographic (l1, l2) order.
+#include "k_common.hpp"
+
+namespace pnm {
+
+constexpr int TOPO_ZB = 4;          // z-slices per block
+constexpr int TOPO_TABLE = 2048;    // LDS hash slots (16 KiB)
+constexpr int TOPO_PROBES = 32;     // linear-probe limit before a key bypasses the table
+constexpr uint64_t kEmpty = ~0ull;
+
+__device__ __forceinline__ uint32_t hash64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+__global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ seg, int nx,
+                                                   int ny, int nz, int32_t lo, int bits,
+                                                   uint64_t* __restrict__ cand,
+                                                   unsigned long long* counter) {
+  __shared__ unsigned long long table[TOPO_TABLE];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  const int lane = threadIdx.x;
+  for (int s = tid; s < TOPO_TABLE; s += 256) table[s] = kEmpty;
+  __syncthreads();
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const bool in = x < nx && y < ny;
+  const int xp = in ? wp1(x, nx) : 0, yp = in ? wp1(y, ny) : 0;
+  for (int zz = 0; zz < TOPO_ZB; ++zz) {
+    const int z = blockIdx.z * TOPO_ZB + zz;
+    if (!in || z >= nz) continue;
+    const int zp = wp1(z, nz);
+    const int i = (z * ny + y) * nx + x;
+    const int my = seg[i];
+    const int nb[3] = {seg[(z * ny + y) * nx + xp], seg[(z * ny + yp) * nx + x],
+                       seg[(zp * ny + y) * nx + x]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int nl = nb[k];
+      if (my == nl) continue;
+      const int l1 = my < nl ? my : nl, l2 = my < nl ? nl : my;
+      const uint64_t key = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
+                           (uint64_t)((uint32_t)l2 - (uint32_t)lo);
+      uint32_t h = hash64(key) & (TOPO_TABLE - 1);
+      bool placed = false;
+      for (int p = 0; p < TOPO_PROBES; ++p) {
+        const unsigned long long prev = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
+        if (prev == kEmpty || prev == key) {
+          placed = true;
+          break;
+        }
+        h = (h + 1) & (TOPO_TABLE - 1);
+      }
+      if (!placed) {  // table crowded: append directly (the global unique pass removes repeats)
+        const unsigned long long s = atomicAdd(counter, 1ull);
+        cand[s] = key;
+      }
+    }
+  }
+  __syncthreads();
+  for (int s0 = 0; s0 < TOPO_TABLE; s0 += 256) {
+    const uint64_t key = table[s0 + tid];
+    const bool has = key != kEmpty;
+    const uint64_t slot = wave_append(counter, has, lane);
+    if (has) cand[slot] = key;
+  }
+}
+
+__global__ void k_topo_decode(const uint64_t* __restrict__ keys, int64_t count, int32_t lo,
+                              int bits, int32_t* __restrict__ pairs) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint64_t key = keys[k];
+  const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1);
+  pairs[2 * k] = (int32_t)((uint32_t)(key >> bits) + (uint32_t)lo);
+  pairs[2 * k + 1] = (int32_t)((uint32_t)(key & mask) + (uint32_t)lo);
+}
+
+static int nbits(uint64_t v) {
+  int b = 0;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi_label,
+                 bool labels_known, int32_t* pairs_d, int64_t cap, int64_t* needed) {
+  int32_t lo = INT_MIN;
+  int bits = 32;
+  if (labels_known) {
+    lo = lo_label;
+    bits = nbits((uint64_t)((int64_t)hi_label - (int64_t)lo_label));
+    if (bits < 1) bits = 1;
+  }
+  const int64_t maxc = 3 * v.n;
+  uint64_t* cand = c->arena.get_n<uint64_t>((size_t)maxc);
+  uint64_t* cnt = c->arena.get_n<uint64_t>(1);
+  fill_u64(c, cnt, 0, 1);
+  const dim3 grid(cdiv(v.nx, 64), cdiv(v.ny, 4), cdiv(v.nz, TOPO_ZB));
+  PNM_LAUNCH(c, "k_topo_cand", k_topo_cand, grid, dim3(64, 4, 1), 0, seg, v.nx, v.ny, v.nz, lo,
+                     bits, cand, reinterpret_cast<unsigned long long*>(cnt));
+  uint64_t h;
+  readback(c, cnt, 1, &h);
+  const int64_t ncand = (int64_t)h;
+  if (ncand == 0) {
+    *needed = 0;
+    return 0;
+  }
+  radix_sort_u64(c, cand, nullptr, ncand, 0, 2 * bits);
+  uint64_t* uniq = c->arena.get_n<uint64_t>((size_t)ncand);
+  const int64_t nu = unique_sorted_u64(c, cand, uniq, ncand);
+  *needed = nu;
+  if (nu > cap) return -1;
+  PNM_LAUNCH(c, "k_topo_decode", k_topo_decode, dim3(cdiv(nu, 256)), dim3(256), 0, uniq, nu, lo,
+                     bits, pairs_d);
+  return nu;
+}
+
+}  // namespace pnm

@@ -1,0 +1,184 @@
+// pnm_internal.hpp — shared host/device internals of libpnm_hip.so (gfx950).
+//
+// Context = one HIP stream + a slot arena of device scratch + pinned readback words.
+// Kernels live in k_*.hip; each exposes host launchers declared here. Host orchestration of the
+// reference pipelines lives in pipeline.hip, the C-ABI in capi.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pnm_capi.h"
+
+namespace pnm {
+
+struct Error {
+  pnm_status code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(pnm_status code, const std::string& msg);
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define PNM_HIP(x) ::pnm::hip_check((x), #x, __FILE__, __LINE__)
+
+// Grow-only slot arena: the k-th request of a call reuses the k-th slot when it is big enough.
+// A pipeline makes the same sequence of requests every call, so after the first call of a given
+// size no hipMalloc/hipFree happens. Memory is never initialised by the arena.
+class Arena {
+ public:
+  void reset() { next_ = 0; }
+  void* get(size_t bytes);
+  template <class T>
+  T* get_n(size_t count) {
+    return static_cast<T*>(get(count * sizeof(T)));
+  }
+  void release();
+  int64_t bytes() const;
+
+ private:
+  struct Slot {
+    void* p;
+    size_t size;
+  };
+  std::vector<Slot> slots_;
+  size_t next_ = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  Arena arena;
+  uint64_t* pinned = nullptr;  // hipHostMalloc'ed readback words
+  static constexpr int kPinnedWords = 64;
+  // Kernel timing: when `prof` is set every launch is bracketed by two HIP events on the
+  // launch stream; pnm_ctx_profile_read aggregates them per kernel name.
+  bool prof = false;
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t take_event();
+};
+
+struct KTimer {
+  KTimer(Ctx* c, const char* name);
+  ~KTimer();
+  Ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr;
+};
+
+// Launch on the context's stream, timed when profiling is on, error-checked.
+#define PNM_LAUNCH(c, name, kern, grid, block, shm, ...)                         \
+  do {                                                                           \
+    ::pnm::KTimer pnm_kt_((c), (name));                                          \
+    hipLaunchKernelGGL(kern, grid, block, shm, (c)->stream, __VA_ARGS__);        \
+    PNM_HIP(hipGetLastError());                                                  \
+  } while (0)
+
+// RAII: make ctx->device current, reset the arena for a new call.
+struct CallScope {
+  explicit CallScope(Ctx* c);
+  ~CallScope();
+  Ctx* ctx;
+  int prev_device;
+};
+
+struct Vol {
+  int nx, ny, nz;
+  int64_t n;
+};
+
+// ---------- small device utilities (k_util.hip) ----------
+void fill_u32(Ctx* c, uint32_t* p, uint32_t v, int64_t count);
+void fill_u64(Ctx* c, uint64_t* p, uint64_t v, int64_t count);
+// Copies `words` device uint64 counters to pinned memory and synchronises the stream.
+void readback(Ctx* c, const void* dev, int words, uint64_t* host_out);
+// Exclusive prefix sum of uint32 (n up to 2^31); returns nothing, out may alias in.
+void scan_exclusive_u32(Ctx* c, const uint32_t* in, uint32_t* out, int64_t n);
+// Stable LSD radix sort by keys[begin_bit, end_bit); vals optional (nullptr). Result lands in
+// keys/vals (the alt buffers are scratch).
+void radix_sort_u64(Ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
+                    int end_bit);
+void radix_sort_u32(Ctx* c, uint32_t* keys, uint32_t* vals, int64_t n, int begin_bit,
+                    int end_bit);
+// Writes keys[i] with keys[i] != keys[i-1] to out; returns the count (synchronises).
+int64_t unique_sorted_u64(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n);
+
+// ---------- plain pipeline kernels (k_extract.hip, k_segment.hip, k_topology.hip) ----------
+struct PreOut {
+  int32_t* labels;       // markers (i or -1)        [n], nullable
+  int32_t* next;         // ascent successor         [n], nullable
+  int32_t* min_pore;     // set to INT_MAX at sdf>0 peaks [n], nullable
+  uint32_t* peak_list;   // peak voxel indices (extract_pores predicate) [cap_peaks]
+  uint32_t* band_list;   // flood candidates [n], nullable
+  uint64_t* counters;    // [0] peaks, [1] band
+  int64_t cap_peaks;
+};
+void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o);
+void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
+                      const uint32_t* peaks, int64_t count, pnm_pore_t* out);
+
+// CCL of markers: union (13 forward neighbours) + flatten; flatten appends component roots to
+// key_list, sets min_solid[root] = INT_MAX. counters[2] = number of roots.
+void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_list,
+                uint64_t* counters);
+// One Jacobi flood sweep over the band list; returns the new band count (synchronises).
+int64_t flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, int64_t nband,
+                    uint32_t* band_next, int32_t* newlab, uint64_t* counter);
+// Roots by chasing `next` (cap 512) for sdf>0 voxels + first-appearance atomicMin for pores and
+// labelled solids. counters[3] = capped-walk flag.
+void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next,
+                      const int32_t* labels, int32_t* roots, int32_t* min_pore,
+                      int32_t* min_solid, uint64_t* counters, bool do_min);
+// Fallback when a walk hits the 512-step cap at a non-fixed point: full reinitialisation.
+void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots,
+                     const int32_t* labels, int32_t* min_pore, int32_t* min_solid);
+int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_pore,
+                          uint32_t* key_list, uint64_t* counter);
+// Ranks keys by their minimum voxel; writes ids into id_arr[key] (pores: rank+1; solids:
+// -(rank+1) when sdf[first] <= 0 else 0). Returns the number of ids assigned.
+int64_t assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys,
+                   int32_t* id_arr, bool solid);
+void launch_seg(Ctx* c, const float* sdf, Vol v, const int32_t* roots, const int32_t* labels,
+                const int32_t* pore_id, const int32_t* solid_id, int32_t* seg);
+// Roots for every voxel (flow basins) or sdf>0 only, computed stand-alone.
+void launch_next(Ctx* c, const float* sdf, Vol v, int32_t* next, bool all_voxels);
+void launch_chase(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
+                  bool all_voxels);
+
+// Topology: block-deduplicated candidate keys, then sort + unique. key = (l1 - lo) << bits |
+// (l2 - lo). Returns count; pairs_d receives (l1,l2) int32 pairs.
+int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi_label,
+                 bool labels_known, int32_t* pairs_d, int64_t cap, int64_t* needed);
+
+// ---------- pipelines (pipeline.hip) ----------
+struct SegResult {
+  int32_t n_pores;   // pore labels 1..n_pores
+  int32_t n_solids;  // solid labels -1..-n_solids
+};
+SegResult segment_device(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,
+                         int32_t* roots_out);
+// Returns the number of pores written; a negative value -k means k records did not fit `cap`.
+int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
+                     pnm_pore_t* out_d, int64_t cap, int64_t* total);
+int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
+                       pnm_pore_t* pores_d, int64_t cap_pores, int64_t* total_pores,
+                       int32_t* seg, int32_t* pairs_d, int64_t cap_pairs, int64_t* npairs,
+                       bool* pairs_overflow);
+
+// ---------- network flow (k_flow.hip) ----------
+// Fields are device pointers; ox/oy/oz all null = fully open. Results are malloc'ed host arrays.
+void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
+                 const double* u, const double* vv, const double* w, const double* p,
+                 const double* ox, const double* oy, const double* oz, const double grad_p[3],
+                 pnm_flow_result_t* out);
+
+}  // namespace pnm

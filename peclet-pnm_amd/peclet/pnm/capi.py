@@ -1,0 +1,242 @@
+"""ctypes view of the C-ABI of libpnm_hip.so (declared in include/pnm_capi.h).
+
+Used for device-resident calls (torch tensors already in HBM) by the benchmark and the GPU
+parity tests; the `peclet.pnm` functions themselves go through the compiled `_pnm` module.
+Every call raises RuntimeError with the library's message on a non-zero status. There is no
+fallback: a missing library or GPU is an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpnm_hip.so")
+
+PNM_OK, PNM_EINVAL, PNM_ENOMEM, PNM_EHIP, PNM_ECAPACITY, PNM_ENOCONVERGE = range(6)
+PNM_MAX_PORES = 1000000
+
+# Every exported symbol include/pnm_capi.h declares.
+EXPORTS = (
+    "pnm_version", "pnm_execution_space", "pnm_last_error", "pnm_device_count",
+    "pnm_ctx_create", "pnm_ctx_destroy", "pnm_ctx_set_stream", "pnm_ctx_stream", "pnm_ctx_trim",
+    "pnm_ctx_arena_bytes", "pnm_ctx_profile", "pnm_ctx_profile_read", "pnm_free", "pnm_extract_pores", "pnm_segment_volume",
+    "pnm_extract_topology", "pnm_extract_pore_network", "pnm_extract_network_flow",
+    "pnm_flow_result_free", "pnm_extract_pores_d", "pnm_segment_volume_d",
+    "pnm_extract_topology_d", "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
+    "pnm_stage_labels_d", "pnm_stage_roots_d",
+)
+
+
+class Pore(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float),
+                ("radius", ctypes.c_float)]
+
+
+PORE_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("radius", "<f4")])
+
+
+class FlowResult(ctypes.Structure):
+    _fields_ = [("np", ctypes.c_int64), ("nt", ctypes.c_int64),
+                ("pores", ctypes.POINTER(Pore)),
+                ("pore_pressure", ctypes.POINTER(ctypes.c_double)),
+                ("pore_residual", ctypes.POINTER(ctypes.c_double)),
+                ("throats", ctypes.POINTER(ctypes.c_int32)),
+                ("throat_flow", ctypes.POINTER(ctypes.c_double)),
+                ("throat_area", ctypes.POINTER(ctypes.c_double)),
+                ("throat_dp", ctypes.POINTER(ctypes.c_double))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `python peclet-pnm_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
+        st = ctypes.c_int
+        L.pnm_version.restype = ctypes.c_char_p
+        L.pnm_execution_space.restype = ctypes.c_char_p
+        L.pnm_last_error.restype = ctypes.c_char_p
+        L.pnm_device_count.restype = ctypes.c_int
+        L.pnm_ctx_create.restype = st
+        L.pnm_ctx_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.pnm_ctx_destroy.argtypes = [vp]
+        L.pnm_ctx_destroy.restype = None
+        L.pnm_ctx_set_stream.argtypes = [vp, vp]
+        L.pnm_ctx_set_stream.restype = st
+        L.pnm_ctx_stream.argtypes = [vp]
+        L.pnm_ctx_stream.restype = vp
+        L.pnm_ctx_trim.argtypes = [vp]
+        L.pnm_ctx_trim.restype = st
+        L.pnm_ctx_arena_bytes.argtypes = [vp]
+        L.pnm_ctx_arena_bytes.restype = ctypes.c_int64
+        L.pnm_ctx_profile.argtypes = [vp, ctypes.c_int]
+        L.pnm_ctx_profile.restype = st
+        L.pnm_ctx_profile_read.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64,
+                                           ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_int64,
+                                           i64p]
+        L.pnm_ctx_profile_read.restype = st
+        L.pnm_free.argtypes = [vp]
+        L.pnm_free.restype = None
+        L.pnm_extract_pores_d.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64, i64p, i64p]
+        L.pnm_segment_volume_d.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.pnm_extract_topology_d.argtypes = [vp, vp, vp, vp, ctypes.c_int64, i64p]
+        L.pnm_extract_pore_network_d.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64, i64p,
+                                                 i64p, vp, vp, ctypes.c_int64, i64p]
+        L.pnm_extract_network_flow_d.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                                 vp, ctypes.POINTER(FlowResult)]
+        L.pnm_extract_network_flow.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                               vp, ctypes.POINTER(FlowResult)]
+        L.pnm_flow_result_free.argtypes = [ctypes.POINTER(FlowResult)]
+        L.pnm_flow_result_free.restype = None
+        L.pnm_stage_labels_d.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.pnm_stage_roots_d.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+        for name in ("pnm_extract_pores_d", "pnm_segment_volume_d", "pnm_extract_topology_d",
+                     "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
+                     "pnm_extract_network_flow", "pnm_stage_labels_d", "pnm_stage_roots_d"):
+            getattr(L, name).restype = st
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != PNM_OK:
+        msg = lib().pnm_last_error().decode()
+        raise RuntimeError(f"pnm status {status}: {msg}")
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(a) for a in v])
+
+
+def _i3(v):
+    return (ctypes.c_int32 * 3)(*[int(a) for a in v])
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """A libpnm_hip context (own HIP stream + scratch arena) on one device."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().pnm_ctx_create(ctypes.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pnm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle):
+        _check(lib().pnm_ctx_set_stream(self.h, ctypes.c_void_p(stream_handle or 0)))
+
+    @property
+    def stream(self):
+        return lib().pnm_ctx_stream(self.h)
+
+    def arena_bytes(self):
+        return lib().pnm_ctx_arena_bytes(self.h)
+
+    def profile(self, enable=True):
+        """Bracket every kernel launch with HIP events on the launch stream."""
+        _check(lib().pnm_ctx_profile(self.h, int(bool(enable))))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} since the last read (synchronises)."""
+        cap = 128
+        names = ctypes.create_string_buffer(1 << 14)
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_int64 * cap)()
+        k = ctypes.c_int64()
+        _check(lib().pnm_ctx_profile_read(self.h, names, len(names), ms, cnt, cap,
+                                          ctypes.byref(k)))
+        keys = names.value.decode().split("\n")[: k.value]
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(keys)}
+
+    # ---- device-pointer entry points (torch tensors) ----
+    def extract_pores_d(self, sdf, dims_xyz, origin_xyz, spacing_xyz, pores_out):
+        """pores_out: torch uint8/float tensor with room for cap records (16 B each)."""
+        cap = pores_out.numel() * pores_out.element_size() // 16
+        cnt, tot = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().pnm_extract_pores_d(self.h, _ptr(sdf), _i3(dims_xyz), _f3(origin_xyz),
+                                         _f3(spacing_xyz), _ptr(pores_out), cap,
+                                         ctypes.byref(cnt), ctypes.byref(tot)))
+        return cnt.value, tot.value
+
+    def segment_volume_d(self, sdf, dims_xyz, spacing_xyz, seg_out, roots_out=None):
+        _check(lib().pnm_segment_volume_d(self.h, _ptr(sdf), _i3(dims_xyz), _f3(spacing_xyz),
+                                          _ptr(seg_out), _ptr(roots_out)))
+
+    def extract_topology_d(self, seg, dims_xyz, pairs_out):
+        cap = pairs_out.numel() // 2
+        cnt = ctypes.c_int64()
+        _check(lib().pnm_extract_topology_d(self.h, _ptr(seg), _i3(dims_xyz), _ptr(pairs_out),
+                                            cap, ctypes.byref(cnt)))
+        return cnt.value
+
+    def extract_pore_network_d(self, sdf, dims_xyz, origin_xyz, spacing_xyz, pores_out, seg_out,
+                               pairs_out):
+        cap_p = pores_out.numel() * pores_out.element_size() // 16
+        cap_q = pairs_out.numel() // 2
+        npores, tot, npairs = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().pnm_extract_pore_network_d(
+            self.h, _ptr(sdf), _i3(dims_xyz), _f3(origin_xyz), _f3(spacing_xyz), _ptr(pores_out),
+            cap_p, ctypes.byref(npores), ctypes.byref(tot), _ptr(seg_out), _ptr(pairs_out), cap_q,
+            ctypes.byref(npairs)))
+        return npores.value, tot.value, npairs.value
+
+    def stage_labels_d(self, sdf, dims_xyz, spacing_xyz, labels_out, ccl_out=None):
+        _check(lib().pnm_stage_labels_d(self.h, _ptr(sdf), _i3(dims_xyz), _f3(spacing_xyz),
+                                        _ptr(labels_out), _ptr(ccl_out)))
+
+    def stage_roots_d(self, sdf, dims_xyz, all_voxels, roots_out):
+        _check(lib().pnm_stage_roots_d(self.h, _ptr(sdf), _i3(dims_xyz), int(bool(all_voxels)),
+                                       _ptr(roots_out)))
+
+    def extract_network_flow_d(self, sdf, dims_xyz, origin_xyz, spacing_xyz, u, v, w, p,
+                               ox=None, oy=None, oz=None, grad_p_xyz=(0.0, 0.0, 0.0)):
+        r = FlowResult()
+        gp = (ctypes.c_double * 3)(*[float(a) for a in grad_p_xyz])
+        _check(lib().pnm_extract_network_flow_d(
+            self.h, _ptr(sdf), _i3(dims_xyz), _f3(origin_xyz), _f3(spacing_xyz), _ptr(u), _ptr(v),
+            _ptr(w), _ptr(p), _ptr(ox), _ptr(oy), _ptr(oz), gp, ctypes.byref(r)))
+        return _flow_to_numpy(r)
+
+
+def _flow_to_numpy(r):
+    npores, nt = r.np, r.nt
+
+    def arr(ptr, count, dtype):
+        if count == 0 or not ptr:
+            return np.zeros(0, dtype=dtype)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+    out = {
+        "pores": (np.ctypeslib.as_array(ctypes.cast(r.pores, ctypes.POINTER(ctypes.c_float)),
+                                        shape=(npores * 4,)).copy().view(PORE_DTYPE)
+                  if npores else np.zeros(0, dtype=PORE_DTYPE)),
+        "pore_pressure": arr(r.pore_pressure, npores, np.float64),
+        "pore_residual": arr(r.pore_residual, npores, np.float64),
+        "throats": arr(r.throats, 2 * nt, np.int32).reshape(-1, 2),
+        "throat_flow": arr(r.throat_flow, nt, np.float64),
+        "throat_area": arr(r.throat_area, nt, np.float64),
+        "throat_dp": arr(r.throat_dp, nt, np.float64),
+    }
+    lib().pnm_flow_result_free(ctypes.byref(r))
+    return out
