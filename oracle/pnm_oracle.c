@@ -642,10 +642,9 @@ int pnmo_extract_network_flow(const float* sdf, const int32_t dims[3], const flo
                               origin[2] + (pkj / (Nx * Ny)) * (double)spacing[2]};
       const double pip[3] = {(double)pi.x, (double)pi.y, (double)pi.z};
       const double pjp[3] = {(double)pj.x, (double)pj.y, (double)pj.z};
-      const double cc[3] = {Cx[t], Cy[t], Cz[t]};
       double macro = 0.0;
       for (int a = 0; a < 3; ++a) {
-        const double ct = cc[a] / aw;
+        const double ct = (Cx[t] * (a == 0) + Cy[t] * (a == 1) + Cz[t] * (a == 2)) / aw;
         const long long Dc = llround((ancj[a] - anc[a]) / (double)spacing[a]) -
                              llround(ct / (double)spacing[a] + 1e-6);
         const long long k = llround((double)Dc / N[a]);

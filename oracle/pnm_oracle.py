@@ -71,7 +71,7 @@ def lib():
         L.pnmo_extract_topology.restype = ctypes.c_int64
         L.pnmo_extract_topology.argtypes = [vp, vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))]
         L.pnmo_extract_network_flow.restype = ctypes.c_int
-        L.pnmo_extract_network_flow.argtypes = [vp] * 13 + [ctypes.c_int, ctypes.POINTER(_Flow)]
+        L.pnmo_extract_network_flow.argtypes = [vp] * 12 + [ctypes.c_int, ctypes.POINTER(_Flow)]
         L.pnmo_flow_free.argtypes = [ctypes.POINTER(_Flow)]
         L.pnmo_free.argtypes = [vp]
         _lib = L

@@ -1,15 +1,661 @@
-// k_flow.hip — network-flow bookkeeping (extract_network_flow_k, pore_extraction.hpp:493-997).
+// k_flow.hip — network-flow bookkeeping (extract_network_flow_k, pore_extraction.hpp:493-997)
+// on gfx950, FP64 throughout where the reference is FP64.
+//
+// Stages (one stream):
+//   segmentation (pipeline.hip) -> per-label peak voxels -> FP32 centroids -> FP64 trilinear
+//   pore pressure -> flow basins (capped ascent from every cell) -> face init (+ compacted core,
+//   film and basin-boundary face lists) -> core-tier face CCL (lock-free union-find, int64) ->
+//   film attachment (monotone min propagation to a fixpoint) -> leftover-tier CCL -> patch roots
+//   sorted by (pair, root face) -> per-face records sorted by (throat or pore, face) -> one
+//   thread per throat / pore sums its faces in face order.
+//
+// The reference accumulates Q, A, C and the residual with unordered FP64 atomics (:898-938).
+// Summing each key's contributions sequentially in ascending face order makes the result
+// deterministic and identical to a serial loop over the faces (the oracle's order).
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cstdlib>
+#include <new>
+
 #include "k_common.hpp"
 
 namespace pnm {
+
+constexpr int64_t kSent = 0x7ffffffffffffffeLL;  // film sentinel, pore_extraction.hpp:671
+
+static int nbits64(uint64_t v) {
+  int b = 0;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+struct Geo {
+  int nx, ny, nz;
+};
+
+__device__ __forceinline__ void cell_xyz(int c, Geo g, int& x, int& y, int& z) {
+  x = c % g.nx;
+  y = (c / g.nx) % g.ny;
+  z = c / (g.nx * g.ny);
+}
+
+__device__ __forceinline__ int nb_cell(int x, int y, int z, int d, Geo g) {
+  const int xx = d == 0 ? wp1(x, g.nx) : x;
+  const int yy = d == 1 ? wp1(y, g.ny) : y;
+  const int zz = d == 2 ? wp1(z, g.nz) : z;
+  return (zz * g.ny + yy) * g.nx + xx;
+}
+
+__global__ void k_iota_u32(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+// ---------------- pore centres and pressures ----------------
+// nf::peaks (:528-533): peak[seg[r]-1] = r for every sdf>0 self-root r (= the sdf>0 peaks).
+__global__ void k_flow_peaks(const float* __restrict__ sdf, const int32_t* __restrict__ seg,
+                             const uint32_t* __restrict__ peaks, int64_t npeaks,
+                             int32_t* __restrict__ peak) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npeaks) return;
+  const uint32_t r = peaks[j];
+  if (sdf[r] > 0.0f) peak[seg[r] - 1] = (int32_t)r;
+}
+
+// nf::pore_pressure (:582-613): grid coordinate in FP32, then FP64 trilinear weights in the
+// reference's dz -> dy -> dx order.
+__global__ void k_pore_pressure(const pnm_pore_t* __restrict__ pores, int64_t np, Geo g,
+                                float ox, float oy, float oz, float sx, float sy, float sz,
+                                const double* __restrict__ p, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const pnm_pore_t po = pores[k];
+  const float gx = (po.x - ox) / sx, gy = (po.y - oy) / sy, gz = (po.z - oz) / sz;
+  const double gg[3] = {(double)gx, (double)gy, (double)gz};
+  int b[3];
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    const double fl = floor(gg[a]);
+    b[a] = (int)fl;
+    f[a] = gg[a] - fl;
+  }
+  const int N[3] = {g.nx, g.ny, g.nz};
+  double acc = 0.0;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const double wt =
+            (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+        int q[3] = {b[0] + dx, b[1] + dy, b[2] + dz};
+        for (int a = 0; a < 3; ++a) q[a] = ((q[a] % N[a]) + N[a]) % N[a];
+        acc += wt * p[((int64_t)q[2] * g.ny + q[1]) * g.nx + q[0]];
+      }
+  out[k] = acc;
+}
+
+// nf::flow_basins (:621-652): flowLab = seg at the capped ascent root of every cell.
+__global__ __launch_bounds__(256) void k_flow_lab(int64_t n, const int32_t* __restrict__ next,
+                                                  const int32_t* __restrict__ seg,
+                                                  int32_t* __restrict__ lab) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int w = (int)i;
+  for (int s = 0; s < kMaxSteps; ++s) {
+    const int nx = next[w];
+    if (nx == w) break;
+    w = nx;
+  }
+  lab[i] = seg[w];
+}
+
+// ---------------- faces ----------------
+// nf::face_init (:681-696). Also compacts the core faces, the film faces and every basin
+// boundary face touching a pore (the residual's faces) into lists.
+__global__ __launch_bounds__(256) void k_face_init(
+    const float* __restrict__ sdf, const int32_t* __restrict__ lab, Geo g, int64_t n,
+    const double* __restrict__ ox, const double* __restrict__ oy, const double* __restrict__ oz,
+    int64_t* __restrict__ fpar, int64_t* __restrict__ fpair, int64_t* __restrict__ par,
+    uint32_t* __restrict__ core_list, uint32_t* __restrict__ film_list,
+    uint64_t* __restrict__ res_rec, unsigned long long* __restrict__ ctr) {
+  const int64_t c64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool in = c64 < n;
+  const int c = (int)c64;
+  int x = 0, y = 0, z = 0;
+  if (in) cell_xyz(c, g, x, y, z);
+  const int a = in ? lab[c] : 0;
+  const float sc = in ? sdf[c] : 0.0f;
+  for (int d = 0; d < 3; ++d) {
+    bool is_core = false, is_film = false;
+    int nres = 0;
+    int64_t f = 0;
+    int b = 0;
+    if (in) {
+      f = 3 * c64 + d;
+      const int nb = nb_cell(x, y, z, d, g);
+      b = lab[nb];
+      const double opn = ox ? (d == 0 ? ox[nb] : (d == 1 ? oy[nb] : oz[nb])) : 1.0;
+      const bool itf = (a != b && a > 0 && b > 0 && opn > 0.0);
+      is_core = itf && sc > 0.0f && sdf[nb] > 0.0f;
+      is_film = itf && !is_core;
+      par[f] = is_core ? f : -1;
+      fpar[f] = itf ? (is_core ? f : kSent) : -1;
+      fpair[f] = itf ? (((int64_t)(a < b ? a : b) << 32) | (int64_t)(a < b ? b : a)) : -1;
+      if (a != b) nres = (a > 0) + (b > 0);
+    }
+    const uint64_t s1 = wave_append(&ctr[0], is_core, lane);
+    if (is_core) core_list[s1] = (uint32_t)f;
+    const uint64_t s2 = wave_append(&ctr[1], is_film, lane);
+    if (is_film) film_list[s2] = (uint32_t)f;
+    // residual records: (pore index << 32 | face)
+    const uint64_t s3 = wave_append(&ctr[2], nres > 0, lane);
+    const uint64_t s4 = wave_append(&ctr[2], nres > 1, lane);
+    if (nres > 0) {
+      const int k0 = a > 0 ? a - 1 : b - 1;
+      res_rec[s3] = ((uint64_t)k0 << 32) | (uint64_t)f;
+    }
+    if (nres > 1) res_rec[s4] = ((uint64_t)(b - 1) << 32) | (uint64_t)f;
+  }
+}
+
+__device__ __forceinline__ int64_t uf_find64(const int64_t* P, int64_t x) {
+  int64_t p = ld_agent64(P + x);
+  while (p != x) {
+    x = p;
+    p = ld_agent64(P + x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_unite64(int64_t* P, int64_t a, int64_t b) {
+  while (true) {
+    a = uf_find64(P, a);
+    b = uf_find64(P, b);
+    if (a == b) return;
+    if (a > b) {
+      const int64_t t = a;
+      a = b;
+      b = t;
+    }
+    const unsigned long long old = atomicMin(reinterpret_cast<unsigned long long*>(P + b),
+                                             (unsigned long long)a);
+    if ((int64_t)old == b) return;
+    b = (int64_t)old;
+  }
+}
+
+// Iterates the 81 candidate faces (27 cells x 3 directions) of face f that share its pair key
+// and pass the doubled-offset adjacency |2dc + e_d2 - e_d|^2 <= 8 (:711-723, :776-791).
+template <class F>
+__device__ __forceinline__ void for_adjacent(int64_t f, Geo g, const int64_t* __restrict__ fpair,
+                                             F&& fn) {
+  const int c = (int)(f / 3), d = (int)(f % 3);
+  int x, y, z;
+  cell_xyz(c, g, x, y, z);
+  const int64_t pk = fpair[f];
+  const int xs[3] = {wm1(x, g.nx), x, wp1(x, g.nx)};
+  const int ys[3] = {wm1(y, g.ny), y, wp1(y, g.ny)};
+  const int zs[3] = {wm1(z, g.nz), z, wp1(z, g.nz)};
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int c2 = (zs[dz + 1] * g.ny + ys[dy + 1]) * g.nx + xs[dx + 1];
+        for (int d2 = 0; d2 < 3; ++d2) {
+          const int64_t f2 = 3 * (int64_t)c2 + d2;
+          if (f2 == f) continue;
+          const int Dx = 2 * dx + (d2 == 0) - (d == 0);
+          const int Dy = 2 * dy + (d2 == 1) - (d == 1);
+          const int Dz = 2 * dz + (d2 == 2) - (d == 2);
+          if (Dx * Dx + Dy * Dy + Dz * Dz > 8) continue;
+          if (fpair[f2] != pk) continue;
+          fn(f2);
+        }
+      }
+}
+
+// cclPass face_merge (:703-738): unions restricted to f2 > f (the relation is symmetric).
+__global__ __launch_bounds__(256) void k_face_union(const uint32_t* __restrict__ list,
+                                                    int64_t cnt, Geo g,
+                                                    const int64_t* __restrict__ fpair,
+                                                    int64_t* par) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  const int64_t f = list[j];
+  for_adjacent(f, g, fpair, [&](int64_t f2) {
+    if (f2 > f && ld_agent64(par + f2) >= 0) uf_unite64(par, f, f2);
+  });
+}
+
+// face_flatten (:740-749) fused with core_label / leftover_label: fpar[f] = root.
+__global__ void k_face_flatten(const uint32_t* __restrict__ list, int64_t cnt, int64_t* par,
+                               int64_t* __restrict__ fpar) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  const int64_t f = list[j];
+  int64_t l = par[f];
+  int64_t p = par[l];
+  while (p != l) {
+    l = p;
+    p = par[l];
+  }
+  par[f] = l;
+  fpar[f] = l;
+}
+
+// film_attach (:767-800): in-place monotone minimum over adjacent same-pair faces labelled
+// with a core root (or an attached film); the fixpoint does not depend on the order.
+__global__ __launch_bounds__(256) void k_film_attach(const uint32_t* __restrict__ list,
+                                                     int64_t cnt, Geo g,
+                                                     const int64_t* __restrict__ fpair,
+                                                     int64_t* fpar, unsigned long long* changed) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  const int64_t f = list[j];
+  const int64_t cur = fpar[f];
+  int64_t best = cur;
+  for_adjacent(f, g, fpair, [&](int64_t f2) {
+    const int64_t l2 = fpar[f2];
+    if (l2 >= 0 && l2 < kSent && l2 < best) best = l2;
+  });
+  if (best < cur) {
+    fpar[f] = best;
+    atomicAdd(changed, 1ull);
+  }
+}
+
+// leftover_init (:807-809) over the film list: par = f for unattached films (core entries of
+// par were reset to -1 beforehand); compacts them.
+__global__ void k_leftover_init(const uint32_t* __restrict__ film, int64_t cnt,
+                                const int64_t* __restrict__ fpar, int64_t* __restrict__ par,
+                                uint32_t* __restrict__ left, unsigned long long* ctr) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool keep = false;
+  int64_t f = 0;
+  if (j < cnt) {
+    f = film[j];
+    keep = fpar[f] == kSent;
+    if (keep) par[f] = f;
+  }
+  const uint64_t s = wave_append(ctr, keep, lane);
+  if (keep) left[s] = (uint32_t)f;
+}
+
+__global__ void k_reset_par(const uint32_t* __restrict__ list, int64_t cnt,
+                            int64_t* __restrict__ par) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < cnt) par[list[j]] = -1;
+}
+
+// face_roots (:826-833): interface faces that are their own patch root.
+__global__ void k_face_roots(const uint32_t* __restrict__ list, int64_t cnt,
+                             const int64_t* __restrict__ fpar, uint64_t* __restrict__ roots,
+                             unsigned long long* ctr) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool is_root = false;
+  int64_t f = 0;
+  if (j < cnt) {
+    f = list[j];
+    is_root = fpar[f] == f;
+  }
+  const uint64_t s = wave_append(ctr, is_root, lane);
+  if (is_root) roots[s] = (uint64_t)f;
+}
+
+// key for the (pair, root face) order: compact pair (lo << B | hi)
+__global__ void k_root_pairkeys(const uint64_t* __restrict__ root_fid,
+                                const uint32_t* __restrict__ order, int64_t nt,
+                                const int64_t* __restrict__ fpair, int B,
+                                uint64_t* __restrict__ key) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int64_t pk = fpair[root_fid[order[t]]];
+  const uint64_t lo = (uint64_t)(pk >> 32), hi = (uint64_t)(pk & 0x7fffffff);
+  key[t] = (lo << B) | hi;
+}
+
+// Final throat order: throats[t] = pair; the slot of each root face is stored in par[root].
+__global__ void k_throat_slots(const uint64_t* __restrict__ root_fid,
+                               const uint32_t* __restrict__ order, int64_t nt,
+                               const int64_t* __restrict__ fpair, int64_t* __restrict__ par,
+                               int32_t* __restrict__ throats) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int64_t f = (int64_t)root_fid[order[t]];
+  const int64_t pk = fpair[f];
+  throats[2 * t] = (int32_t)(pk >> 32);
+  throats[2 * t + 1] = (int32_t)(pk & 0x7fffffff);
+  par[f] = t;
+}
+
+// Throat records for every interface face (core and film lists): (slot << 32 | face).
+__global__ void k_throat_records(const uint32_t* __restrict__ list, int64_t cnt,
+                                 const int64_t* __restrict__ fpar,
+                                 const int64_t* __restrict__ slot_of,
+                                 uint64_t* __restrict__ rec) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  const int64_t f = list[j];
+  const int64_t slot = slot_of[fpar[f]];
+  rec[j] = ((uint64_t)slot << 32) | (uint64_t)f;
+}
+
+// Segment [start, end) of each key in a (key << 32 | face)-sorted record list.
+__global__ void k_seg_bounds(const uint64_t* __restrict__ rec, int64_t cnt,
+                             int64_t* __restrict__ start, int64_t* __restrict__ end) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint64_t k = rec[i] >> 32;
+  if (i == 0 || (rec[i - 1] >> 32) != k) start[k] = i;
+  if (i == cnt - 1 || (rec[i + 1] >> 32) != k) end[k] = i + 1;
+}
+
+struct FluxArgs {
+  Geo g;
+  const int32_t* lab;
+  const double *u, *v, *w, *ox, *oy, *oz;
+  double Ax, Ay, Az;
+};
+
+// q of face f (throat_flux :889-897): the shared face is the -d face of the +d neighbour.
+__device__ __forceinline__ double face_q(const FluxArgs& A, int64_t f, int& a, int& b,
+                                         double& w0, int& x, int& y, int& z, int& d) {
+  const int c = (int)(f / 3);
+  d = (int)(f % 3);
+  cell_xyz(c, A.g, x, y, z);
+  const int nb = nb_cell(x, y, z, d, A.g);
+  a = A.lab[c];
+  b = A.lab[nb];
+  const double vel = d == 0 ? A.u[nb] : (d == 1 ? A.v[nb] : A.w[nb]);
+  const double opn = A.ox ? (d == 0 ? A.ox[nb] : (d == 1 ? A.oy[nb] : A.oz[nb])) : 1.0;
+  const double area = d == 0 ? A.Ax : (d == 1 ? A.Ay : A.Az);
+  w0 = opn * area;
+  return opn * vel * area;
+}
+
+// Per-pore residual: the pore's boundary faces in face order (:898-901).
+__global__ void k_resid_sum(const uint64_t* __restrict__ rec, const int64_t* __restrict__ start,
+                            const int64_t* __restrict__ end, int64_t np, FluxArgs A,
+                            double* __restrict__ resid) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  double acc = 0.0;
+  for (int64_t i = start[k]; i < end[k]; ++i) {
+    const int64_t f = (int64_t)(rec[i] & 0xffffffffull);
+    int a, b, x, y, z, d;
+    double w0;
+    const double q = face_q(A, f, a, b, w0, x, y, z, d);
+    acc += (a - 1 == (int)k) ? q : -q;
+  }
+  resid[k] = acc;
+}
+
+// Per-throat Q, A, area-weighted min-imaged face centre (:903-939), face order.
+__global__ void k_throat_sum(const uint64_t* __restrict__ rec, const int64_t* __restrict__ start,
+                             const int64_t* __restrict__ end, int64_t nt, FluxArgs A,
+                             const int32_t* __restrict__ peak, float oxo, float oyo, float ozo,
+                             float sx, float sy, float sz, double* __restrict__ Q,
+                             double* __restrict__ Ar, double* __restrict__ Cx,
+                             double* __restrict__ Cy, double* __restrict__ Cz) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const double Lw[3] = {(double)A.g.nx * sx, (double)A.g.ny * sy, (double)A.g.nz * sz};
+  double q_acc = 0.0, a_acc = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  for (int64_t i = start[t]; i < end[t]; ++i) {
+    const int64_t f = (int64_t)(rec[i] & 0xffffffffull);
+    int a, b, ixx, iyy, izz, d;
+    double w0;
+    const double q = face_q(A, f, a, b, w0, ixx, iyy, izz, d);
+    const int lo = a < b ? a : b;
+    q_acc += (a < b ? q : -q);
+    a_acc += w0;
+    const int pk = peak[lo - 1];
+    int pkx, pky, pkz;
+    cell_xyz(pk, A.g, pkx, pky, pkz);
+    double fp[3] = {oxo + (ixx + (d == 0 ? 0.5 : 0.0)) * (double)sx,
+                    oyo + (iyy + (d == 1 ? 0.5 : 0.0)) * (double)sy,
+                    ozo + (izz + (d == 2 ? 0.5 : 0.0)) * (double)sz};
+    const double pc[3] = {oxo + pkx * (double)sx, oyo + pky * (double)sy,
+                          ozo + pkz * (double)sz};
+    for (int a2 = 0; a2 < 3; ++a2) {
+      double dv = fp[a2] - pc[a2];
+      dv -= Lw[a2] * round(dv / Lw[a2]);
+      fp[a2] = dv;
+    }
+    cx += w0 * fp[0];
+    cy += w0 * fp[1];
+    cz += w0 * fp[2];
+  }
+  Q[t] = q_acc;
+  Ar[t] = a_acc;
+  Cx[t] = cx;
+  Cy[t] = cy;
+  Cz[t] = cz;
+}
+
+// throat_dp (host in the reference, :958-995).
+__global__ void k_throat_dp(const int32_t* __restrict__ throats, int64_t nt,
+                            const pnm_pore_t* __restrict__ pores, const double* __restrict__ pp,
+                            const int32_t* __restrict__ peak, const double* __restrict__ Ar,
+                            const double* __restrict__ Cx, const double* __restrict__ Cy,
+                            const double* __restrict__ Cz, Geo g, float ox, float oy, float oz,
+                            float sx, float sy, float sz, double gpx, double gpy, double gpz,
+                            double* __restrict__ dp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int li = throats[2 * t], lj = throats[2 * t + 1];
+  const pnm_pore_t pi = pores[li - 1], pj = pores[lj - 1];
+  const double aw = Ar[t];
+  const int pk = peak[li - 1], pkj = peak[lj - 1];
+  const float o[3] = {ox, oy, oz};
+  const float s[3] = {sx, sy, sz};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  const double L[3] = {(double)g.nx * sx, (double)g.ny * sy, (double)g.nz * sz};
+  const int pki[3] = {pk % g.nx, (pk / g.nx) % g.ny, pk / (g.nx * g.ny)};
+  const int pkj3[3] = {pkj % g.nx, (pkj / g.nx) % g.ny, pkj / (g.nx * g.ny)};
+  const double pip[3] = {(double)pi.x, (double)pi.y, (double)pi.z};
+  const double pjp[3] = {(double)pj.x, (double)pj.y, (double)pj.z};
+  const double gp[3] = {gpx, gpy, gpz};
+  double macro = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double anc = o[a] + pki[a] * (double)s[a];
+    const double ancj = o[a] + pkj3[a] * (double)s[a];
+    const double ct = (Cx[t] * (a == 0) + Cy[t] * (a == 1) + Cz[t] * (a == 2)) / aw;
+    const long long Dc = (long long)round((ancj - anc) / (double)s[a]) -
+                         (long long)round(ct / (double)s[a] + 1e-6);
+    const long long k = (long long)round((double)Dc / N[a]);
+    macro += gp[a] * ((pjp[a] - pip[a]) - L[a] * (double)k);
+  }
+  dp[t] = (pp[li - 1] - pp[lj - 1]) - macro;
+}
+
+// ------------------------------------------------------------------------------------------
+template <class T>
+static T* host_array(Ctx* c, const T* dev, int64_t count) {
+  T* h = static_cast<T*>(std::malloc(sizeof(T) * (size_t)std::max<int64_t>(count, 1)));
+  if (!h) throw std::bad_alloc();
+  if (count > 0)
+    PNM_HIP(hipMemcpyAsync(h, dev, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+  return h;
+}
+
+int64_t face_ccl(Ctx* c, const uint32_t* list, int64_t cnt, Geo g, const int64_t* fpair,
+                 int64_t* par, int64_t* fpar) {
+  if (cnt <= 0) return 0;
+  PNM_LAUNCH(c, "k_face_union", k_face_union, dim3(cdiv(cnt, 256)), dim3(256), 0, list, cnt, g,
+             fpair, par);
+  PNM_LAUNCH(c, "k_face_flatten", k_face_flatten, dim3(cdiv(cnt, 256)), dim3(256), 0, list, cnt,
+             par, fpar);
+  return cnt;
+}
 
 void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                  const double* u, const double* vv, const double* w, const double* p,
                  const double* ox, const double* oy, const double* oz, const double grad_p[3],
                  pnm_flow_result_t* out) {
-  (void)c; (void)sdf; (void)v; (void)org; (void)spc; (void)u; (void)vv; (void)w; (void)p;
-  (void)ox; (void)oy; (void)oz; (void)grad_p; (void)out;
-  fail(PNM_EHIP, "extract_network_flow: not built yet");
+  const int64_t n = v.n;
+  const Geo g{v.nx, v.ny, v.nz};
+  int32_t* seg = c->arena.get_n<int32_t>((size_t)n);
+  int32_t* roots = c->arena.get_n<int32_t>((size_t)n);
+  SegPeaks sp = segment_with_peaks(c, sdf, v, spc, seg, roots);
+  const int64_t np = sp.n_pores;
+  if (np <= 0) {  // np == 0 returns empty in the reference; np < 0 cannot happen here
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  uint64_t* ctr = c->arena.get_n<uint64_t>(8);
+  fill_u64(c, ctr, 0, 8);
+  unsigned long long* uctr = reinterpret_cast<unsigned long long*>(ctr);
+
+  // peaks and centres
+  int32_t* peak = c->arena.get_n<int32_t>((size_t)np);
+  fill_u32(c, reinterpret_cast<uint32_t*>(peak), 0u, np);
+  PNM_LAUNCH(c, "k_flow_peaks", k_flow_peaks, dim3(cdiv(std::max<int64_t>(sp.npeaks, 1), 256)),
+             dim3(256), 0, sdf, seg, sp.peaks, sp.npeaks, peak);
+  pnm_pore_t* pores = c->arena.get_n<pnm_pore_t>((size_t)np);
+  launch_centroids(c, sdf, v, org, spc, reinterpret_cast<const uint32_t*>(peak), np, pores);
+  double* pp = c->arena.get_n<double>((size_t)np);
+  PNM_LAUNCH(c, "k_pore_pressure", k_pore_pressure, dim3(cdiv(np, 256)), dim3(256), 0, pores, np,
+             g, org[0], org[1], org[2], spc[0], spc[1], spc[2], p, pp);
+
+  // flow basins
+  int32_t* next_all = c->arena.get_n<int32_t>((size_t)n);
+  int32_t* lab = c->arena.get_n<int32_t>((size_t)n);
+  launch_next(c, sdf, v, next_all, true);
+  PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256)), dim3(256), 0, n, next_all, seg,
+             lab);
+
+  // faces
+  const int64_t nf = 3 * n;
+  int64_t* fpar = c->arena.get_n<int64_t>((size_t)nf);
+  int64_t* fpair = c->arena.get_n<int64_t>((size_t)nf);
+  int64_t* par = c->arena.get_n<int64_t>((size_t)nf);
+  uint32_t* core_list = c->arena.get_n<uint32_t>((size_t)nf);
+  uint32_t* film_list = c->arena.get_n<uint32_t>((size_t)nf);
+  uint64_t* res_rec = c->arena.get_n<uint64_t>((size_t)(2 * nf));
+  PNM_LAUNCH(c, "k_face_init", k_face_init, dim3(cdiv(n, 256)), dim3(256), 0, sdf, lab, g, n, ox,
+             oy, oz, fpar, fpair, par, core_list, film_list, res_rec, uctr);
+  uint64_t h[3];
+  readback(c, ctr, 3, h);
+  const int64_t ncore = (int64_t)h[0], nfilm = (int64_t)h[1], nres = (int64_t)h[2];
+
+  // core tier
+  face_ccl(c, core_list, ncore, g, fpair, par, fpar);
+  // film attachment to a fixpoint
+  if (nfilm > 0) {
+    int64_t sweeps = 0;
+    while (true) {
+      fill_u64(c, ctr + 3, 0, 1);
+      PNM_LAUNCH(c, "k_film_attach", k_film_attach, dim3(cdiv(nfilm, 256)), dim3(256), 0,
+                 film_list, nfilm, g, fpair, fpar, uctr + 3);
+      uint64_t ch;
+      readback(c, ctr + 3, 1, &ch);
+      if (ch == 0) break;
+      if (++sweeps > nfilm + 2) fail(PNM_ENOCONVERGE, "film attachment did not converge");
+    }
+  }
+  // leftover tier
+  uint32_t* left = c->arena.get_n<uint32_t>((size_t)std::max<int64_t>(nfilm, 1));
+  if (ncore > 0)
+    PNM_LAUNCH(c, "k_reset_par", k_reset_par, dim3(cdiv(ncore, 256)), dim3(256), 0, core_list,
+               ncore, par);
+  int64_t nleft = 0;
+  if (nfilm > 0) {
+    fill_u64(c, ctr + 4, 0, 1);
+    PNM_LAUNCH(c, "k_leftover_init", k_leftover_init, dim3(cdiv(nfilm, 256)), dim3(256), 0,
+               film_list, nfilm, fpar, par, left, uctr + 4);
+    readback(c, ctr + 4, 1, h);
+    nleft = (int64_t)h[0];
+    face_ccl(c, left, nleft, g, fpair, par, fpar);
+  }
+
+  // patch roots -> throats ordered by (pair, root face)
+  uint64_t* root_fid = c->arena.get_n<uint64_t>((size_t)std::max<int64_t>(ncore + nfilm, 1));
+  fill_u64(c, ctr + 5, 0, 1);
+  if (ncore > 0)
+    PNM_LAUNCH(c, "k_face_roots", k_face_roots, dim3(cdiv(ncore, 256)), dim3(256), 0, core_list,
+               ncore, fpar, root_fid, uctr + 5);
+  if (nfilm > 0)
+    PNM_LAUNCH(c, "k_face_roots", k_face_roots, dim3(cdiv(nfilm, 256)), dim3(256), 0, film_list,
+               nfilm, fpar, root_fid, uctr + 5);
+  readback(c, ctr + 5, 1, h);
+  const int64_t nt = (int64_t)h[0];
+  int32_t* throats = c->arena.get_n<int32_t>((size_t)std::max<int64_t>(2 * nt, 2));
+  double* Q = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* Ar = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* Cx = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* Cy = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* Cz = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* dp = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* resid = c->arena.get_n<double>((size_t)np);
+  FluxArgs fa{g, lab, u, vv, w, ox, oy, oz, (double)spc[1] * spc[2], (double)spc[0] * spc[2],
+              (double)spc[0] * spc[1]};
+  if (nt > 0) {
+    uint32_t* order = c->arena.get_n<uint32_t>((size_t)nt);
+    uint64_t* key = c->arena.get_n<uint64_t>((size_t)nt);
+    // order = sort roots by fid, then stably by compact pair key
+    PNM_LAUNCH(c, "k_iota_u32", k_iota_u32, dim3(cdiv(nt, 256)), dim3(256), 0, order, nt);
+    {
+      uint64_t* fk = c->arena.get_n<uint64_t>((size_t)nt);
+      PNM_HIP(hipMemcpyAsync(fk, root_fid, sizeof(uint64_t) * nt, hipMemcpyDeviceToDevice,
+                             c->stream));
+      radix_sort_u64(c, fk, order, nt, 0, nbits64((uint64_t)nf));
+    }
+    const int B = std::max(1, nbits64((uint64_t)np));
+    PNM_LAUNCH(c, "k_root_pairkeys", k_root_pairkeys, dim3(cdiv(nt, 256)), dim3(256), 0, root_fid,
+               order, nt, fpair, B, key);
+    radix_sort_u64(c, key, order, nt, 0, 2 * B);
+    PNM_LAUNCH(c, "k_throat_slots", k_throat_slots, dim3(cdiv(nt, 256)), dim3(256), 0, root_fid,
+               order, nt, fpair, par, throats);
+    // throat records over every interface face
+    const int64_t nitf = ncore + nfilm;
+    uint64_t* trec = c->arena.get_n<uint64_t>((size_t)nitf);
+    if (ncore > 0)
+      PNM_LAUNCH(c, "k_throat_records", k_throat_records, dim3(cdiv(ncore, 256)), dim3(256), 0,
+                 core_list, ncore, fpar, par, trec);
+    if (nfilm > 0)
+      PNM_LAUNCH(c, "k_throat_records", k_throat_records, dim3(cdiv(nfilm, 256)), dim3(256), 0,
+                 film_list, nfilm, fpar, par, trec + ncore);
+    radix_sort_u64(c, trec, nullptr, nitf, 0, 32 + nbits64((uint64_t)nt));
+    int64_t* ts = c->arena.get_n<int64_t>((size_t)nt);
+    int64_t* te = c->arena.get_n<int64_t>((size_t)nt);
+    fill_u64(c, reinterpret_cast<uint64_t*>(ts), 0, nt);
+    fill_u64(c, reinterpret_cast<uint64_t*>(te), 0, nt);
+    PNM_LAUNCH(c, "k_seg_bounds", k_seg_bounds, dim3(cdiv(nitf, 256)), dim3(256), 0, trec, nitf,
+               ts, te);
+    PNM_LAUNCH(c, "k_throat_sum", k_throat_sum, dim3(cdiv(nt, 64)), dim3(64), 0, trec, ts, te, nt,
+               fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], Q, Ar, Cx, Cy, Cz);
+    PNM_LAUNCH(c, "k_throat_dp", k_throat_dp, dim3(cdiv(nt, 256)), dim3(256), 0, throats, nt,
+               pores, pp, peak, Ar, Cx, Cy, Cz, g, org[0], org[1], org[2], spc[0], spc[1],
+               spc[2], grad_p[0], grad_p[1], grad_p[2], dp);
+  }
+  // residual
+  {
+    int64_t* rs = c->arena.get_n<int64_t>((size_t)np);
+    int64_t* re = c->arena.get_n<int64_t>((size_t)np);
+    fill_u64(c, reinterpret_cast<uint64_t*>(rs), 0, np);
+    fill_u64(c, reinterpret_cast<uint64_t*>(re), 0, np);
+    if (nres > 0) {
+      radix_sort_u64(c, res_rec, nullptr, nres, 0, 32 + nbits64((uint64_t)np));
+      PNM_LAUNCH(c, "k_seg_bounds", k_seg_bounds, dim3(cdiv(nres, 256)), dim3(256), 0, res_rec,
+                 nres, rs, re);
+    }
+    PNM_LAUNCH(c, "k_resid_sum", k_resid_sum, dim3(cdiv(np, 64)), dim3(64), 0, res_rec, rs, re,
+               np, fa, resid);
+  }
+
+  out->np = np;
+  out->nt = nt;
+  out->pores = host_array(c, pores, np);
+  out->pore_pressure = host_array(c, pp, np);
+  out->pore_residual = host_array(c, resid, np);
+  out->throats = host_array(c, throats, 2 * nt);
+  out->throat_flow = host_array(c, Q, nt);
+  out->throat_area = host_array(c, Ar, nt);
+  out->throat_dp = host_array(c, dp, nt);
+  PNM_HIP(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace pnm

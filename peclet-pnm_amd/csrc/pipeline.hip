@@ -128,6 +128,12 @@ SegResult segment_device(Ctx* c, const float* sdf, Vol v, const float spc[3], in
   return segment_core(c, sdf, v, spc, seg, roots_out).r;
 }
 
+SegPeaks segment_with_peaks(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,
+                            int32_t* roots_out) {
+  const SegInternal s = segment_core(c, sdf, v, spc, seg, roots_out);
+  return SegPeaks{s.r.n_pores, s.r.n_solids, s.peaks, s.npeaks};
+}
+
 // Fused pipeline: returns pore count (negative = capacity needed) and pair count via args.
 int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                        pnm_pore_t* pores_d, int64_t cap_pores, int64_t* total_pores,

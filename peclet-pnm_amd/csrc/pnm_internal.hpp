@@ -166,6 +166,16 @@ struct SegResult {
 };
 SegResult segment_device(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,
                          int32_t* roots_out);
+// Segmentation that also hands back the (unsorted) device peak list of the extract_pores
+// predicate; the sdf>0 entries are exactly the self-roots.
+struct SegPeaks {
+  int32_t n_pores;
+  int32_t n_solids;
+  const uint32_t* peaks;
+  int64_t npeaks;
+};
+SegPeaks segment_with_peaks(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,
+                            int32_t* roots_out);
 // Returns the number of pores written; a negative value -k means k records did not fit `cap`.
 int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                      pnm_pore_t* out_d, int64_t cap, int64_t* total);

@@ -1,0 +1,115 @@
+"""Network-flow oracle pinned by closed forms and invariants (CPU only).
+
+  - divergence-free MAC field  -> every pore residual ~ 0 (the residual is the basin's summed
+    discrete divergence); pins the face -> velocity mapping (velocity at the +d neighbour)
+  - uniform x-flow on an x-aligned chamber chain -> Q/A = +-U on every throat, #throats = #pores
+  - two chambers joined directly and through the periodic wrap -> the pair appears twice
+  - p = const -> pore_pressure = const; p linear in the cell index -> exact trilinear value
+  - p = 0 with grad_p along x -> dp + gx (xj - xi) is gx * L * (integer image count)
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import flowcases as fc
+import pnm_oracle as orc
+
+
+def lattice48():
+    from peclet.pnm import synth
+
+    return synth.chamber_sdf((48, 48, 48), m=2, chamber_r=7.0, tube_rmin=2.0, tube_rmax=4.0,
+                             seed=5)
+
+
+def test_divergence_free_field_has_zero_residual():
+    sdf = lattice48()
+    spc = (1.5, 1.0, 0.5)
+    f = fc.divergence_free(sdf.shape, spc, seed=1)
+    div = fc.mac_divergence(f, spc)
+    assert np.abs(div).max() < 1e-12
+    r = orc.extract_network_flow(sdf, (0, 0, 0), spc, f["u"], f["v"], f["w"], f["p"])
+    scale = np.abs(f["u"]).max() * spc[0] * spc[1]
+    assert r["pore_residual"].size >= 8
+    assert np.abs(r["pore_residual"]).max() < 1e-11 * scale * sdf.size ** 0.5
+    # a mis-mapped velocity (u at the cell instead of its +x neighbour) breaks it
+    g = dict(f)
+    g["u"] = np.roll(f["u"], 1, axis=2)
+    r2 = orc.extract_network_flow(sdf, (0, 0, 0), spc, g["u"], g["v"], g["w"], g["p"])
+    assert np.abs(r2["pore_residual"]).max() > 1e-6 * scale
+
+
+def test_uniform_flow_on_chain():
+    sdf = fc.chain_x(3, 24, 7.0, 3.0)
+    U = 2.5
+    f = fc.uniform_x(sdf.shape, U)
+    r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"], f["p"])
+    assert len(r["pores"]) == 3
+    assert len(r["throats"]) == 3  # periodic ring: #throats = #pores
+    np.testing.assert_array_equal(np.abs(r["throat_flow"] / r["throat_area"]), np.full(3, U))
+    assert np.all(r["pore_residual"] == 0.0)  # uniform flux: exact cancellation
+    # flow runs +x: through each throat from the lower-x chamber to the next
+    for (li, lj), q in zip(r["throats"], r["throat_flow"]):
+        xi, xj = r["pores"][li - 1]["x"], r["pores"][lj - 1]["x"]
+        dx = (xj - xi + 36) % 72 - 36
+        assert np.sign(q) == np.sign(dx)
+
+
+def test_parallel_patches_repeat_the_pair():
+    sdf = fc.two_chambers_two_tubes()
+    f = fc.uniform_x(sdf.shape, 1.0)
+    r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"], f["p"])
+    assert len(r["pores"]) == 2
+    assert [tuple(t) for t in r["throats"]] == [(1, 2), (1, 2)]
+    q = r["throat_flow"]
+    assert q[0] == -q[1] and abs(q[0]) > 0  # one patch carries 1->2, the other 2->1
+
+
+def test_pressure_constant_and_linear():
+    sdf = fc.chain_x(3, 24, 7.0, 3.0)
+    shape = sdf.shape
+    f = fc.uniform_x(shape, 0.0)
+    c = 3.25
+    r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"],
+                                 np.full(shape, c))
+    np.testing.assert_allclose(r["pore_pressure"], c, rtol=0, atol=4 * np.spacing(c))
+    z, y, x = fc.grid(shape)
+    al, be, ga = 0.125, -0.5, 0.25
+    p = al * x + be * y + ga * z
+    org, spc = (2.0, -1.0, 0.5), (1.5, 1.0, 0.75)  # zyx
+    r = orc.extract_network_flow(sdf, org, spc, f["u"], f["v"], f["w"], p)
+    for k, po in enumerate(r["pores"]):
+        gx = np.float32(np.float32(po["x"]) - np.float32(org[2])) / np.float32(spc[2])
+        gy = np.float32(np.float32(po["y"]) - np.float32(org[1])) / np.float32(spc[1])
+        gz = np.float32(np.float32(po["z"]) - np.float32(org[0])) / np.float32(spc[0])
+        if not (0 <= gx < shape[2] - 1 and 0 <= gy < shape[1] - 1 and 0 <= gz < shape[0] - 1):
+            continue  # the trilinear stencil wraps: linear p is not linear across the seam
+        want = al * float(gx) + be * float(gy) + ga * float(gz)
+        assert abs(r["pore_pressure"][k] - want) < 1e-12 * (1 + abs(want))
+
+
+def test_dp_image_count_is_integer():
+    sdf = fc.chain_x(3, 24, 7.0, 3.0)
+    f = fc.uniform_x(sdf.shape, 1.0)
+    gx = 0.01
+    spc = (1.0, 1.0, 0.5)  # zyx
+    r = orc.extract_network_flow(sdf, (0, 0, 0), spc, f["u"], f["v"], f["w"], f["p"],
+                                 grad_p_zyx=(0.0, 0.0, gx))
+    Lx = sdf.shape[2] * spc[2]
+    for (li, lj), dp in zip(r["throats"], r["throat_dp"]):
+        xi = float(r["pores"][li - 1]["x"])
+        xj = float(r["pores"][lj - 1]["x"])
+        k = (dp / gx + (xj - xi)) / Lx  # dp = -gx ((xj - xi) - L k)
+        assert abs(k - round(k)) < 1e-9
+    # the wrap-around throat of the ring has a non-zero image count
+    ks = [round((dp / gx + float(r["pores"][lj - 1]["x"]) - float(r["pores"][li - 1]["x"])) / Lx)
+          for (li, lj), dp in zip(r["throats"], r["throat_dp"])]
+    assert sorted(abs(k) for k in ks) == [0, 0, 1]
+
+
+def test_empty_when_no_pores():
+    sdf = np.full((6, 6, 6), -2.0, np.float32)
+    z = np.zeros(sdf.shape)
+    r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), z, z, z, z)
+    assert len(r["pores"]) == 0 and len(r["throats"]) == 0
