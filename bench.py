@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 ALGO_BYTES_PER_VOXEL = {
     "k_pre": 12,          # read sdf 4; write marker label 4, ascent successor 4
     "k_pre_peaks": 4,     # read sdf 4
-    "k_ccl_union": 4,     # read marker labels 4 (union-find writes are sparse)
+    "k_ccl_runs": 4,      # read marker labels 4 (neighbour rows are cache hits; unions sparse)
     "k_ccl_flatten": 8,   # read + write labels
     "k_roots_min": 16,    # read sdf 4, labels 4, successor 4; write roots 4
     "k_seg": 16,          # read sdf 4, roots 4, labels 4; write seg 4

@@ -66,12 +66,23 @@ __global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int 
       pos_peak = is_peak && (cv > 0.0f);
     }
     if (SEG) {
-      const bool mk = cv < thr;  // init_markers (:177)
-      o.labels[i] = mk ? i : -1;
       o.next[i] = nxt;
-      band = !(cv >= 0.0f) && !mk;  // flood candidates (:245-248)
       if (pos_peak) o.min_pore[i] = INT_MAX;
     }
+  }
+  if (SEG) {
+    // init_markers (:177) with the label of a marker set to the start of its x-run inside this
+    // wave's 64-voxel row chunk (the run is connected along x, so the component minimum is
+    // unchanged); the CCL then only unites runs.
+    const float cv = in ? sdf[i] : 0.0f;
+    const bool mk = in && (cv < thr);
+    const uint64_t m = __ballot(mk);
+    const uint64_t starts = m & ~(m << 1);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+    const uint64_t sm = starts & upto;
+    const int s = sm ? 63 - __clzll((long long)sm) : lane;
+    if (in) o.labels[i] = mk ? i - (lane - s) : -1;
+    band = in && !(cv >= 0.0f) && !mk;  // flood candidates (:245-248)
   }
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(o.counters);
   const uint64_t slot = wave_append(&ctr[0], is_peak, lane);

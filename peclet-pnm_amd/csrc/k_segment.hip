@@ -40,40 +40,57 @@ __device__ __forceinline__ void uf_unite(int* L, int a, int b) {
   }
 }
 
-// merge_markers (pore_extraction.hpp:198-224): 13 forward periodic neighbours (:204-206).
-__global__ __launch_bounds__(256) void k_ccl_union(int* L, int nx, int ny, int nz) {
+// merge_markers (pore_extraction.hpp:198-224) over x-runs. Every marker carries the index of
+// the start of its x-run within its 64-voxel row chunk. A wave owns one row chunk; for each of
+// the forward rows of the 13-neighbourhood (:204-206) — (y+1, z), (y-1, z+1), (y, z+1),
+// (y+1, z+1) — it unites each of its runs with every run of that row it touches (26-adjacency:
+// columns x-1, x, x+1). One union per touching pair suffices: at column x where both rows are
+// markers (skipped when column x-1 gave the same pair), and at a run's first / last column for
+// the diagonal-only contact. The +x neighbour across chunk (and periodic row) boundaries is the
+// remaining forward relation. The resulting components, hence the component minima, are those of
+// the voxel-level relation.
+__global__ __launch_bounds__(256) void k_ccl_runs(int* L, int nx, int ny, int nz) {
   const int x = blockIdx.x * 64 + threadIdx.x;
   const int y = blockIdx.y * 4 + threadIdx.y;
   const int z = blockIdx.z;
-  if (x >= nx || y >= ny) return;
-  const int i = (z * ny + y) * nx + x;
-  if (L[i] == -1) return;
-  const int xm = wm1(x, nx), xp = wp1(x, nx);
-  const int yp = wp1(y, ny), ym = wm1(y, ny);
-  const int zp = wp1(z, nz);
-  const int xs[3] = {xm, x, xp};
-  // (dz=1: dy=-1,0,1 x dx=-1,0,1), (dz=0, dy=1: dx=-1,0,1), (dz=0, dy=0, dx=1)
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const int yy = b == 0 ? ym : (b == 1 ? y : yp);
-    const int row = (zp * ny + yy) * nx;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int ni = row + xs[c];
-      if (L[ni] != -1) uf_unite(L, i, ni);
-    }
+  const int lane = threadIdx.x;
+  const bool in = (x < nx) && (y < ny);
+  const int i = in ? (z * ny + y) * nx + x : 0;
+  const int my = in ? L[i] : -1;
+  const bool mk = my != -1;
+  const uint64_t m = __ballot(mk);
+  if (!__any(mk)) return;  // wave-uniform
+  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+  const bool run_end = mk && (lane == 63 || !((m >> (lane + 1)) & 1ull));
+  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
+  // +x relation across the chunk start; at x = 0 this is the periodic wrap from x = nx-1
+  if (mk && lane == 0) {
+    const int nl = L[(z * ny + y) * nx + xm];
+    if (nl != -1) uf_unite(L, my, nl);
   }
-  {
-    const int row = (z * ny + yp) * nx;
+  const int yp = in ? wp1(y, ny) : 0, ym = in ? wm1(y, ny) : 0, zp = wp1(z, nz);
+  const int rows[4] = {(z * ny + yp) * nx, (zp * ny + ym) * nx, (zp * ny + y) * nx,
+                       (zp * ny + yp) * nx};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int ni = row + xs[c];
-      if (L[ni] != -1) uf_unite(L, i, ni);
+  for (int r = 0; r < 4; ++r) {
+    const int c = in ? L[rows[r] + x] : -1;
+    const bool cm = c != -1;
+    // previous lane's pair for the duplicate test at column x
+    const int prev_c = __shfl_up(c, 1, 64);
+    if (mk && cm) {
+      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+      if (!dup) uf_unite(L, my, c);
     }
-  }
-  {
-    const int ni = (z * ny + y) * nx + xp;
-    if (L[ni] != -1) uf_unite(L, i, ni);
+    if (mk && !cm) {
+      if (run_start) {
+        const int a = L[rows[r] + xm];
+        if (a != -1) uf_unite(L, my, a);
+      }
+      if (run_end) {
+        const int b = L[rows[r] + xp];
+        if (b != -1) uf_unite(L, my, b);
+      }
+    }
   }
 }
 
@@ -104,8 +121,7 @@ __global__ __launch_bounds__(256) void k_ccl_flatten(int* L, int64_t n, int32_t*
 
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_list,
                 uint64_t* counters) {
-  PNM_LAUNCH(c, "k_ccl_union", k_ccl_union, vox_grid(v), kVoxBlock, 0, labels, v.nx, v.ny,
-                     v.nz);
+  PNM_LAUNCH(c, "k_ccl_runs", k_ccl_runs, vox_grid(v), kVoxBlock, 0, labels, v.nx, v.ny, v.nz);
   PNM_LAUNCH(c, "k_ccl_flatten", k_ccl_flatten, dim3(cdiv(v.n, 256)), dim3(256), 0, labels, v.n,
                      min_solid, key_list, reinterpret_cast<unsigned long long*>(counters + 2));
 }
