@@ -179,6 +179,23 @@ pnm_status pnm_extract_network_flow_d(pnm_ctx_t ctx, const float* sdf_d, const i
                                       const double* oz_d, const double grad_p[3],
                                       pnm_flow_result_t* out);
 
+/* ---- batched segmentation (project extension; no single-call counterpart in the reference:
+ * the result equals nvol separate segment_volume calls, pore_extraction.hpp:389-397, each volume
+ * its own periodic box numbered on its own) ---- */
+#define PNM_BATCH_GRAPH 1    /* replay the persistent launch from a cached hipGraph */
+#define PNM_BATCH_AUTOTUNE 2 /* time launch shapes once per batch shape and keep the fastest */
+/* sdf, seg: host arrays of nvol DEVICE pointers (float32[n_k] in, int32[n_k] out);
+ * dims: host int32[3*nvol], X,Y,Z per volume; spacing: common, min(spacing) must be > 0.
+ * n_pores_d / n_solids_d: nullable device int32[nvol] receiving each volume's label counts.
+ * One persistent kernel segments all volumes (work queue, largest volumes first). */
+pnm_status pnm_segment_batch_d(pnm_ctx_t ctx, int32_t nvol, const float* const* sdf,
+                               int32_t* const* seg, const int32_t* dims, const float spacing[3],
+                               int32_t* n_pores_d, int32_t* n_solids_d, int32_t flags);
+/* Launch shape used by the last batched call: threads per workgroup, persistent workgroups per
+ * CU, total workgroups; tuned_ms = the winning time of the last autotune (0 if none). */
+pnm_status pnm_batch_config(pnm_ctx_t ctx, int32_t* block, int32_t* per_cu, int32_t* groups,
+                            float* tuned_ms);
+
 /* ---- stage-level device entry points (parity tests of single kernels; no reference
  * counterpart as separate calls: each is one step of segmentVolumeView) ---- */
 

@@ -14,7 +14,11 @@
 #include <pybind11/stl.h>
 
 #include <array>
+#include <cstdint>
 #include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -119,6 +123,73 @@ void warn_truncated(int64_t total) {
   }
 }
 
+// ---- SDFReader.read_vti: sdf_reader.cpp:28-152 (raw appended VTI, Float32 data after a UInt64
+// byte count, header in the first 8 KiB, WholeExtent as point extents) ----
+std::string vti_attr(const std::string& content, const std::string& name) {
+  const size_t pos = content.find(name + "=\"");
+  if (pos == std::string::npos) return "";
+  const size_t start = pos + name.size() + 2;
+  const size_t end = content.find('"', start);
+  if (end == std::string::npos) return "";
+  return content.substr(start, end - start);
+}
+
+py::tuple read_vti(const std::string& filename) {
+  std::ifstream file(filename, std::ios::binary);
+  if (!file.is_open()) throw std::runtime_error("Could not open file: " + filename);
+  std::vector<char> buffer(8192);
+  file.read(buffer.data(), (std::streamsize)buffer.size());
+  const std::string header(buffer.data(), (size_t)file.gcount());
+  const std::string ext = vti_attr(header, "WholeExtent");
+  if (ext.empty()) throw std::runtime_error("Could not find WholeExtent in VTI header");
+  int x1 = 0, x2 = -1, y1 = 0, y2 = -1, z1 = 0, z2 = -1;
+  std::stringstream se(ext);
+  se >> x1 >> x2 >> y1 >> y2 >> z1 >> z2;
+  const int64_t nx = (int64_t)x2 - x1 + 1, ny = (int64_t)y2 - y1 + 1, nz = (int64_t)z2 - z1 + 1;
+  if (nx < 0 || ny < 0 || nz < 0) throw std::runtime_error("negative WholeExtent in VTI header");
+  float org[3] = {0.0f, 0.0f, 0.0f}, spc[3] = {0.0f, 0.0f, 0.0f};
+  {
+    std::stringstream so(vti_attr(header, "Origin"));
+    so >> org[0] >> org[1] >> org[2];
+    std::stringstream ss(vti_attr(header, "Spacing"));
+    ss >> spc[0] >> spc[1] >> spc[2];
+  }
+  const std::string tag = "<AppendedData encoding=\"raw\">";
+  const size_t tag_pos = header.find(tag);
+  if (tag_pos == std::string::npos) throw std::runtime_error("Could not find AppendedData tag");
+  const size_t us = header.find('_', tag_pos);
+  if (us == std::string::npos)
+    throw std::runtime_error("Could not find binary data marker '_' in first 8KB");
+  file.clear();
+  file.seekg((std::streamoff)(us + 1), std::ios::beg);
+  uint64_t data_bytes = 0;
+  file.read(reinterpret_cast<char*>(&data_bytes), sizeof(uint64_t));
+  if (!file) throw std::runtime_error("Failed to read data_bytes (UInt64) from file.");
+  py::print("Read data length:", data_bytes);
+  const uint64_t expected = (uint64_t)(nx * ny * nz) * sizeof(float);
+  if (data_bytes != expected)
+    std::cerr << "Warning: Data length in file (" << data_bytes
+              << ") does not match expected size from resolution (" << expected << ")."
+              << std::endl;
+  py::array_t<float> arr({(py::ssize_t)nz, (py::ssize_t)ny, (py::ssize_t)nx});
+  float* dst = arr.mutable_data();
+  std::fill(dst, dst + nx * ny * nz, 0.0f);
+  // the reference reads data_bytes into a buffer of `expected` bytes; never read past it
+  const uint64_t want = data_bytes < expected ? data_bytes : expected;
+  file.read(reinterpret_cast<char*>(dst), (std::streamsize)want);
+  if ((uint64_t)file.gcount() != want)
+    std::cerr << "Warning: Could not read all data. Read " << file.gcount() << " bytes."
+              << std::endl;
+  py::list o, sp;
+  for (int a = 2; a >= 0; --a) {
+    o.append((double)org[a]);
+    sp.append((double)spc[a]);
+  }
+  return py::make_tuple(arr, o, sp);
+}
+
+struct SDFReaderTag {};
+
 }  // namespace
 
 PYBIND11_MODULE(_pnm, m) {
@@ -130,6 +201,11 @@ PYBIND11_MODULE(_pnm, m) {
       "set_device", [](int d) { g_device = d; }, py::arg("device"),
       "Select the HIP device used by subsequent calls on every thread (default: $PNM_DEVICE or 0).");
   m.def("device_count", []() { return pnm_device_count(); });
+
+  // pnm_bindings.cpp:87-102
+  py::class_<SDFReaderTag>(m, "SDFReader")
+      .def_static("read_vti", &read_vti, py::arg("filename"),
+                  "Reads VTI; returns (sdf_3d[nz,ny,nx], origin_zyx, spacing_zyx)");
 
   py::class_<pnm_pore_t>(m, "Pore")
       .def(py::init<>())

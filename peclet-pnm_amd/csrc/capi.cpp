@@ -223,6 +223,7 @@ void pnm_ctx_destroy(pnm_ctx_t ctx) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->batch_graph) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(c->batch_graph));
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (prev >= 0) (void)hipSetDevice(prev);
@@ -400,6 +401,39 @@ pnm_status pnm_extract_pore_network_d(pnm_ctx_t ctx, const float* sdf_d, const i
     *npairs = np2;
     if (k < 0) fail(PNM_ECAPACITY, "pores buffer too small: need " + std::to_string(-k));
     if (povf) fail(PNM_ECAPACITY, "pairs buffer too small: need " + std::to_string(np2));
+  });
+}
+
+pnm_status pnm_segment_batch_d(pnm_ctx_t ctx, int32_t nvol, const float* const* sdf,
+                               int32_t* const* seg, const int32_t* dims, const float spacing[3],
+                               int32_t* n_pores_d, int32_t* n_solids_d, int32_t flags) {
+  return guarded([&] {
+    need(nvol >= 0, "nvol must be non-negative");
+    need(spacing != nullptr, "spacing must not be NULL");
+    need(nvol == 0 || (sdf && seg && dims), "sdf, seg and dims arrays are required");
+    const float mn = std::min(spacing[0], std::min(spacing[1], spacing[2]));
+    need(mn > 0.0f, "batched segmentation needs min(spacing) > 0");
+    need((flags & ~(PNM_BATCH_GRAPH | PNM_BATCH_AUTOTUNE)) == 0, "unknown flags");
+    for (int k = 0; k < nvol; ++k) {
+      const Vol v = check_dims(dims + 3 * k);
+      need(v.n > 0, "every volume of a batch must be non-empty");
+      need(sdf[k] && seg[k], "NULL volume pointer in the batch");
+    }
+    Ctx* c = check_ctx(ctx);
+    if (nvol == 0) return;
+    CallScope cs(c);
+    segment_batch_device(c, nvol, sdf, seg, dims, spacing, flags, n_pores_d, n_solids_d);
+  });
+}
+
+pnm_status pnm_batch_config(pnm_ctx_t ctx, int32_t* block, int32_t* per_cu, int32_t* groups,
+                            float* tuned_ms) {
+  return guarded([&] {
+    Ctx* c = check_ctx(ctx);
+    if (block) *block = c->batch_block;
+    if (per_cu) *per_cu = c->batch_per_cu;
+    if (groups) *groups = c->batch_groups;
+    if (tuned_ms) *tuned_ms = c->batch_tuned_ms;
   });
 }
 

@@ -40,4 +40,125 @@ __device__ __forceinline__ int64_t ld_agent64(const int64_t* p) {
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Lexicographic (sdf, flat index) maximum over the voxel and its 26 periodic neighbours — the
+// ascent step of pore_extraction.hpp:280-304 (ties to the larger index; NaN never wins).
+__device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x, int y, int z,
+                                           int nx, int ny, int nz, int ci, float cv) {
+  const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
+  const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
+  const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
+  int best = ci;
+  float bv = cv;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int row = (zs[a] * ny + ys[b]) * nx;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (a == 1 && b == 1 && c == 1) continue;
+        const int ni = row + xs[c];
+        const float nv = sdf[ni];
+        if (nv > bv) {
+          bv = nv;
+          best = ni;
+        } else if (nv == bv && ni > best) {
+          best = ni;
+        }
+      }
+    }
+  return best;
+}
+
+// Lock-free union-find over int32 parents (root = smallest index of its tree): the larger root
+// is hooked under the smaller with atomicMin; a lost race retries with the value it found.
+__device__ __forceinline__ int uf_find(const int* L, int x) {
+  int p = ld_agent(L + x);
+  while (p != x) {
+    x = p;
+    p = ld_agent(L + x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_unite(int* L, int a, int b) {
+  while (true) {
+    a = uf_find(L, a);
+    b = uf_find(L, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(L + b, a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+// Wave-aggregated append to an LDS counter (batched kernels). Every lane must call it.
+__device__ __forceinline__ unsigned wave_append_lds(unsigned* ctr, bool pred, int lane) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return 0;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(m));
+  base = __shfl(base, leader, 64);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return base + (unsigned)__popcll(m & below);
+}
+
+// merge_markers (pore_extraction.hpp:198-224) over x-runs. Every marker carries the index of
+// the start of its x-run within its 64-voxel row chunk. A wave owns one row chunk; for each of
+// the forward rows of the 13-neighbourhood (:204-206) — (y+1, z), (y-1, z+1), (y, z+1),
+// (y+1, z+1) — it unites each of its runs with every run of that row it touches (26-adjacency:
+// columns x-1, x, x+1). One union per touching pair suffices: at column x where both rows are
+// markers (skipped when column x-1 gave the same pair), and at a run's first / last column for
+// the diagonal-only contact. The +x neighbour across chunk (and periodic row) boundaries is the
+// remaining forward relation. The resulting components, hence the component minima, are those of
+// the voxel-level relation.
+// Called by a whole wave for one 64-voxel row chunk (x = chunk start + lane).
+__device__ __forceinline__ void runs_union_chunk(int* L, int nx, int ny, int nz, int x, int y,
+                                                 int z, int lane) {
+  const bool in = (x < nx) && (y < ny);
+  const int i = in ? (z * ny + y) * nx + x : 0;
+  const int my = in ? L[i] : -1;
+  const bool mk = my != -1;
+  const uint64_t m = __ballot(mk);
+  if (!__any(mk)) return;  // wave-uniform
+  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+  const bool run_end = mk && (lane == 63 || !((m >> (lane + 1)) & 1ull));
+  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
+  // +x relation across the chunk start; at x = 0 this is the periodic wrap from x = nx-1
+  if (mk && lane == 0) {
+    const int nl = L[(z * ny + y) * nx + xm];
+    if (nl != -1) uf_unite(L, my, nl);
+  }
+  const int yp = in ? wp1(y, ny) : 0, ym = in ? wm1(y, ny) : 0, zp = wp1(z, nz);
+  const int rows[4] = {(z * ny + yp) * nx, (zp * ny + ym) * nx, (zp * ny + y) * nx,
+                       (zp * ny + yp) * nx};
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = in ? L[rows[r] + x] : -1;
+    const bool cm = c != -1;
+    // previous lane's pair for the duplicate test at column x
+    const int prev_c = __shfl_up(c, 1, 64);
+    if (mk && cm) {
+      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+      if (!dup) uf_unite(L, my, c);
+    }
+    if (mk && !cm) {
+      if (run_start) {
+        const int a = L[rows[r] + xm];
+        if (a != -1) uf_unite(L, my, a);
+      }
+      if (run_end) {
+        const int b = L[rows[r] + xp];
+        if (b != -1) uf_unite(L, my, b);
+      }
+    }
+  }
+}
+
 }  // namespace pnm

@@ -14,36 +14,6 @@ namespace pnm {
 static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz); }
 static const dim3 kVoxBlock(64, 4, 1);
 
-// Lexicographic (sdf, flat index) maximum over the voxel and its 26 periodic neighbours — the
-// ascent step of pore_extraction.hpp:280-304 (ties to the larger index; NaN never wins).
-__device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x, int y, int z,
-                                           int nx, int ny, int nz, int ci, float cv) {
-  const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
-  const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
-  const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
-  int best = ci;
-  float bv = cv;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const int row = (zs[a] * ny + ys[b]) * nx;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (a == 1 && b == 1 && c == 1) continue;
-        const int ni = row + xs[c];
-        const float nv = sdf[ni];
-        if (nv > bv) {
-          bv = nv;
-          best = ni;
-        } else if (nv == bv && ni > best) {
-          best = ni;
-        }
-      }
-    }
-  return best;
-}
-
 // SEG = false: peaks only (extract_pores). SEG = true: also markers, next, band, min_pore init.
 template <bool SEG>
 __global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int nx, int ny,

@@ -15,83 +15,9 @@ static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsign
 static const dim3 kVoxBlock(64, 4, 1);
 
 // ---------------- CCL ----------------
-__device__ __forceinline__ int uf_find(const int* L, int x) {
-  int p = ld_agent(L + x);
-  while (p != x) {
-    x = p;
-    p = ld_agent(L + x);
-  }
-  return x;
-}
-
-__device__ __forceinline__ void uf_unite(int* L, int a, int b) {
-  while (true) {
-    a = uf_find(L, a);
-    b = uf_find(L, b);
-    if (a == b) return;
-    if (a > b) {
-      const int t = a;
-      a = b;
-      b = t;
-    }
-    const int old = atomicMin(L + b, a);
-    if (old == b) return;
-    b = old;
-  }
-}
-
-// merge_markers (pore_extraction.hpp:198-224) over x-runs. Every marker carries the index of
-// the start of its x-run within its 64-voxel row chunk. A wave owns one row chunk; for each of
-// the forward rows of the 13-neighbourhood (:204-206) — (y+1, z), (y-1, z+1), (y, z+1),
-// (y+1, z+1) — it unites each of its runs with every run of that row it touches (26-adjacency:
-// columns x-1, x, x+1). One union per touching pair suffices: at column x where both rows are
-// markers (skipped when column x-1 gave the same pair), and at a run's first / last column for
-// the diagonal-only contact. The +x neighbour across chunk (and periodic row) boundaries is the
-// remaining forward relation. The resulting components, hence the component minima, are those of
-// the voxel-level relation.
 __global__ __launch_bounds__(256) void k_ccl_runs(int* L, int nx, int ny, int nz) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
-  const int lane = threadIdx.x;
-  const bool in = (x < nx) && (y < ny);
-  const int i = in ? (z * ny + y) * nx + x : 0;
-  const int my = in ? L[i] : -1;
-  const bool mk = my != -1;
-  const uint64_t m = __ballot(mk);
-  if (!__any(mk)) return;  // wave-uniform
-  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
-  const bool run_end = mk && (lane == 63 || !((m >> (lane + 1)) & 1ull));
-  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
-  // +x relation across the chunk start; at x = 0 this is the periodic wrap from x = nx-1
-  if (mk && lane == 0) {
-    const int nl = L[(z * ny + y) * nx + xm];
-    if (nl != -1) uf_unite(L, my, nl);
-  }
-  const int yp = in ? wp1(y, ny) : 0, ym = in ? wm1(y, ny) : 0, zp = wp1(z, nz);
-  const int rows[4] = {(z * ny + yp) * nx, (zp * ny + ym) * nx, (zp * ny + y) * nx,
-                       (zp * ny + yp) * nx};
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int c = in ? L[rows[r] + x] : -1;
-    const bool cm = c != -1;
-    // previous lane's pair for the duplicate test at column x
-    const int prev_c = __shfl_up(c, 1, 64);
-    if (mk && cm) {
-      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
-      if (!dup) uf_unite(L, my, c);
-    }
-    if (mk && !cm) {
-      if (run_start) {
-        const int a = L[rows[r] + xm];
-        if (a != -1) uf_unite(L, my, a);
-      }
-      if (run_end) {
-        const int b = L[rows[r] + xp];
-        if (b != -1) uf_unite(L, my, b);
-      }
-    }
-  }
+  runs_union_chunk(L, nx, ny, nz, blockIdx.x * 64 + threadIdx.x, blockIdx.y * 4 + threadIdx.y,
+                   blockIdx.z, threadIdx.x);
 }
 
 // flatten (pore_extraction.hpp:183-194). Component roots (= component minima) are appended to

@@ -65,6 +65,13 @@ struct Ctx {
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
   hipEvent_t take_event();
+  // Batched segmentation: autotuned launch shape and the cached graph of the last batch.
+  bool batch_cfg_valid = false;
+  int batch_block = 1024, batch_per_cu = 2, batch_groups = 0;
+  uint64_t batch_shape = 0;
+  float batch_tuned_ms = 0.0f;
+  void* batch_graph = nullptr;  // hipGraphExec_t
+  uint64_t batch_key = 0;
 };
 
 struct KTimer {
@@ -183,6 +190,11 @@ int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], cons
                        pnm_pore_t* pores_d, int64_t cap_pores, int64_t* total_pores,
                        int32_t* seg, int32_t* pairs_d, int64_t cap_pairs, int64_t* npairs,
                        bool* pairs_overflow);
+
+// ---------- batched segmentation (k_batch.hip) ----------
+void segment_batch_device(Ctx* c, int nvol, const float* const* sdf, int32_t* const* seg,
+                          const int32_t* dims, const float spc[3], int flags, int32_t* out_np,
+                          int32_t* out_ns);
 
 // ---------- network flow (k_flow.hip) ----------
 // Fields are device pointers; ox/oy/oz all null = fully open. Results are malloc'ed host arrays.

@@ -26,8 +26,11 @@ EXPORTS = (
     "pnm_extract_topology", "pnm_extract_pore_network", "pnm_extract_network_flow",
     "pnm_flow_result_free", "pnm_extract_pores_d", "pnm_segment_volume_d",
     "pnm_extract_topology_d", "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
-    "pnm_stage_labels_d", "pnm_stage_roots_d",
+    "pnm_stage_labels_d", "pnm_stage_roots_d", "pnm_segment_batch_d", "pnm_batch_config",
 )
+
+PNM_BATCH_GRAPH = 1
+PNM_BATCH_AUTOTUNE = 2
 
 
 class Pore(ctypes.Structure):
@@ -95,6 +98,11 @@ def lib():
                                                vp, ctypes.POINTER(FlowResult)]
         L.pnm_flow_result_free.argtypes = [ctypes.POINTER(FlowResult)]
         L.pnm_flow_result_free.restype = None
+        L.pnm_segment_batch_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp, vp, vp,
+                                          ctypes.c_int32]
+        L.pnm_segment_batch_d.restype = st
+        L.pnm_batch_config.argtypes = [vp, vp, vp, vp, vp]
+        L.pnm_batch_config.restype = st
         L.pnm_stage_labels_d.argtypes = [vp, vp, vp, vp, vp, vp]
         L.pnm_stage_roots_d.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         for name in ("pnm_extract_pores_d", "pnm_segment_volume_d", "pnm_extract_topology_d",
@@ -200,6 +208,26 @@ class Context:
             cap_p, ctypes.byref(npores), ctypes.byref(tot), _ptr(seg_out), _ptr(pairs_out), cap_q,
             ctypes.byref(npairs)))
         return npores.value, tot.value, npairs.value
+
+    def segment_batch_d(self, sdfs, segs, dims_xyz, spacing_xyz, n_pores=None, n_solids=None,
+                        graph=True, autotune=False):
+        """Segment many device volumes in one persistent launch. sdfs/segs: lists of torch
+        tensors (float32 / int32, contiguous); dims_xyz: list of (nx, ny, nz)."""
+        nvol = len(sdfs)
+        ps = (ctypes.c_void_p * max(nvol, 1))(*[t.data_ptr() for t in sdfs])
+        qs = (ctypes.c_void_p * max(nvol, 1))(*[t.data_ptr() for t in segs])
+        d = np.ascontiguousarray(np.asarray(dims_xyz, dtype=np.int32).reshape(-1))
+        flags = (PNM_BATCH_GRAPH if graph else 0) | (PNM_BATCH_AUTOTUNE if autotune else 0)
+        _check(lib().pnm_segment_batch_d(self.h, nvol, ps, qs, ctypes.c_void_p(d.ctypes.data),
+                                         _f3(spacing_xyz), _ptr(n_pores), _ptr(n_solids),
+                                         flags))
+
+    def batch_config(self):
+        b, pc, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = ctypes.c_float()
+        _check(lib().pnm_batch_config(self.h, ctypes.byref(b), ctypes.byref(pc), ctypes.byref(g),
+                                      ctypes.byref(ms)))
+        return {"block": b.value, "per_cu": pc.value, "groups": g.value, "tuned_ms": ms.value}
 
     def stage_labels_d(self, sdf, dims_xyz, spacing_xyz, labels_out, ccl_out=None):
         _check(lib().pnm_stage_labels_d(self.h, _ptr(sdf), _i3(dims_xyz), _f3(spacing_xyz),
