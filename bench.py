@@ -33,7 +33,7 @@ ALGO_BYTES_PER_VOXEL = {
     "k_pre_peaks": 4,     # read sdf 4
     "k_ccl_runs": 4,      # read marker labels 4 (neighbour rows are cache hits; unions sparse)
     "k_ccl_flatten": 8,   # read + write labels
-    "k_roots_min": 16,    # read sdf 4, labels 4, successor 4; write roots 4
+    "k_roots_min": 12,    # read sdf 4, successor 4; write roots 4
     "k_seg": 16,          # read sdf 4, roots 4, labels 4; write seg 4
     "k_topo_cand": 4,     # read seg 4
 }
@@ -94,6 +94,7 @@ def main():
 
     from peclet.pnm import capi
 
+    torch.zeros(1, device="cuda:0")  # bring up the GPU context before the OpenMP generators
     sdf, desc = make_sdf(args.workload, args.size, args.seed)
     shape = sdf.shape
     n = sdf.size

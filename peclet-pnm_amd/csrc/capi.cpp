@@ -446,41 +446,7 @@ pnm_status pnm_stage_labels_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t d
     Ctx* c = check_ctx(ctx);
     if (v.n == 0) return;
     CallScope cs(c);
-    const float mn = std::min(spacing[0], std::min(spacing[1], spacing[2]));
-    int32_t* next = c->arena.get_n<int32_t>((size_t)v.n);
-    int32_t* min_pore = c->arena.get_n<int32_t>((size_t)v.n);
-    int32_t* min_solid = c->arena.get_n<int32_t>((size_t)v.n);
-    uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)v.n);
-    uint32_t* band = c->arena.get_n<uint32_t>((size_t)v.n);
-    uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)v.n);
-    int32_t* newlab = c->arena.get_n<int32_t>((size_t)v.n);
-    uint32_t* skeys = c->arena.get_n<uint32_t>((size_t)v.n);
-    uint64_t* ctr = c->arena.get_n<uint64_t>(8);
-    fill_u64(c, ctr, 0, 8);
-    PreOut o{};
-    o.labels = labels_d;
-    o.next = next;
-    o.min_pore = min_pore;
-    o.peak_list = peaks;
-    o.band_list = band;
-    o.counters = ctr;
-    o.cap_peaks = v.n;
-    launch_pre(c, sdf_d, v, -1.5f * mn, o);
-    launch_ccl(c, v, labels_d, min_solid, skeys, ctr);
-    if (ccl_only_d)
-      PNM_HIP(hipMemcpyAsync(ccl_only_d, labels_d, sizeof(int32_t) * v.n,
-                             hipMemcpyDeviceToDevice, c->stream));
-    uint64_t h[2];
-    readback(c, ctr, 2, h);
-    uint32_t* cur = band;
-    uint32_t* nxt = band2;
-    int64_t nb = (int64_t)h[1];
-    while (nb > 0) {
-      const int64_t nb2 = flood_sweep(c, v, labels_d, cur, nb, nxt, newlab, ctr + 5);
-      if (nb2 == nb) break;
-      std::swap(cur, nxt);
-      nb = nb2;
-    }
+    labels_device(c, sdf_d, v, spacing, labels_d, ccl_only_d);
     PNM_HIP(hipStreamSynchronize(c->stream));
   });
 }

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
       if (in) {
         W[i] = mk ? i - (lane - s) : -1;
         nxt[i] = nx_i;
-        if (pos_peak) mn[i] = INT_MAX;
+        if (pos_peak) mn[i] = i;  // a basin's first voxel is never after its peak
       }
       const unsigned ps = wave_append_lds(&ctr[0], pos_peak, lane);
       if (pos_peak) pkeys[ps] = (uint32_t)i;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
           }
           W[i] = l;
           root = l == i;
-          if (root) mn[i] = INT_MAX;
+          if (root) mn[i] = i;  // the component's smallest marker; flood may lower it
         }
       }
       const unsigned sidx = wave_append_lds(&ctr[2], root, lane);
@@ -253,7 +253,11 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
         __syncthreads();
         for (unsigned j = tid; j < cnt; j += BS) {
           const int b = newlab[j];
-          if (b != -1) W[cur[j]] = b;
+          if (b != -1) {
+            const int i = (int)cur[j];
+            W[i] = b;
+            if (i < b) atomicMin(mn + b, i);
+          }
         }
         __syncthreads();
         const unsigned kept = ctr[3];
@@ -266,29 +270,24 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
       }
     }
 
-    // ---- P5: roots into W (pore voxels), first-appearance minima ----
+    // ---- P5: roots into W (pore voxels), pore first-appearance minima (voxels before their
+    // peak only; solid minima were completed by the flood) ----
     for (int base = 0; base < n; base += BS) {
       const int i = base + tid;
-      int pk = -1, sk = -1;
-      if (i < n) {
-        if (sdf[i] > 0.0f) {
-          int w = i, st = 0;
-          for (; st < kMaxSteps; ++st) {
-            const int t = nxt[w];
-            if (t == w) break;
-            w = t;
-          }
-          if (st == kMaxSteps && nxt[w] != w) ctr[4] = 1;
-          W[i] = w;
-          pk = w;
-        } else {
-          sk = W[i];
+      int pk = -1;
+      if (i < n && sdf[i] > 0.0f) {
+        int w = i, st = 0;
+        for (; st < kMaxSteps; ++st) {
+          const int t = nxt[w];
+          if (t == w) break;
+          w = t;
         }
+        if (st == kMaxSteps && nxt[w] != w) ctr[4] = 1;
+        W[i] = w;
+        pk = w;
       }
       const int ppk = __shfl_up(pk, 1, 64);
-      const int psk = __shfl_up(sk, 1, 64);
-      if (pk >= 0 && (lane == 0 || ppk != pk)) atomicMin(mn + pk, i);
-      if (sk >= 0 && (lane == 0 || psk != sk)) atomicMin(mn + sk, i);
+      if (pk >= 0 && i < pk && (lane == 0 || ppk != pk)) atomicMin(mn + pk, i);
     }
     __syncthreads();
     unsigned npk = ctr[0];

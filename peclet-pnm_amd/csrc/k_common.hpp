@@ -97,6 +97,26 @@ __device__ __forceinline__ void uf_unite(int* L, int a, int b) {
   }
 }
 
+// Block-ordered staging slot for `pred`: items are ordered by wave, then lane. Every thread of
+// the block calls it (one barrier); `wcnt` is an LDS array of NW words used by this call only.
+template <int NW>
+__device__ __forceinline__ unsigned block_stage_pos(bool pred, unsigned* wcnt, int wave, int lane,
+                                                    unsigned& total) {
+  const uint64_t m = __ballot(pred);
+  if (lane == 0) wcnt[wave] = (unsigned)__popcll(m);
+  __syncthreads();
+  unsigned base = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const unsigned cc = wcnt[k];
+    if (k < wave) base += cc;
+    tot += cc;
+  }
+  total = tot;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return base + (unsigned)__popcll(m & below);
+}
+
 // Wave-aggregated append to an LDS counter (batched kernels). Every lane must call it.
 __device__ __forceinline__ unsigned wave_append_lds(unsigned* ctr, bool pred, int lane) {
   const uint64_t m = __ballot(pred);

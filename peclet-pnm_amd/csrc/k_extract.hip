@@ -14,21 +14,26 @@ namespace pnm {
 static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz); }
 static const dim3 kVoxBlock(64, 4, 1);
 
-// SEG = false: peaks only (extract_pores). SEG = true: also markers, next, band, min_pore init.
+// SEG = false: peaks only (extract_pores). SEG = true: also markers, successor, band, and the
+// first-appearance minimum of every sdf>0 peak initialised to the peak itself (a basin's first
+// voxel is never after its peak). Lists are staged per block in wave/lane order.
 template <bool SEG>
 __global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int nx, int ny,
                                              int nz, float thr, PreOut o) {
+  __shared__ unsigned wc_peak[4], wc_band[4];
   const int x = blockIdx.x * 64 + threadIdx.x;
   const int y = blockIdx.y * 4 + threadIdx.y;
   const int z = blockIdx.z;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, wave = threadIdx.y;
   const bool in = (x < nx) && (y < ny);
-  bool is_peak = false, pos_peak = false, band = false;
+  bool is_peak = false, band = false;
   int i = 0;
+  float cv = 0.0f;
   if (in) {
     i = (z * ny + y) * nx + x;
-    const float cv = sdf[i];
+    cv = sdf[i];
     int nxt = i;
+    bool pos_peak = false;
     if (!(cv <= 0.0f)) {  // extract_pores: `if (cv <= 0.0f) return;` (:90)
       const int best = ascent_best(sdf, x, y, z, nx, ny, nz, i, cv);
       is_peak = (best == i);
@@ -37,14 +42,13 @@ __global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int 
     }
     if (SEG) {
       o.next[i] = nxt;
-      if (pos_peak) o.min_pore[i] = INT_MAX;
+      if (pos_peak) o.min_pore[i] = i;
     }
   }
   if (SEG) {
     // init_markers (:177) with the label of a marker set to the start of its x-run inside this
     // wave's 64-voxel row chunk (the run is connected along x, so the component minimum is
     // unchanged); the CCL then only unites runs.
-    const float cv = in ? sdf[i] : 0.0f;
     const bool mk = in && (cv < thr);
     const uint64_t m = __ballot(mk);
     const uint64_t starts = m & ~(m << 1);
@@ -54,15 +58,20 @@ __global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int 
     if (in) o.labels[i] = mk ? i - (lane - s) : -1;
     band = in && !(cv >= 0.0f) && !mk;  // flood candidates (:245-248)
   }
-  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(o.counters);
-  const uint64_t slot = wave_append(&ctr[0], is_peak, lane);
-  if (is_peak && (int64_t)slot < o.cap_peaks) o.peak_list[slot] = (uint32_t)i;
+  const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  unsigned ptot;
+  const unsigned ppos = block_stage_pos<4>(is_peak, wc_peak, wave, lane, ptot);
+  if (is_peak) o.peak_stage[bid * 256 + ppos] = (uint32_t)i;
+  if (threadIdx.x == 0 && threadIdx.y == 0) o.peak_cnt[bid] = ptot;
   if (SEG) {
-    wave_append(&ctr[4], pos_peak, lane);
-    const uint64_t bs = wave_append(&ctr[1], band, lane);
-    if (band) o.band_list[bs] = (uint32_t)i;
+    unsigned btot;
+    const unsigned bpos = block_stage_pos<4>(band, wc_band, wave, lane, btot);
+    if (band) o.band_stage[bid * 256 + bpos] = (uint32_t)i;
+    if (threadIdx.x == 0 && threadIdx.y == 0) o.band_cnt[bid] = btot;
   }
 }
+
+int64_t pre_blocks(Vol v) { return (int64_t)cdiv(v.nx, 64) * cdiv(v.ny, 4) * v.nz; }
 
 void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o) {
   if (o.labels)

@@ -20,13 +20,14 @@ __global__ __launch_bounds__(256) void k_ccl_runs(int* L, int nx, int ny, int nz
                    blockIdx.z, threadIdx.x);
 }
 
-// flatten (pore_extraction.hpp:183-194). Component roots (= component minima) are appended to
-// the solid key list and get min_solid[root] = INT_MAX for the renumbering.
+// flatten (pore_extraction.hpp:183-194). Component roots (= component minima) are staged per
+// block as the solid keys; min_solid[root] starts at the root itself (the component's smallest
+// marker); only flood-filled voxels with a smaller index can lower it (k_flood_apply).
 __global__ __launch_bounds__(256) void k_ccl_flatten(int* L, int64_t n, int32_t* min_solid,
-                                                     uint32_t* keys,
-                                                     unsigned long long* counter) {
+                                                     uint32_t* stage, uint32_t* cnt) {
+  __shared__ unsigned wc[4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   bool root = false;
   if (i < n) {
     int l = L[i];
@@ -38,30 +39,37 @@ __global__ __launch_bounds__(256) void k_ccl_flatten(int* L, int64_t n, int32_t*
       }
       L[i] = l;
       root = (l == (int)i);
-      if (root) min_solid[i] = INT_MAX;
+      if (root) min_solid[i] = (int)i;
     }
   }
-  const uint64_t s = wave_append(counter, root, lane);
-  if (root) keys[s] = (uint32_t)i;
+  unsigned tot;
+  const unsigned pos = block_stage_pos<4>(root, wc, wave, lane, tot);
+  if (root) stage[(int64_t)blockIdx.x * 256 + pos] = (uint32_t)i;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
-void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_list,
-                uint64_t* counters) {
+void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
+                uint32_t* key_cnt) {
   PNM_LAUNCH(c, "k_ccl_runs", k_ccl_runs, vox_grid(v), kVoxBlock, 0, labels, v.nx, v.ny, v.nz);
   PNM_LAUNCH(c, "k_ccl_flatten", k_ccl_flatten, dim3(cdiv(v.n, 256)), dim3(256), 0, labels, v.n,
-                     min_solid, key_list, reinterpret_cast<unsigned long long*>(counters + 2));
+             min_solid, key_stage, key_cnt);
 }
 
 // ---------------- flood (pore_extraction.hpp:237-269) ----------------
+// One Jacobi sweep over the current band list (count read on the device): the new label is the
+// minimum labelled neighbour of the previous state; voxels still unlabelled are staged per block
+// for the next band list.
 __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__ L, int nx,
                                                      int ny, int nz,
                                                      const uint32_t* __restrict__ band,
-                                                     int64_t nband,
-                                                     uint32_t* __restrict__ band_next,
+                                                     const uint64_t* __restrict__ count,
                                                      int32_t* __restrict__ newlab,
-                                                     unsigned long long* counter) {
+                                                     uint32_t* __restrict__ stage,
+                                                     uint32_t* __restrict__ cnt) {
+  __shared__ unsigned wc[4];
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t nband = (int64_t)*count;
   bool keep = false;
   uint32_t i = 0;
   if (j < nband) {
@@ -88,48 +96,52 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
     newlab[j] = best;
     keep = (best == -1);
   }
-  const uint64_t s = wave_append(counter, keep, lane);
-  if (keep) band_next[s] = i;
+  unsigned tot;
+  const unsigned pos = block_stage_pos<4>(keep, wc, wave, lane, tot);
+  if (keep) stage[(int64_t)blockIdx.x * 256 + pos] = i;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
+// Applies a sweep; a newly labelled voxel before its label's first appearance lowers it
+// (relabel_min for solids, :335-336 — every other voxel of the label comes after the root).
 __global__ void k_flood_apply(int32_t* __restrict__ L, const uint32_t* __restrict__ band,
-                              const int32_t* __restrict__ newlab, int64_t nband) {
+                              const int32_t* __restrict__ newlab,
+                              const uint64_t* __restrict__ count, int32_t* min_solid) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nband) return;
+  if (j >= (int64_t)*count) return;
   const int32_t b = newlab[j];
-  if (b != -1) L[band[j]] = b;
+  if (b != -1) {
+    const uint32_t i = band[j];
+    L[i] = b;
+    if ((int32_t)i < b) atomicMin(min_solid + b, (int32_t)i);
+  }
 }
 
-int64_t flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, int64_t nband,
-                    uint32_t* band_next, int32_t* newlab, uint64_t* counter) {
-  fill_u64(c, counter, 0, 1);
-  PNM_LAUNCH(c, "k_flood_sweep", k_flood_sweep, dim3(cdiv(nband, 256)), dim3(256), 0, labels,
-                     v.nx, v.ny, v.nz, band, nband, band_next, newlab,
-                     reinterpret_cast<unsigned long long*>(counter));
-  PNM_LAUNCH(c, "k_flood_apply", k_flood_apply, dim3(cdiv(nband, 256)), dim3(256), 0, labels, band,
-                     newlab, nband);
-  uint64_t h;
-  readback(c, counter, 1, &h);
-  return (int64_t)h;
+void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uint64_t* count,
+                 int64_t grid_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
+                 uint32_t* stage, uint32_t* stage_cnt, int32_t* min_solid) {
+  const int64_t nb = cdiv(grid_items, 256);
+  PNM_LAUNCH(c, "k_flood_sweep", k_flood_sweep, dim3((unsigned)nb), dim3(256), 0, labels, v.nx,
+             v.ny, v.nz, band, count, newlab, stage, stage_cnt);
+  compact_staged(c, stage, stage_cnt, nb, 256, band_next, count_next);
+  PNM_LAUNCH(c, "k_flood_apply", k_flood_apply, dim3((unsigned)nb), dim3(256), 0, labels, band,
+             newlab, count, min_solid);
 }
 
-// ---------------- roots + first appearance ----------------
-// relabel_min (pore_extraction.hpp:331-337) fused with the root walk. Lanes are consecutive
-// voxels, so a lane whose key equals its left neighbour's key skips its atomicMin (the left lane
-// holds a smaller index for the same key).
+// ---------------- roots + pore first appearance ----------------
+// The capped walk (pore_extraction.hpp:279-309) fused with relabel_min for pores (:333-334):
+// the basin's minimum starts at its peak, so only voxels before the peak issue an atomicMin, and
+// of a run of equal roots in a wave only the first lane does.
 __global__ __launch_bounds__(256) void k_roots_min(const float* __restrict__ sdf, int64_t n,
                                                    const int32_t* __restrict__ next,
-                                                   const int32_t* __restrict__ labels,
                                                    int32_t* __restrict__ roots,
                                                    int32_t* __restrict__ min_pore,
-                                                   int32_t* __restrict__ min_solid,
                                                    unsigned long long* flags, bool do_min) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  int pk = -1, sk = -1;
+  int pk = -1;
   if (i < n) {
-    const float cv = sdf[i];
-    if (cv > 0.0f) {
+    if (sdf[i] > 0.0f) {
       int w = (int)i;
       int s = 0;
       for (; s < kMaxSteps; ++s) {
@@ -142,22 +154,17 @@ __global__ __launch_bounds__(256) void k_roots_min(const float* __restrict__ sdf
       pk = w;
     } else {
       roots[i] = (int)i;
-      sk = labels[i];
     }
   }
   if (!do_min) return;
   const int ppk = __shfl_up(pk, 1, 64);
-  const int psk = __shfl_up(sk, 1, 64);
-  if (pk >= 0 && (lane == 0 || ppk != pk)) atomicMin(min_pore + pk, (int)i);
-  if (sk >= 0 && (lane == 0 || psk != sk)) atomicMin(min_solid + sk, (int)i);
+  if (pk >= 0 && (int)i < pk && (lane == 0 || ppk != pk)) atomicMin(min_pore + pk, (int)i);
 }
 
-void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next,
-                      const int32_t* labels, int32_t* roots, int32_t* min_pore,
-                      int32_t* min_solid, uint64_t* counters, bool do_min) {
+void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
+                      int32_t* min_pore, uint64_t* flag, bool do_min) {
   PNM_LAUNCH(c, "k_roots_min", k_roots_min, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, next,
-                     labels, roots, min_pore, min_solid,
-                     reinterpret_cast<unsigned long long*>(counters + 3), do_min);
+             roots, min_pore, reinterpret_cast<unsigned long long*>(flag), do_min);
 }
 
 // Capped-walk fallback: the root set is no longer the peak set, so redo the pore minima from a
@@ -168,30 +175,33 @@ __global__ void k_min_pore_full(const float* __restrict__ sdf, int64_t n,
   if (i < n && sdf[i] > 0.0f) atomicMin(min_pore + roots[i], (int)i);
 }
 
-void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots,
-                     const int32_t* labels, int32_t* min_pore, int32_t* min_solid) {
-  (void)labels;
-  (void)min_solid;
+void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* min_pore) {
   fill_u32(c, reinterpret_cast<uint32_t*>(min_pore), (uint32_t)INT_MAX, v.n);
   PNM_LAUNCH(c, "k_min_pore_full", k_min_pore_full, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
-                     roots, min_pore);
+             roots, min_pore);
 }
 
 __global__ void k_collect_keys(const float* __restrict__ sdf, int64_t n,
-                               const int32_t* __restrict__ min_pore, uint32_t* keys,
-                               unsigned long long* counter) {
+                               const int32_t* __restrict__ min_pore, uint32_t* stage,
+                               uint32_t* cnt) {
+  __shared__ unsigned wc[4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool key = (i < n) && sdf[i] > 0.0f && min_pore[i] != INT_MAX;
-  const uint64_t s = wave_append(counter, key, lane);
-  if (key) keys[s] = (uint32_t)i;
+  unsigned tot;
+  const unsigned pos = block_stage_pos<4>(key, wc, wave, lane, tot);
+  if (key) stage[(int64_t)blockIdx.x * 256 + pos] = (uint32_t)i;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
 int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_pore,
                           uint32_t* key_list, uint64_t* counter) {
-  fill_u64(c, counter, 0, 1);
-  PNM_LAUNCH(c, "k_collect_keys", k_collect_keys, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
-                     min_pore, key_list, reinterpret_cast<unsigned long long*>(counter));
+  const int64_t nb = cdiv(v.n, 256);
+  uint32_t* stage = c->arena.get_n<uint32_t>((size_t)(nb * 256));
+  uint32_t* cnt = c->arena.get_n<uint32_t>((size_t)nb);
+  PNM_LAUNCH(c, "k_collect_keys", k_collect_keys, dim3((unsigned)nb), dim3(256), 0, sdf, v.n,
+             min_pore, stage, cnt);
+  compact_staged(c, stage, cnt, nb, 256, key_list, counter);
   uint64_t h;
   readback(c, counter, 1, &h);
   return (int64_t)h;
@@ -232,40 +242,31 @@ __global__ void k_assign(const uint32_t* __restrict__ sk, const uint32_t* __rest
     ids[r] = v ? -(int32_t)(rank[j] + 1) : 0;
     if (j == nkeys - 1) *count = rank[j] + (v ? 1u : 0u);
   } else {
-    if (sk[j] != 0xFFFFFFFFu) ids[r] = (int32_t)(j + 1);
-    if (j == nkeys - 1) *count = 0;
+    const bool v = sk[j] != 0xFFFFFFFFu;
+    if (v) ids[r] = (int32_t)(j + 1);
+    if (v && (j == nkeys - 1 || sk[j + 1] == 0xFFFFFFFFu)) *count = (unsigned long long)(j + 1);
   }
 }
 
-static int bits_for(uint64_t maxval) {
-  int b = 0;
-  while (b < 64 && (maxval >> b) != 0) ++b;
-  return b;
-}
-
-int64_t assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys,
-                   int32_t* id_arr, bool solid) {
-  if (nkeys <= 0) return 0;
+// Writes the number of ids assigned into *count_dev (no synchronisation).
+void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, int32_t* id_arr,
+                bool solid, uint64_t* count_dev) {
+  fill_u64(c, count_dev, 0, 1);
+  if (nkeys <= 0) return;
   uint32_t* sk = c->arena.get_n<uint32_t>((size_t)nkeys);
   uint32_t* sv = c->arena.get_n<uint32_t>((size_t)nkeys);
   uint32_t* flag = c->arena.get_n<uint32_t>((size_t)nkeys);
   uint32_t* rank = c->arena.get_n<uint32_t>((size_t)nkeys);
-  uint64_t* cnt = c->arena.get_n<uint64_t>(1);
   const unsigned nb = cdiv(nkeys, 256);
   PNM_LAUNCH(c, "k_gather_first", k_gather_first, dim3(nb), dim3(256), 0, sdf, keys, nkeys,
-                     (const int32_t*)id_arr, solid, sk, sv);
+             (const int32_t*)id_arr, solid, sk, sv);
   radix_sort_u32(c, sk, sv, nkeys, 0, 32);
   if (solid) {
     PNM_LAUNCH(c, "k_solid_flags", k_solid_flags, dim3(nb), dim3(256), 0, sdf, sk, nkeys, flag);
     scan_exclusive_u32(c, flag, rank, nkeys);
   }
-  PNM_LAUNCH(c, "k_assign", k_assign, dim3(nb), dim3(256), 0, sk, sv, flag, rank, nkeys,
-                     solid, id_arr, reinterpret_cast<unsigned long long*>(cnt));
-  if (!solid) return -1;  // caller knows the pore count
-  uint64_t h;
-  readback(c, cnt, 1, &h);
-  (void)bits_for;
-  return (int64_t)h;
+  PNM_LAUNCH(c, "k_assign", k_assign, dim3(nb), dim3(256), 0, sk, sv, flag, rank, nkeys, solid,
+             id_arr, reinterpret_cast<unsigned long long*>(count_dev));
 }
 
 // relabel_seg (pore_extraction.hpp:378-381).

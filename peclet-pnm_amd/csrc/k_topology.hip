@@ -11,7 +11,7 @@
 
 namespace pnm {
 
-constexpr int TOPO_ZB = 4;          // z-slices per block
+constexpr int TOPO_ZB = 16;         // z-slices per block (a 64x4x16 brick)
 constexpr int TOPO_TABLE = 2048;    // LDS hash slots (16 KiB)
 constexpr int TOPO_PROBES = 32;     // linear-probe limit before a key bypasses the table
 constexpr uint64_t kEmpty = ~0ull;
@@ -23,14 +23,23 @@ __device__ __forceinline__ uint32_t hash64(uint64_t k) {
   return (uint32_t)k;
 }
 
+// Each block deduplicates the differing-face pairs of its brick: a lane whose key equals the
+// previous lane's key for the same direction skips (rows along a boundary repeat the pair), the
+// rest go through an LDS hash set; the set is staged per block (no global counter). Keys that
+// find no slot within the probe limit are appended to a small overflow list.
 __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ seg, int nx,
                                                    int ny, int nz, int32_t lo, int bits,
-                                                   uint64_t* __restrict__ cand,
-                                                   unsigned long long* counter) {
+                                                   uint64_t* __restrict__ stage,
+                                                   uint32_t* __restrict__ scnt,
+                                                   uint64_t* __restrict__ overflow,
+                                                   unsigned long long* ocount) {
   __shared__ unsigned long long table[TOPO_TABLE];
+  __shared__ unsigned wc[4];
+  __shared__ unsigned s_base;
   const int tid = threadIdx.y * 64 + threadIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, wave = threadIdx.y;
   for (int s = tid; s < TOPO_TABLE; s += 256) table[s] = kEmpty;
+  if (tid == 0) s_base = 0;
   __syncthreads();
   const int x = blockIdx.x * 64 + threadIdx.x;
   const int y = blockIdx.y * 4 + threadIdx.y;
@@ -38,42 +47,63 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   const int xp = in ? wp1(x, nx) : 0, yp = in ? wp1(y, ny) : 0;
   for (int zz = 0; zz < TOPO_ZB; ++zz) {
     const int z = blockIdx.z * TOPO_ZB + zz;
-    if (!in || z >= nz) continue;
-    const int zp = wp1(z, nz);
-    const int i = (z * ny + y) * nx + x;
-    const int my = seg[i];
-    const int nb[3] = {seg[(z * ny + y) * nx + xp], seg[(z * ny + yp) * nx + x],
-                       seg[(zp * ny + y) * nx + x]};
+    const bool act = in && z < nz;
+    uint64_t key[3] = {kEmpty, kEmpty, kEmpty};
+    if (act) {
+      const int zp = wp1(z, nz);
+      const int i = (z * ny + y) * nx + x;
+      const int my = seg[i];
+      const int nb[3] = {seg[(z * ny + y) * nx + xp], seg[(z * ny + yp) * nx + x],
+                         seg[(zp * ny + y) * nx + x]};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int nl = nb[k];
+        if (my == nl) continue;
+        const int l1 = my < nl ? my : nl, l2 = my < nl ? nl : my;
+        key[k] = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
+                 (uint64_t)((uint32_t)l2 - (uint32_t)lo);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const int nl = nb[k];
-      if (my == nl) continue;
-      const int l1 = my < nl ? my : nl, l2 = my < nl ? nl : my;
-      const uint64_t key = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
-                           (uint64_t)((uint32_t)l2 - (uint32_t)lo);
-      uint32_t h = hash64(key) & (TOPO_TABLE - 1);
+      const uint64_t prev = __shfl_up(key[k], 1, 64);
+      if (key[k] == kEmpty || (lane > 0 && prev == key[k])) continue;
+      uint32_t h = hash64(key[k]) & (TOPO_TABLE - 1);
       bool placed = false;
       for (int p = 0; p < TOPO_PROBES; ++p) {
-        const unsigned long long prev = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
-        if (prev == kEmpty || prev == key) {
+        const unsigned long long cur = table[h];
+        if (cur == key[k]) {
           placed = true;
           break;
         }
+        if (cur == kEmpty) {
+          const unsigned long long prv = atomicCAS(&table[h], kEmpty, (unsigned long long)key[k]);
+          if (prv == kEmpty || prv == key[k]) {
+            placed = true;
+            break;
+          }
+        }
         h = (h + 1) & (TOPO_TABLE - 1);
       }
-      if (!placed) {  // table crowded: append directly (the global unique pass removes repeats)
-        const unsigned long long s = atomicAdd(counter, 1ull);
-        cand[s] = key;
+      if (!placed) {  // crowded table: the global unique pass removes repeats
+        const unsigned long long s = atomicAdd(ocount, 1ull);
+        overflow[s] = key[k];
       }
     }
   }
   __syncthreads();
+  const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   for (int s0 = 0; s0 < TOPO_TABLE; s0 += 256) {
     const uint64_t key = table[s0 + tid];
     const bool has = key != kEmpty;
-    const uint64_t slot = wave_append(counter, has, lane);
-    if (has) cand[slot] = key;
+    unsigned tot;
+    const unsigned pos = block_stage_pos<4>(has, wc, wave, lane, tot);
+    if (has) stage[bid * TOPO_TABLE + s_base + pos] = key;
+    __syncthreads();
+    if (tid == 0) s_base += tot;
+    __syncthreads();
   }
+  if (tid == 0) scnt[bid] = s_base;
 }
 
 __global__ void k_topo_decode(const uint64_t* __restrict__ keys, int64_t count, int32_t lo,
@@ -101,16 +131,24 @@ int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi
     bits = nbits((uint64_t)((int64_t)hi_label - (int64_t)lo_label));
     if (bits < 1) bits = 1;
   }
-  const int64_t maxc = 3 * v.n;
-  uint64_t* cand = c->arena.get_n<uint64_t>((size_t)maxc);
-  uint64_t* cnt = c->arena.get_n<uint64_t>(1);
-  fill_u64(c, cnt, 0, 1);
   const dim3 grid(cdiv(v.nx, 64), cdiv(v.ny, 4), cdiv(v.nz, TOPO_ZB));
+  const int64_t nblk = (int64_t)grid.x * grid.y * grid.z;
+  const int64_t maxc = 3 * v.n;
+  uint64_t* stage = c->arena.get_n<uint64_t>((size_t)(nblk * TOPO_TABLE));
+  uint32_t* scnt = c->arena.get_n<uint32_t>((size_t)nblk);
+  uint64_t* cand = c->arena.get_n<uint64_t>((size_t)maxc);
+  uint64_t* cnt = c->arena.get_n<uint64_t>(2);
+  fill_u64(c, cnt, 0, 2);
+  // overflow keys go to the tail of `cand`: the staged tables never exceed the brick's 3*voxels
+  // faces, and together both never exceed 3n
   PNM_LAUNCH(c, "k_topo_cand", k_topo_cand, grid, dim3(64, 4, 1), 0, seg, v.nx, v.ny, v.nz, lo,
-                     bits, cand, reinterpret_cast<unsigned long long*>(cnt));
-  uint64_t h;
-  readback(c, cnt, 1, &h);
-  const int64_t ncand = (int64_t)h;
+             bits, stage, scnt, cand, reinterpret_cast<unsigned long long*>(cnt + 1));
+  uint64_t h[2];
+  readback(c, cnt, 2, h);
+  const int64_t nover = (int64_t)h[1];
+  compact_staged64(c, stage, scnt, nblk, TOPO_TABLE, cand + nover, cnt);
+  readback(c, cnt, 1, h);
+  const int64_t ncand = nover + (int64_t)h[0];
   if (ncand == 0) {
     *needed = 0;
     return 0;
@@ -121,7 +159,7 @@ int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi
   *needed = nu;
   if (nu > cap) return -1;
   PNM_LAUNCH(c, "k_topo_decode", k_topo_decode, dim3(cdiv(nu, 256)), dim3(256), 0, uniq, nu, lo,
-                     bits, pairs_d);
+             bits, pairs_d);
   return nu;
 }
 

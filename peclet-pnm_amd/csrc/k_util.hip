@@ -238,6 +238,63 @@ void radix_sort_u32(Ctx* c, uint32_t* keys, uint32_t* vals, int64_t n, int begin
   radix_sort_impl<uint32_t>(c, keys, vals, n, begin_bit, end_bit);
 }
 
+// ---------------- block-staged compaction ----------------
+// Producer kernels write the k-th item of block b to stage[b * cap + k] and the item count to
+// counts[b]; this gathers them in block order (deterministic, no contended atomics).
+__global__ void k_gather_staged(const uint32_t* __restrict__ stage,
+                                const uint32_t* __restrict__ counts,
+                                const uint32_t* __restrict__ offs, int64_t cap,
+                                uint32_t* __restrict__ out) {
+  const int64_t b = blockIdx.x;
+  const uint32_t cnt = counts[b];
+  const uint32_t base = offs[b];
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) out[base + k] = stage[b * cap + k];
+}
+
+__global__ void k_gather_staged64(const uint64_t* __restrict__ stage,
+                                  const uint32_t* __restrict__ counts,
+                                  const uint32_t* __restrict__ offs, int64_t cap,
+                                  uint64_t* __restrict__ out) {
+  const int64_t b = blockIdx.x;
+  const uint32_t cnt = counts[b];
+  const uint32_t base = offs[b];
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) out[base + k] = stage[b * cap + k];
+}
+
+__global__ void k_staged_total(const uint32_t* __restrict__ counts,
+                               const uint32_t* __restrict__ offs, int64_t nblocks,
+                               uint64_t* __restrict__ total) {
+  *total = (uint64_t)offs[nblocks - 1] + counts[nblocks - 1];
+}
+
+void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64_t nblocks,
+                    int64_t cap, uint32_t* out, uint64_t* total_dev) {
+  if (nblocks <= 0) {
+    fill_u64(c, total_dev, 0, 1);
+    return;
+  }
+  uint32_t* offs = c->arena.get_n<uint32_t>((size_t)nblocks);
+  scan_exclusive_u32(c, counts, offs, nblocks);
+  PNM_LAUNCH(c, "k_gather_staged", k_gather_staged, dim3((unsigned)nblocks), dim3(64), 0, stage,
+             counts, offs, cap, out);
+  PNM_LAUNCH(c, "k_staged_total", k_staged_total, dim3(1), dim3(1), 0, counts, offs, nblocks,
+             total_dev);
+}
+
+void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
+                      int64_t cap, uint64_t* out, uint64_t* total_dev) {
+  if (nblocks <= 0) {
+    fill_u64(c, total_dev, 0, 1);
+    return;
+  }
+  uint32_t* offs = c->arena.get_n<uint32_t>((size_t)nblocks);
+  scan_exclusive_u32(c, counts, offs, nblocks);
+  PNM_LAUNCH(c, "k_gather_staged", k_gather_staged64, dim3((unsigned)nblocks), dim3(64), 0,
+             stage, counts, offs, cap, out);
+  PNM_LAUNCH(c, "k_staged_total", k_staged_total, dim3(1), dim3(1), 0, counts, offs, nblocks,
+             total_dev);
+}
+
 // ---------------- unique of a sorted list ----------------
 __global__ void k_unique_flags(const uint64_t* __restrict__ k, int64_t n,
                                uint32_t* __restrict__ f) {

@@ -34,14 +34,16 @@ static int64_t finish_pores(Ctx* c, const float* sdf, Vol v, const float org[3],
 
 int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                      pnm_pore_t* out_d, int64_t cap, int64_t* total) {
+  const int64_t nb = pre_blocks(v);
+  uint32_t* stage = c->arena.get_n<uint32_t>((size_t)(nb * 256));
+  uint32_t* cnt = c->arena.get_n<uint32_t>((size_t)nb);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)v.n);
   uint64_t* ctr = c->arena.get_n<uint64_t>(8);
-  fill_u64(c, ctr, 0, 8);
   PreOut o{};
-  o.peak_list = peaks;
-  o.cap_peaks = v.n;
-  o.counters = ctr;
+  o.peak_stage = stage;
+  o.peak_cnt = cnt;
   launch_pre(c, sdf, v, 0.0f, o);
+  compact_staged(c, stage, cnt, nb, 256, peaks, ctr);
   uint64_t h[1];
   readback(c, ctr, 1, h);
   return finish_pores(c, sdf, v, org, spc, peaks, (int64_t)h[0], out_d, cap, total);
@@ -53,79 +55,104 @@ struct SegInternal {
   int64_t npeaks;
 };
 
+// labels_out / ccl_out (nullable): caller buffers receiving the marker labels after the flood /
+// after the CCL; labels_only stops after the flood (stage-level tests).
 static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc[3],
-                                int32_t* seg, int32_t* roots_out) {
+                                int32_t* seg, int32_t* roots_out, int32_t* labels_out = nullptr,
+                                int32_t* ccl_out = nullptr, bool labels_only = false) {
   const int64_t n = v.n;
   const float mn = std::min(spc[0], std::min(spc[1], spc[2]));
   const float thr = -1.5f * mn;  // pore_extraction.hpp:163-164
-  int32_t* labels = c->arena.get_n<int32_t>((size_t)n);
+  const int64_t nbp = pre_blocks(v);
+  const int64_t nb1 = cdiv(n, 256);
+  int32_t* labels = labels_out ? labels_out : c->arena.get_n<int32_t>((size_t)n);
   int32_t* next = c->arena.get_n<int32_t>((size_t)n);
   int32_t* roots = roots_out ? roots_out : c->arena.get_n<int32_t>((size_t)n);
   int32_t* min_pore = c->arena.get_n<int32_t>((size_t)n);
   int32_t* min_solid = c->arena.get_n<int32_t>((size_t)n);
+  uint32_t* stage_a = c->arena.get_n<uint32_t>((size_t)(nbp * 256));
+  uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)(nbp * 256));
+  uint32_t* cnt_a = c->arena.get_n<uint32_t>((size_t)std::max(nbp, nb1));
+  uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)nbp);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)n);
   int32_t* newlab = c->arena.get_n<int32_t>((size_t)n);
   uint32_t* skeys = c->arena.get_n<uint32_t>((size_t)n);
-  uint64_t* ctr = c->arena.get_n<uint64_t>(8);
-  fill_u64(c, ctr, 0, 8);
+  uint64_t* ctr = c->arena.get_n<uint64_t>(64);
+  // ctr: [0] peaks [1] band [2] solid keys [3] cap flag [4] pores [5] solids [8..] band counts
+  fill_u64(c, ctr, 0, 64);
 
   PreOut o{};
   o.labels = labels;
   o.next = next;
   o.min_pore = min_pore;
-  o.peak_list = peaks;
-  o.band_list = band;
-  o.counters = ctr;
-  o.cap_peaks = n;
+  o.peak_stage = stage_a;
+  o.peak_cnt = cnt_a;
+  o.band_stage = stage_b;
+  o.band_cnt = cnt_b;
   launch_pre(c, sdf, v, thr, o);
-  launch_ccl(c, v, labels, min_solid, skeys, ctr);
-  uint64_t h[5];
-  readback(c, ctr, 5, h);
-  const int64_t npeaks = (int64_t)h[0], nband = (int64_t)h[1], nskeys = (int64_t)h[2];
-  int64_t npos = (int64_t)h[4];
+  compact_staged(c, stage_a, cnt_a, nbp, 256, peaks, ctr + 0);
+  compact_staged(c, stage_b, cnt_b, nbp, 256, band, ctr + 8);
+  launch_ccl(c, v, labels, min_solid, stage_a, cnt_a);
+  compact_staged(c, stage_a, cnt_a, nb1, 256, skeys, ctr + 2);
+  if (ccl_out)
+    PNM_HIP(hipMemcpyAsync(ccl_out, labels, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
+                           c->stream));
+  uint64_t h[9];
+  readback(c, ctr, 9, h);
+  const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2], nband = (int64_t)h[8];
 
-  // Jacobi flood until a sweep labels nothing (pore_extraction.hpp:238-269)
-  {
+  // Jacobi flood until a sweep labels nothing (pore_extraction.hpp:238-269). Band lengths live
+  // on the device (ctr[8 + s]); the host checks them every two sweeps.
+  if (nband > 0) {
     uint32_t* cur = band;
     uint32_t* nxt = band2;
-    int64_t nb = nband;
-    int64_t sweeps = 0;
-    while (nb > 0) {
-      const int64_t nb2 = flood_sweep(c, v, labels, cur, nb, nxt, newlab, ctr + 5);
-      if (++sweeps > n + 2) fail(PNM_ENOCONVERGE, "flood did not converge");
-      if (nb2 == nb) break;
-      std::swap(cur, nxt);
-      nb = nb2;
+    int s = 0;
+    while (true) {
+      if (s + 2 >= 56) fail(PNM_ENOCONVERGE, "flood did not converge within 54 sweeps");
+      for (int k = 0; k < 2; ++k, ++s) {
+        flood_sweep(c, v, labels, cur, ctr + 8 + s, nband, nxt, ctr + 8 + s + 1, newlab, stage_b,
+                    cnt_b, min_solid);
+        std::swap(cur, nxt);
+      }
+      uint64_t hh[2];
+      readback(c, ctr + 8 + s - 1, 2, hh);
+      if (hh[1] == hh[0] || hh[1] == 0) break;
     }
   }
 
-  launch_roots_min(c, sdf, v, next, labels, roots, min_pore, min_solid, ctr, true);
+  SegInternal out{};
+  if (labels_only) return out;
+  launch_roots_min(c, sdf, v, next, roots, min_pore, ctr + 3, true);
   readback(c, ctr + 3, 1, h);
   const uint32_t* pkeys = peaks;
   int64_t npk = npeaks;
   if (h[0]) {  // a walk stopped at the 512-step cap on a non-fixed point
-    launch_min_full(c, sdf, v, roots, labels, min_pore, min_solid);
+    launch_min_full(c, sdf, v, roots, min_pore);
     uint32_t* full = c->arena.get_n<uint32_t>((size_t)n);
     npk = collect_keys_full(c, sdf, v, min_pore, full, ctr + 6);
     pkeys = full;
-    npos = npk;
   }
-  assign_ids(c, sdf, pkeys, npk, min_pore, false);
-  const int64_t ns = assign_ids(c, sdf, skeys, nskeys, min_solid, true);
+  assign_ids(c, sdf, pkeys, npk, min_pore, false, ctr + 4);
+  assign_ids(c, sdf, skeys, nskeys, min_solid, true, ctr + 5);
   launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
-  SegInternal s;
-  s.r.n_pores = (int32_t)npos;
-  s.r.n_solids = (int32_t)ns;
-  s.peaks = peaks;
-  s.npeaks = npeaks;
-  return s;
+  readback(c, ctr + 4, 2, h);
+  out.r.n_pores = (int32_t)h[0];
+  out.r.n_solids = (int32_t)h[1];
+  out.peaks = peaks;
+  out.npeaks = npeaks;
+  return out;
 }
 
 SegResult segment_device(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,
                          int32_t* roots_out) {
   return segment_core(c, sdf, v, spc, seg, roots_out).r;
+}
+
+void labels_device(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* labels,
+                   int32_t* ccl_only) {
+  segment_core(c, sdf, v, spc, nullptr, nullptr, labels, ccl_only, true);
 }
 
 SegPeaks segment_with_peaks(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,

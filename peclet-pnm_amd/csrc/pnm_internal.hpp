@@ -116,44 +116,52 @@ void radix_sort_u64(Ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int begin
                     int end_bit);
 void radix_sort_u32(Ctx* c, uint32_t* keys, uint32_t* vals, int64_t n, int begin_bit,
                     int end_bit);
+// Gathers block-staged items (stage[b*cap + k], k < counts[b]) in block order into out; the
+// total lands in *total_dev (device word).
+void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64_t nblocks,
+                    int64_t cap, uint32_t* out, uint64_t* total_dev);
+void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
+                      int64_t cap, uint64_t* out, uint64_t* total_dev);
 // Writes keys[i] with keys[i] != keys[i-1] to out; returns the count (synchronises).
 int64_t unique_sorted_u64(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n);
 
 // ---------- plain pipeline kernels (k_extract.hip, k_segment.hip, k_topology.hip) ----------
 struct PreOut {
-  int32_t* labels;       // markers (i or -1)        [n], nullable
-  int32_t* next;         // ascent successor         [n], nullable
-  int32_t* min_pore;     // set to INT_MAX at sdf>0 peaks [n], nullable
-  uint32_t* peak_list;   // peak voxel indices (extract_pores predicate) [cap_peaks]
-  uint32_t* band_list;   // flood candidates [n], nullable
-  uint64_t* counters;    // [0] peaks, [1] band
-  int64_t cap_peaks;
+  int32_t* labels;        // markers (x-run start index) or -1 [n], nullable
+  int32_t* next;          // ascent successor                  [n], nullable
+  int32_t* min_pore;      // set to the peak's own index at sdf>0 peaks [n], nullable
+  uint32_t* peak_stage;   // per-block staged peaks (extract_pores predicate), 256 per block
+  uint32_t* peak_cnt;     // per-block peak counts
+  uint32_t* band_stage;   // per-block staged flood candidates, nullable
+  uint32_t* band_cnt;     // per-block band counts, nullable
 };
+// Number of blocks (staging slots of 256) launch_pre uses for a volume.
+int64_t pre_blocks(Vol v);
 void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o);
 void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                       const uint32_t* peaks, int64_t count, pnm_pore_t* out);
 
-// CCL of markers: union (13 forward neighbours) + flatten; flatten appends component roots to
-// key_list, sets min_solid[root] = INT_MAX. counters[2] = number of roots.
-void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_list,
-                uint64_t* counters);
-// One Jacobi flood sweep over the band list; returns the new band count (synchronises).
-int64_t flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, int64_t nband,
-                    uint32_t* band_next, int32_t* newlab, uint64_t* counter);
-// Roots by chasing `next` (cap 512) for sdf>0 voxels + first-appearance atomicMin for pores and
-// labelled solids. counters[3] = capped-walk flag.
-void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next,
-                      const int32_t* labels, int32_t* roots, int32_t* min_pore,
-                      int32_t* min_solid, uint64_t* counters, bool do_min);
-// Fallback when a walk hits the 512-step cap at a non-fixed point: full reinitialisation.
-void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots,
-                     const int32_t* labels, int32_t* min_pore, int32_t* min_solid);
+// CCL of markers: x-run unions + flatten; flatten stages the component roots (solid keys) per
+// 256-voxel block and sets min_solid[root] = root.
+void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
+                uint32_t* key_cnt);
+// One Jacobi flood sweep over the band list whose length is the device word *count (grid sized
+// for grid_items >= that length); writes the remaining band list and its length.
+void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uint64_t* count,
+                 int64_t grid_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
+                 uint32_t* stage, uint32_t* stage_cnt, int32_t* min_solid);
+// Roots by chasing `next` (cap 512) for sdf>0 voxels + pore first-appearance atomicMin; *flag is
+// set when a walk stopped at the cap on a non-fixed point.
+void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
+                      int32_t* min_pore, uint64_t* flag, bool do_min);
+// Capped-walk fallback: full reinitialisation of the pore minima.
+void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* min_pore);
 int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_pore,
                           uint32_t* key_list, uint64_t* counter);
 // Ranks keys by their minimum voxel; writes ids into id_arr[key] (pores: rank+1; solids:
-// -(rank+1) when sdf[first] <= 0 else 0). Returns the number of ids assigned.
-int64_t assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys,
-                   int32_t* id_arr, bool solid);
+// -(rank+1) when sdf[first] <= 0 else 0) and the number of ids into *count_dev.
+void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, int32_t* id_arr,
+                bool solid, uint64_t* count_dev);
 void launch_seg(Ctx* c, const float* sdf, Vol v, const int32_t* roots, const int32_t* labels,
                 const int32_t* pore_id, const int32_t* solid_id, int32_t* seg);
 // Roots for every voxel (flow basins) or sdf>0 only, computed stand-alone.
@@ -173,6 +181,9 @@ struct SegResult {
 };
 SegResult segment_device(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* seg,
                          int32_t* roots_out);
+// Marker labels after CCL (ccl_only, nullable) and after the flood (labels).
+void labels_device(Ctx* c, const float* sdf, Vol v, const float spc[3], int32_t* labels,
+                   int32_t* ccl_only);
 // Segmentation that also hands back the (unsorted) device peak list of the extract_pores
 // predicate; the sdf>0 entries are exactly the self-roots.
 struct SegPeaks {

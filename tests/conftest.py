@@ -19,7 +19,11 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def capi_ctx():
-    """A libpnm_hip context on device 0 (GPU tests only)."""
+    """A libpnm_hip context on device 0 (GPU tests only). torch's GPU context is brought up
+    first, before any OpenMP region of the geometry generators runs in this process."""
+    import torch
+
+    torch.zeros(1, device="cuda")
     from peclet.pnm import capi
 
     ctx = capi.Context(0)
