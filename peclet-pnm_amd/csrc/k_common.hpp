@@ -72,11 +72,16 @@ __device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x,
 
 // Lock-free union-find over int32 parents (root = smallest index of its tree): the larger root
 // is hooked under the smaller with atomicMin; a lost race retries with the value it found.
-__device__ __forceinline__ int uf_find(const int* L, int x) {
-  int p = ld_agent(L + x);
+// find() reads through the L1 and halves the path with plain stores: every value it can read or
+// write is an ancestor of the node, so a stale read only shortens progress, never connectivity,
+// and the atomicMin results (coherent) decide every hook.
+__device__ __forceinline__ int uf_find(int* L, int x) {
+  int p = L[x];
   while (p != x) {
+    const int g = L[p];
+    if (g != p) L[x] = g;
     x = p;
-    p = ld_agent(L + x);
+    p = g;
   }
   return x;
 }

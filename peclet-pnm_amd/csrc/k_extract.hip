@@ -14,72 +14,149 @@ namespace pnm {
 static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz); }
 static const dim3 kVoxBlock(64, 4, 1);
 
-// SEG = false: peaks only (extract_pores). SEG = true: also markers, successor, band, and the
-// first-appearance minimum of every sdf>0 peak initialised to the peak itself (a basin's first
-// voxel is never after its peak). Lists are staged per block in wave/lane order.
+// One pass over the SDF, LDS-tiled: a 512-thread block owns a 64(x) x 8(y) column tile and
+// marches PRE_ZC slices in z, keeping three haloed slices (66 x 10) in an LDS ring, so every SDF
+// value is read from memory about once. Per voxel:
+//   - the 26-neighbour lexicographic (value, index) maximum (pore_extraction.hpp:93-104 and
+//     :280-304): peak of the extract_pores predicate (`cv <= 0` excluded, :90) and the ascent
+//     successor;
+//   - SEG only: marker label = start of its x-run inside the wave's 64-voxel row chunk
+//     (init_markers :177; the run is connected along x, so component minima are unchanged),
+//     flood-band membership (:245-248), and the first-appearance minimum of each sdf>0 peak
+//     seeded with the peak's own index (basin voxels before it lower it later, in the chase).
+// Peaks and band voxels are staged per block in (slice, wave, lane) order.
+constexpr int PRE_TY = 8;
+constexpr int PRE_ZC = 16;
+constexpr int PRE_CAP = 64 * PRE_TY * PRE_ZC;  // staging slots per block
+constexpr int PRE_LX = 66, PRE_LY = PRE_TY + 2;
+
+__device__ __forceinline__ int wrapm(int v, int N) { return ((v % N) + N) % N; }
+
 template <bool SEG>
-__global__ __launch_bounds__(256) void k_pre(const float* __restrict__ sdf, int nx, int ny,
-                                             int nz, float thr, PreOut o) {
-  __shared__ unsigned wc_peak[4], wc_band[4];
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
+__global__ __launch_bounds__(64 * PRE_TY) void k_pre(const float* __restrict__ sdf, int nx,
+                                                     int ny, int nz, float thr, PreOut o) {
+  __shared__ float ring[3][PRE_LY][PRE_LX];
+  __shared__ unsigned wc_peak[PRE_TY], wc_band[PRE_TY];
+  __shared__ unsigned s_pbase, s_bbase;
   const int lane = threadIdx.x, wave = threadIdx.y;
-  const bool in = (x < nx) && (y < ny);
-  bool is_peak = false, band = false;
-  int i = 0;
-  float cv = 0.0f;
-  if (in) {
-    i = (z * ny + y) * nx + x;
-    cv = sdf[i];
-    int nxt = i;
-    bool pos_peak = false;
-    if (!(cv <= 0.0f)) {  // extract_pores: `if (cv <= 0.0f) return;` (:90)
-      const int best = ascent_best(sdf, x, y, z, nx, ny, nz, i, cv);
-      is_peak = (best == i);
-      if (cv > 0.0f) nxt = best;
-      pos_peak = is_peak && (cv > 0.0f);
+  const int tid = wave * 64 + lane;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * PRE_TY, z0 = blockIdx.z * PRE_ZC;
+  const int x = x0 + lane, y = y0 + wave;
+  const bool col_in = (x < nx) && (y < ny);
+  const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  // wrapped global coordinates of this thread's column and its x/y neighbours
+  const int xs[3] = {wrapm(x - 1, nx), wrapm(x, nx), wrapm(x + 1, nx)};
+  const int ys[3] = {wrapm(y - 1, ny), wrapm(y, ny), wrapm(y + 1, ny)};
+
+  auto load_slice = [&](int zg, int slot) {
+    const int zw = wrapm(zg, nz);
+    const float* __restrict__ plane = sdf + (int64_t)zw * ny * nx;
+    for (int e = tid; e < PRE_LX * PRE_LY; e += 64 * PRE_TY) {
+      const int ly = e / PRE_LX, lx = e - ly * PRE_LX;
+      const int gx = wrapm(x0 - 1 + lx, nx), gy = wrapm(y0 - 1 + ly, ny);
+      ring[slot][ly][lx] = plane[(int64_t)gy * nx + gx];
+    }
+  };
+  if (tid == 0) {
+    s_pbase = 0;
+    s_bbase = 0;
+  }
+  load_slice(z0 - 1, 0);
+  load_slice(z0, 1);
+  const int zend = min(z0 + PRE_ZC, nz);
+  for (int z = z0; z < zend; ++z) {
+    const int rel = z - z0;
+    load_slice(z + 1, (rel + 2) % 3);
+    __syncthreads();
+    const int sm1 = rel % 3, s0 = (rel + 1) % 3, sp1 = (rel + 2) % 3;
+    const int slots[3] = {sm1, s0, sp1};
+    const int zs[3] = {wrapm(z - 1, nz), z, wrapm(z + 1, nz)};
+    bool is_peak = false, band = false;
+    int i = 0;
+    float cv = 0.0f;
+    if (col_in) {
+      i = (z * ny + y) * nx + x;
+      cv = ring[s0][wave + 1][lane + 1];
+      int nxt = i;
+      bool pos_peak = false;
+      if (!(cv <= 0.0f)) {
+        int best = i;
+        float bv = cv;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            const int row = (zs[a] * ny + ys[b]) * nx;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              if (a == 1 && b == 1 && c == 1) continue;
+              const float nv = ring[slots[a]][wave + b][lane + c];
+              const int ni = row + xs[c];
+              if (nv > bv) {
+                bv = nv;
+                best = ni;
+              } else if (nv == bv && ni > best) {
+                best = ni;
+              }
+            }
+          }
+        is_peak = (best == i);
+        if (cv > 0.0f) nxt = best;
+        pos_peak = is_peak && (cv > 0.0f);
+      }
+      if (SEG) {
+        o.next[i] = nxt;
+        if (pos_peak) o.min_pore[i] = i;
+      }
     }
     if (SEG) {
-      o.next[i] = nxt;
-      if (pos_peak) o.min_pore[i] = i;
+      const bool mk = col_in && (cv < thr);
+      const uint64_t m = __ballot(mk);
+      const uint64_t starts = m & ~(m << 1);
+      const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+      const uint64_t smask = starts & upto;
+      const int st = smask ? 63 - __clzll((long long)smask) : lane;
+      if (col_in) o.labels[i] = mk ? i - (lane - st) : -1;
+      band = col_in && !(cv >= 0.0f) && !mk;
+    }
+    unsigned ptot;
+    const unsigned ppos = block_stage_pos<PRE_TY>(is_peak, wc_peak, wave, lane, ptot);
+    if (is_peak) o.peak_stage[bid * PRE_CAP + s_pbase + ppos] = (uint32_t)i;
+    unsigned btot = 0;
+    if (SEG) {
+      const unsigned bpos = block_stage_pos<PRE_TY>(band, wc_band, wave, lane, btot);
+      if (band) o.band_stage[bid * PRE_CAP + s_bbase + bpos] = (uint32_t)i;
+    }
+    __syncthreads();  // ring slot and the wave counts are reused by the next slice
+    if (tid == 0) {
+      s_pbase += ptot;
+      s_bbase += btot;
     }
   }
-  if (SEG) {
-    // init_markers (:177) with the label of a marker set to the start of its x-run inside this
-    // wave's 64-voxel row chunk (the run is connected along x, so the component minimum is
-    // unchanged); the CCL then only unites runs.
-    const bool mk = in && (cv < thr);
-    const uint64_t m = __ballot(mk);
-    const uint64_t starts = m & ~(m << 1);
-    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
-    const uint64_t sm = starts & upto;
-    const int s = sm ? 63 - __clzll((long long)sm) : lane;
-    if (in) o.labels[i] = mk ? i - (lane - s) : -1;
-    band = in && !(cv >= 0.0f) && !mk;  // flood candidates (:245-248)
-  }
-  const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  unsigned ptot;
-  const unsigned ppos = block_stage_pos<4>(is_peak, wc_peak, wave, lane, ptot);
-  if (is_peak) o.peak_stage[bid * 256 + ppos] = (uint32_t)i;
-  if (threadIdx.x == 0 && threadIdx.y == 0) o.peak_cnt[bid] = ptot;
-  if (SEG) {
-    unsigned btot;
-    const unsigned bpos = block_stage_pos<4>(band, wc_band, wave, lane, btot);
-    if (band) o.band_stage[bid * 256 + bpos] = (uint32_t)i;
-    if (threadIdx.x == 0 && threadIdx.y == 0) o.band_cnt[bid] = btot;
+  __syncthreads();
+  if (tid == 0) {
+    o.peak_cnt[bid] = s_pbase;
+    if (SEG) o.band_cnt[bid] = s_bbase;
   }
 }
 
-int64_t pre_blocks(Vol v) { return (int64_t)cdiv(v.nx, 64) * cdiv(v.ny, 4) * v.nz; }
+static dim3 pre_grid(Vol v) {
+  return dim3(cdiv(v.nx, 64), cdiv(v.ny, PRE_TY), cdiv(v.nz, PRE_ZC));
+}
+
+int64_t pre_blocks(Vol v) {
+  const dim3 g = pre_grid(v);
+  return (int64_t)g.x * g.y * g.z;
+}
+int64_t pre_stage_cap() { return PRE_CAP; }
 
 void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o) {
   if (o.labels)
-    PNM_LAUNCH(c, "k_pre", k_pre<true>, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz,
-                       thr, o);
-  else
-    PNM_LAUNCH(c, "k_pre_peaks", k_pre<false>, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz,
+    PNM_LAUNCH(c, "k_pre", k_pre<true>, pre_grid(v), dim3(64, PRE_TY), 0, sdf, v.nx, v.ny, v.nz,
                thr, o);
+  else
+    PNM_LAUNCH(c, "k_pre_peaks", k_pre<false>, pre_grid(v), dim3(64, PRE_TY), 0, sdf, v.nx, v.ny,
+               v.nz, thr, o);
 }
 
 // Squared-positive-SDF weighted centroid, dz -> dy -> dx order, FP32

@@ -34,8 +34,8 @@ static int64_t finish_pores(Ctx* c, const float* sdf, Vol v, const float org[3],
 
 int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                      pnm_pore_t* out_d, int64_t cap, int64_t* total) {
-  const int64_t nb = pre_blocks(v);
-  uint32_t* stage = c->arena.get_n<uint32_t>((size_t)(nb * 256));
+  const int64_t nb = pre_blocks(v), scap = pre_stage_cap();
+  uint32_t* stage = c->arena.get_n<uint32_t>((size_t)(nb * scap));
   uint32_t* cnt = c->arena.get_n<uint32_t>((size_t)nb);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)v.n);
   uint64_t* ctr = c->arena.get_n<uint64_t>(8);
@@ -43,7 +43,7 @@ int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const 
   o.peak_stage = stage;
   o.peak_cnt = cnt;
   launch_pre(c, sdf, v, 0.0f, o);
-  compact_staged(c, stage, cnt, nb, 256, peaks, ctr);
+  compact_staged(c, stage, cnt, nb, scap, peaks, ctr);
   uint64_t h[1];
   readback(c, ctr, 1, h);
   return finish_pores(c, sdf, v, org, spc, peaks, (int64_t)h[0], out_d, cap, total);
@@ -63,17 +63,19 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   const int64_t n = v.n;
   const float mn = std::min(spc[0], std::min(spc[1], spc[2]));
   const float thr = -1.5f * mn;  // pore_extraction.hpp:163-164
-  const int64_t nbp = pre_blocks(v);
+  const int64_t nbp = pre_blocks(v), pcap = pre_stage_cap();
   const int64_t nb1 = cdiv(n, 256);
   int32_t* labels = labels_out ? labels_out : c->arena.get_n<int32_t>((size_t)n);
   int32_t* next = c->arena.get_n<int32_t>((size_t)n);
   int32_t* roots = roots_out ? roots_out : c->arena.get_n<int32_t>((size_t)n);
   int32_t* min_pore = c->arena.get_n<int32_t>((size_t)n);
   int32_t* min_solid = c->arena.get_n<int32_t>((size_t)n);
-  uint32_t* stage_a = c->arena.get_n<uint32_t>((size_t)(nbp * 256));
-  uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)(nbp * 256));
+  // staging: k_pre uses nbp blocks of pcap slots; flatten and the flood use nb1 blocks of 256
+  const int64_t stage_words = std::max(nbp * pcap, nb1 * 256);
+  uint32_t* stage_a = c->arena.get_n<uint32_t>((size_t)stage_words);
+  uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* cnt_a = c->arena.get_n<uint32_t>((size_t)std::max(nbp, nb1));
-  uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)nbp);
+  uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nbp, nb1));
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)n);
@@ -92,8 +94,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   o.band_stage = stage_b;
   o.band_cnt = cnt_b;
   launch_pre(c, sdf, v, thr, o);
-  compact_staged(c, stage_a, cnt_a, nbp, 256, peaks, ctr + 0);
-  compact_staged(c, stage_b, cnt_b, nbp, 256, band, ctr + 8);
+  compact_staged(c, stage_a, cnt_a, nbp, pcap, peaks, ctr + 0);
+  compact_staged(c, stage_b, cnt_b, nbp, pcap, band, ctr + 8);
   launch_ccl(c, v, labels, min_solid, stage_a, cnt_a);
   compact_staged(c, stage_a, cnt_a, nb1, 256, skeys, ctr + 2);
   if (ccl_out)

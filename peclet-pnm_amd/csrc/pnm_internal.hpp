@@ -130,13 +130,14 @@ struct PreOut {
   int32_t* labels;        // markers (x-run start index) or -1 [n], nullable
   int32_t* next;          // ascent successor                  [n], nullable
   int32_t* min_pore;      // set to the peak's own index at sdf>0 peaks [n], nullable
-  uint32_t* peak_stage;   // per-block staged peaks (extract_pores predicate), 256 per block
+  uint32_t* peak_stage;   // per-block staged peaks (extract_pores predicate), pre_stage_cap()
   uint32_t* peak_cnt;     // per-block peak counts
   uint32_t* band_stage;   // per-block staged flood candidates, nullable
   uint32_t* band_cnt;     // per-block band counts, nullable
 };
-// Number of blocks (staging slots of 256) launch_pre uses for a volume.
+// Number of blocks launch_pre uses for a volume, and the staging slots per block.
 int64_t pre_blocks(Vol v);
+int64_t pre_stage_cap();
 void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o);
 void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                       const uint32_t* peaks, int64_t count, pnm_pore_t* out);

@@ -5,4 +5,7 @@ Drop-in for the reference package's module (packaging/pnm_init.py:7 re-exports `
 ``extract_network_flow``, ``Pore``, ``SDFReader`` and ``execution_space``.
 """
 
-from ._pnm import *  # noqa: F401,F403
+from ._runtime import pin_hip_runtime as _pin
+
+_pin()
+from ._pnm import *  # noqa: F401,F403,E402

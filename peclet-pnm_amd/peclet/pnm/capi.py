@@ -60,6 +60,9 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} missing: run `python peclet-pnm_amd/build.py`")
+        from ._runtime import pin_hip_runtime
+
+        pin_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
         st = ctypes.c_int

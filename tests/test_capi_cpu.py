@@ -67,8 +67,9 @@ def test_validation_before_device_work():
     assert st == capi.PNM_EINVAL and b"openness" in L.pnm_last_error()
 
 
-@pytest.mark.skipif(capi.lib().pnm_device_count() > 0, reason="a GPU is visible")
 def test_context_without_gpu_fails_loudly():
+    if capi.lib().pnm_device_count() > 0:
+        pytest.skip("a GPU is visible")
     h = ctypes.c_void_p()
     st = capi.lib().pnm_ctx_create(ctypes.byref(h), 0)
     assert st == capi.PNM_EHIP
@@ -121,3 +122,13 @@ def test_pore_type():
     p.x, p.y, p.z, p.radius = 1.0, 2.0, 3.0, 4.5
     assert (p.x, p.y, p.z, p.radius) == (1.0, 2.0, 3.0, 4.5)
     assert ctypes.sizeof(capi.Pore) == 16
+
+
+def test_single_hip_runtime_in_process():
+    """libpnm_hip.so and torch must share one HIP runtime: two runtimes in one process cannot
+    both bring up the GPU."""
+    capi.lib()
+    import torch  # noqa: F401
+
+    libs = {line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line}
+    assert len(libs) == 1, libs

@@ -207,9 +207,10 @@ def _torch_topology(seg3):
         d = seg3 != nb
         lo = torch.minimum(seg3, nb)[d].to(torch.int64)
         hi = torch.maximum(seg3, nb)[d].to(torch.int64)
-        keys.append(((lo + 2 ** 31) << 32) | (hi + 2 ** 31))
+        # signed (lo, hi) lexicographic order: lo * 2^32 keeps lo's sign, hi is biased unsigned
+        keys.append((lo << 32) + (hi + 2 ** 31))
     k = torch.unique(torch.cat(keys))
-    return torch.stack([(k >> 32) - 2 ** 31, (k & 0xFFFFFFFF) - 2 ** 31], dim=1).to(torch.int32)
+    return torch.stack([k >> 32, (k & 0xFFFFFFFF) - 2 ** 31], dim=1).to(torch.int32)
 
 
 @pytest.mark.parametrize("kind", ["polydisperse", "boolean"])
