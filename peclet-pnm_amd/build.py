@@ -49,6 +49,7 @@ SOURCES = [
     "k_segment.hip",
     "k_topology.hip",
     "k_flow.hip",
+    "k_tile.hip",
     "k_batch.hip",
     "pipeline.hip",
     "capi.cpp",

@@ -9,10 +9,20 @@ namespace pnm {
 
 constexpr int kMaxSteps = 512;  // walk cap, pore_extraction.hpp:278 and :627
 
+// Segmentation tile: 64 (x) x TILE_Y (y) x TILE_Z (z) voxels per 512-thread block.
+constexpr int TILE_Y = 8;
+constexpr int TILE_Z = 16;
+
 // Periodic +-1 neighbours; equal to the reference's ((x%N)+N)%N for x in [-1, N]
 // (pore_extraction.hpp:59-64), including N = 1 and N = 2 where neighbours alias.
 __device__ __forceinline__ int wm1(int x, int N) { return x > 0 ? x - 1 : N - 1; }
 __device__ __forceinline__ int wp1(int x, int N) { return x + 1 < N ? x + 1 : 0; }
+
+// Any v: the reference's wrap. v in [-N, 2N): two compares.
+__device__ __forceinline__ int wrap_any(int v, int N) { return ((v % N) + N) % N; }
+__device__ __forceinline__ int wrap_near(int v, int N) {
+  return v < 0 ? v + N : (v >= N ? v - N : v);
+}
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -69,6 +79,37 @@ __device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x,
     }
   return best;
 }
+
+// (value, flat index) packed into one ordered 64-bit key: the value's bits mapped to an unsigned
+// order in the high word, the index in the low word. `key(nv,ni) > key(bv,best)` is exactly the
+// reference's `nv > bv || (nv == bv && ni > best)` (pore_extraction.hpp:97-99, :296-298) for
+// numbers: -0 is folded onto +0 (they compare equal) and NaN maps to 0, below every number, so a
+// NaN neighbour never wins. A NaN centre needs its own test (nothing beats it).
+__device__ __forceinline__ uint64_t lex_key(float v, uint32_t idx) {
+  uint32_t u = __float_as_uint(v + 0.0f);  // -0 -> +0
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  if (v != v) u = 0u;
+  return ((uint64_t)u << 32) | idx;
+}
+// Inverse of lex_key's value mapping (-0 comes back as +0, NaN as a quiet NaN).
+__device__ __forceinline__ float key_value(uint64_t k) {
+  const uint32_t u = (uint32_t)(k >> 32);
+  if (u == 0u) return __uint_as_float(0x7fc00000u);
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ uint64_t kmax(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+// Walk record of an sdf>0 voxel after its ascent walk inside its tile: target (low word) is the
+// root if the walk is final (fixed point, or the 512-step cap: `capped`), else the first voxel
+// outside the tile; steps = moves made to reach it.
+__device__ __forceinline__ uint64_t walk_record(int target, int steps, bool fin, bool capped) {
+  return (uint64_t)(uint32_t)target | ((uint64_t)steps << 32) | ((uint64_t)fin << 48) |
+         ((uint64_t)capped << 49);
+}
+__device__ __forceinline__ int walk_target(uint64_t r) { return (int)(uint32_t)r; }
+__device__ __forceinline__ int walk_steps(uint64_t r) { return (int)((r >> 32) & 0xffffu); }
+__device__ __forceinline__ bool walk_final(uint64_t r) { return (r >> 48) & 1u; }
+__device__ __forceinline__ bool walk_capped(uint64_t r) { return (r >> 49) & 1u; }
 
 // Lock-free union-find over int32 parents (root = smallest index of its tree): the larger root
 // is hooked under the smaller with atomicMin; a lost race retries with the value it found.

@@ -15,8 +15,7 @@ static dim3 vox_grid(Vol v) { return dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsign
 static const dim3 kVoxBlock(64, 4, 1);
 
 // One pass over the SDF, LDS-tiled: a 512-thread block owns a 64(x) x 8(y) column tile and
-// marches PRE_ZC slices in z, keeping three haloed slices (66 x 10) in an LDS ring, so every SDF
-// value is read from memory about once. Per voxel:
+// marches PRE_ZC slices in z. Per voxel:
 //   - the 26-neighbour lexicographic (value, index) maximum (pore_extraction.hpp:93-104 and
 //     :280-304): peak of the extract_pores predicate (`cv <= 0` excluded, :90) and the ascent
 //     successor;
@@ -24,119 +23,101 @@ static const dim3 kVoxBlock(64, 4, 1);
 //     (init_markers :177; the run is connected along x, so component minima are unchanged),
 //     flood-band membership (:245-248), and the first-appearance minimum of each sdf>0 peak
 //     seeded with the peak's own index (basin voxels before it lower it later, in the chase).
-// Peaks and band voxels are staged per block in (slice, wave, lane) order.
+//
+// The maximum is separable over the ordered (value, index) keys of lex_key: max over z of max
+// over y of max over x: a haloed slice of keys goes to LDS, the x-triple
+// maxima of its 10 rows to LDS, each thread keeps the 3x3 maximum of its own column for three
+// consecutive slices in registers.
+// Peaks and band voxels are appended to a per-wave staging segment of 64 * PRE_ZC slots.
 constexpr int PRE_TY = 8;
 constexpr int PRE_ZC = 16;
-constexpr int PRE_CAP = 64 * PRE_TY * PRE_ZC;  // staging slots per block
+constexpr int PRE_CAP = 64 * PRE_ZC;  // staging slots per wave
 constexpr int PRE_LX = 66, PRE_LY = PRE_TY + 2;
-
-__device__ __forceinline__ int wrapm(int v, int N) { return ((v % N) + N) % N; }
 
 template <bool SEG>
 __global__ __launch_bounds__(64 * PRE_TY) void k_pre(const float* __restrict__ sdf, int nx,
                                                      int ny, int nz, float thr, PreOut o) {
-  __shared__ float ring[3][PRE_LY][PRE_LX];
-  __shared__ unsigned wc_peak[PRE_TY], wc_band[PRE_TY];
-  __shared__ unsigned s_pbase, s_bbase;
+  __shared__ uint64_t K[PRE_LY][PRE_LX];
+  __shared__ uint64_t XM[PRE_LY][64];
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int tid = wave * 64 + lane;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * PRE_TY, z0 = blockIdx.z * PRE_ZC;
   const int x = x0 + lane, y = y0 + wave;
   const bool col_in = (x < nx) && (y < ny);
-  const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  // wrapped global coordinates of this thread's column and its x/y neighbours
-  const int xs[3] = {wrapm(x - 1, nx), wrapm(x, nx), wrapm(x + 1, nx)};
-  const int ys[3] = {wrapm(y - 1, ny), wrapm(y, ny), wrapm(y + 1, ny)};
+  const bool near = nx >= 64 && ny > PRE_TY;  // halo coordinates lie in [-N, 2N)
+  const int64_t seg_id =
+      (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * PRE_TY + wave;
+  uint32_t* __restrict__ pstage = o.peak_stage + seg_id * PRE_CAP;
+  uint32_t* __restrict__ bstage = SEG ? o.band_stage + seg_id * PRE_CAP : nullptr;
+  unsigned npk = 0, nbd = 0;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
 
-  auto load_slice = [&](int zg, int slot) {
-    const int zw = wrapm(zg, nz);
+  // 3x3 (x, y) key maximum of this thread's column at slice zg; also returns the centre key.
+  auto slice_max = [&](int zg, uint64_t& centre) -> uint64_t {
+    const int zw = wrap_any(zg, nz);
     const float* __restrict__ plane = sdf + (int64_t)zw * ny * nx;
     for (int e = tid; e < PRE_LX * PRE_LY; e += 64 * PRE_TY) {
       const int ly = e / PRE_LX, lx = e - ly * PRE_LX;
-      const int gx = wrapm(x0 - 1 + lx, nx), gy = wrapm(y0 - 1 + ly, ny);
-      ring[slot][ly][lx] = plane[(int64_t)gy * nx + gx];
+      const int gx = near ? wrap_near(x0 - 1 + lx, nx) : wrap_any(x0 - 1 + lx, nx);
+      const int gy = near ? wrap_near(y0 - 1 + ly, ny) : wrap_any(y0 - 1 + ly, ny);
+      const int gi = (zw * ny + gy) * nx + gx;
+      K[ly][lx] = lex_key(plane[(int64_t)gy * nx + gx], (uint32_t)gi);
     }
+    __syncthreads();
+    for (int e = tid; e < PRE_LY * 64; e += 64 * PRE_TY) {
+      const int ly = e >> 6, lx = e & 63;
+      XM[ly][lx] = kmax(kmax(K[ly][lx], K[ly][lx + 1]), K[ly][lx + 2]);
+    }
+    centre = K[wave + 1][lane + 1];
+    __syncthreads();
+    return kmax(kmax(XM[wave][lane], XM[wave + 1][lane]), XM[wave + 2][lane]);
   };
-  if (tid == 0) {
-    s_pbase = 0;
-    s_bbase = 0;
-  }
-  load_slice(z0 - 1, 0);
-  load_slice(z0, 1);
+
+  uint64_t c_prev, c_cur, c_next;
+  uint64_t m_prev = slice_max(z0 - 1, c_prev);
+  uint64_t m_cur = slice_max(z0, c_cur);
   const int zend = min(z0 + PRE_ZC, nz);
   for (int z = z0; z < zend; ++z) {
-    const int rel = z - z0;
-    load_slice(z + 1, (rel + 2) % 3);
-    __syncthreads();
-    const int sm1 = rel % 3, s0 = (rel + 1) % 3, sp1 = (rel + 2) % 3;
-    const int slots[3] = {sm1, s0, sp1};
-    const int zs[3] = {wrapm(z - 1, nz), z, wrapm(z + 1, nz)};
-    bool is_peak = false, band = false;
-    int i = 0;
-    float cv = 0.0f;
-    if (col_in) {
-      i = (z * ny + y) * nx + x;
-      cv = ring[s0][wave + 1][lane + 1];
-      int nxt = i;
-      bool pos_peak = false;
-      if (!(cv <= 0.0f)) {
-        int best = i;
-        float bv = cv;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            const int row = (zs[a] * ny + ys[b]) * nx;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              if (a == 1 && b == 1 && c == 1) continue;
-              const float nv = ring[slots[a]][wave + b][lane + c];
-              const int ni = row + xs[c];
-              if (nv > bv) {
-                bv = nv;
-                best = ni;
-              } else if (nv == bv && ni > best) {
-                best = ni;
-              }
-            }
-          }
-        is_peak = (best == i);
+    const uint64_t m_next = slice_max(z + 1, c_next);
+    const int i = (z * ny + y) * nx + x;
+    const float cv = key_value(c_cur);
+    bool is_peak = false;
+    int nxt = i;
+    if (col_in && !(cv <= 0.0f)) {
+      if (cv != cv) {
+        is_peak = true;  // nothing beats a NaN centre
+      } else {
+        const int best = (int)(uint32_t)kmax(kmax(m_prev, m_cur), m_next);
+        is_peak = best == i;
         if (cv > 0.0f) nxt = best;
-        pos_peak = is_peak && (cv > 0.0f);
-      }
-      if (SEG) {
-        o.next[i] = nxt;
-        if (pos_peak) o.min_pore[i] = i;
       }
     }
+    const uint64_t pm = __ballot(is_peak);
+    if (is_peak) pstage[npk + __popcll(pm & below)] = (uint32_t)i;
+    npk += (unsigned)__popcll(pm);
     if (SEG) {
       const bool mk = col_in && (cv < thr);
       const uint64_t m = __ballot(mk);
       const uint64_t starts = m & ~(m << 1);
-      const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
-      const uint64_t smask = starts & upto;
+      const uint64_t smask = starts & (below | (1ull << lane));
       const int st = smask ? 63 - __clzll((long long)smask) : lane;
-      if (col_in) o.labels[i] = mk ? i - (lane - st) : -1;
-      band = col_in && !(cv >= 0.0f) && !mk;
+      if (col_in) {
+        o.labels[i] = mk ? i - (lane - st) : -1;
+        o.next[i] = nxt;
+        if (is_peak && cv > 0.0f) o.min_pore[i] = i;
+      }
+      const bool band = col_in && !(cv >= 0.0f) && !mk;
+      const uint64_t bm = __ballot(band);
+      if (band) bstage[nbd + __popcll(bm & below)] = (uint32_t)i;
+      nbd += (unsigned)__popcll(bm);
     }
-    unsigned ptot;
-    const unsigned ppos = block_stage_pos<PRE_TY>(is_peak, wc_peak, wave, lane, ptot);
-    if (is_peak) o.peak_stage[bid * PRE_CAP + s_pbase + ppos] = (uint32_t)i;
-    unsigned btot = 0;
-    if (SEG) {
-      const unsigned bpos = block_stage_pos<PRE_TY>(band, wc_band, wave, lane, btot);
-      if (band) o.band_stage[bid * PRE_CAP + s_bbase + bpos] = (uint32_t)i;
-    }
-    __syncthreads();  // ring slot and the wave counts are reused by the next slice
-    if (tid == 0) {
-      s_pbase += ptot;
-      s_bbase += btot;
-    }
+    m_prev = m_cur;
+    m_cur = m_next;
+    c_cur = c_next;
   }
-  __syncthreads();
-  if (tid == 0) {
-    o.peak_cnt[bid] = s_pbase;
-    if (SEG) o.band_cnt[bid] = s_bbase;
+  if (lane == 0) {
+    o.peak_cnt[seg_id] = npk;
+    if (SEG) o.band_cnt[seg_id] = nbd;
   }
 }
 
@@ -146,7 +127,7 @@ static dim3 pre_grid(Vol v) {
 
 int64_t pre_blocks(Vol v) {
   const dim3 g = pre_grid(v);
-  return (int64_t)g.x * g.y * g.z;
+  return (int64_t)g.x * g.y * g.z * PRE_TY;
 }
 int64_t pre_stage_cap() { return PRE_CAP; }
 

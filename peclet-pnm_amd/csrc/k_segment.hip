@@ -48,11 +48,16 @@ __global__ __launch_bounds__(256) void k_ccl_flatten(int* L, int64_t n, int32_t*
   if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
+void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
+                        uint32_t* key_cnt) {
+  PNM_LAUNCH(c, "k_ccl_flatten", k_ccl_flatten, dim3(cdiv(v.n, 256)), dim3(256), 0, labels, v.n,
+             min_solid, key_stage, key_cnt);
+}
+
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                 uint32_t* key_cnt) {
   PNM_LAUNCH(c, "k_ccl_runs", k_ccl_runs, vox_grid(v), kVoxBlock, 0, labels, v.nx, v.ny, v.nz);
-  PNM_LAUNCH(c, "k_ccl_flatten", k_ccl_flatten, dim3(cdiv(v.n, 256)), dim3(256), 0, labels, v.n,
-             min_solid, key_stage, key_cnt);
+  launch_ccl_flatten(c, v, labels, min_solid, key_stage, key_cnt);
 }
 
 // ---------------- flood (pore_extraction.hpp:237-269) ----------------

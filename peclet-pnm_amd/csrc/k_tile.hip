@@ -1,0 +1,609 @@
+// k_tile.hip — the segmentation's one pass over the SDF, tile by tile in LDS.
+//
+// A 512-thread block owns a 64(x) x 8(y) x 16(z) tile. It marches the tile's slices in z,
+// computing for every voxel the 26-neighbour lexicographic (value, index) maximum
+// (pore_extraction.hpp:93-104, :280-304) by the separable key maximum (lex_key in
+// k_common.hpp), and keeps two tile-sized arrays in LDS:
+//   LAB: marker labels (init_markers :177), first as the start of the voxel's x-run;
+//   DIR: for sdf>0 voxels, the direction (0..26, 13 = itself) of the ascent successor.
+// After the march it finishes, without leaving LDS,
+//   - the CCL inside the tile (merge_markers :198-224 restricted to in-tile, non-wrapping
+//     neighbour pairs; union-find with LDS atomicMin, root = smallest local index, which is the
+//     smallest flat index since the tile is a box), and writes every marker's label as the flat
+//     index of its tile-local root. k_ccl_seams then adds the relations that cross tiles or wrap;
+//   - the ascent walk inside the tile (gradient_path :279-309): a walk that reaches a fixed point
+//     or the 512-step cap inside the tile is final and its root is written; a walk that leaves
+//     the tile records (first voxel outside, steps so far) and is queued for k_roots_resolve.
+//     Walk records are also written for every sdf>0 voxel on the tile's boundary — exactly the
+//     voxels a walk can enter the tile through — so a resolver hops tile to tile;
+//   - the first-appearance minimum of each in-tile root over the tile's final walks (relabel_min
+//     :333-334), stored at the root: this seeds min_pore for every sdf>0 peak.
+// Flood candidates (:245-248) and the extract_pores peak predicate (:87-105) are staged per wave.
+#pragma clang fp contract(off)
+
+#include "k_common.hpp"
+
+namespace pnm {
+
+constexpr int T_LX = 66, T_LY = TILE_Y + 2;
+constexpr int T_VOX = 64 * TILE_Y * TILE_Z;
+constexpr int T_SEG = 64 * TILE_Z;  // staging slots per wave
+// DIR byte: bits 0-4 the ascent direction of an sdf>0 voxel (0..26, 13 = itself; kNoDir for
+// other voxels), bit 5 the extract_pores peak predicate, bit 6 flood-band membership.
+constexpr uint8_t kNoDir = 31;
+constexpr uint8_t kDirMask = 31, kPeakBit = 32, kBandBit = 64;
+
+__device__ __forceinline__ int lds_find(int* L, int x) {
+  int p = L[x];
+  while (p != x) {
+    const int g = L[p];
+    if (g != p) L[x] = g;
+    x = p;
+    p = g;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void lds_unite(int* L, int a, int b) {
+  while (true) {
+    a = lds_find(L, a);
+    b = lds_find(L, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(L + b, a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+// Chase state of an sdf>0 voxel during pointer jumping: u (local index, 13 bits), steps to u
+// (10 bits), status of u: ACT = u's own walk not yet known, FIX = u is a fixed point, EXIT = u's
+// successor lies outside the tile, LONG = the walk takes more than 512 steps inside the tile.
+constexpr uint32_t CS_ACT = 0, CS_FIX = 1, CS_EXIT = 2, CS_LONG = 3;
+__device__ __forceinline__ uint32_t cs_pack(int u, int s, uint32_t st) {
+  return (uint32_t)u | ((uint32_t)s << 13) | (st << 23);
+}
+__device__ __forceinline__ int cs_u(uint32_t w) { return (int)(w & 0x1fffu); }
+__device__ __forceinline__ int cs_s(uint32_t w) { return (int)((w >> 13) & 0x3ffu); }
+__device__ __forceinline__ uint32_t cs_st(uint32_t w) { return w >> 23; }
+
+// direction code d = (dz+1)*9 + (dy+1)*3 + (dx+1)
+__device__ __forceinline__ void dir_delta(int d, int& dx, int& dy, int& dz) {
+  dz = d / 9 - 1;
+  dy = (d / 3) % 3 - 1;
+  dx = d % 3 - 1;
+}
+
+__global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ sdf, int nx,
+                                                      int ny, int nz, float thr, TileOut o) {
+  __shared__ uint64_t K[T_LY][T_LX];
+  __shared__ uint64_t XM[T_LY][64];
+  __shared__ uint8_t XD[T_LY][64];
+  __shared__ int LAB[T_VOX];
+  __shared__ uint8_t DIR[T_VOX];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  const int tid = wave * 64 + lane;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TILE_Y, z0 = blockIdx.z * TILE_Z;
+  const int x = x0 + lane, y = y0 + wave;
+  const int nzt = min(TILE_Z, nz - z0);
+  const int xlast = min(x0 + 63, nx - 1), ylast = min(y0 + TILE_Y - 1, ny - 1);
+  const bool col_in = (x < nx) && (y < ny);
+  const int64_t seg_id =
+      (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * TILE_Y + wave;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int plane_n = ny * nx;
+  unsigned npk = 0, nbd = 0, nop = 0;
+  auto gidx = [&](int l) {
+    return ((z0 + (l >> 9)) * ny + y0 + ((l >> 6) & 7)) * nx + x0 + (l & 63);
+  };
+
+  // This thread's two halo-slice elements (e = tid, tid + 512 of the 10 x 66 slice): in-plane
+  // offsets of their wrapped (x, y), fixed for the whole march.
+  constexpr int kSliceElems = T_LX * T_LY;
+  const bool has1 = tid + 64 * TILE_Y < kSliceElems;
+  int off0, off1 = 0;
+  {
+    const int ly = tid / T_LX, lx = tid - ly * T_LX;
+    off0 = wrap_any(y0 - 1 + ly, ny) * nx + wrap_any(x0 - 1 + lx, nx);
+    if (has1) {
+      const int e = tid + 64 * TILE_Y;
+      const int ly1 = e / T_LX, lx1 = e - ly1 * T_LX;
+      off1 = wrap_any(y0 - 1 + ly1, ny) * nx + wrap_any(x0 - 1 + lx1, nx);
+    }
+  }
+  // Loads are unconditional (a thread without a second element re-reads its first; slices past
+  // the tile re-read its last, an L2 hit), so no branch makes the compiler drain the loads in
+  // flight.
+  if (!has1) off1 = off0;
+  auto load_slice = [&](int zg, float& a, float& b) {
+    const float* __restrict__ plane =
+        sdf + (int64_t)wrap_near(min(zg, z0 + nzt), nz) * plane_n;
+    a = plane[off0];
+    b = plane[off1];
+  };
+  // 3x3 (x, y) key maximum of this thread's column at slice zg (values a, b already loaded),
+  // with the winner's (dy, dx) as dy*3+dx, and the centre key.
+  auto slice_max = [&](int zg, float a, float b, uint64_t& centre, int& code) -> uint64_t {
+    const uint32_t zbase = (uint32_t)wrap_near(zg, nz) * (uint32_t)plane_n;
+    (&K[0][0])[tid] = lex_key(a, zbase + (uint32_t)off0);
+    if (has1) (&K[0][0])[tid + 64 * TILE_Y] = lex_key(b, zbase + (uint32_t)off1);
+    __syncthreads();
+    for (int e = tid; e < T_LY * 64; e += 64 * TILE_Y) {
+      const int ly = e >> 6, lx = e & 63;
+      uint64_t m = K[ly][lx];
+      uint8_t d = 0;
+      const uint64_t k1 = K[ly][lx + 1], k2 = K[ly][lx + 2];
+      if (k1 > m) { m = k1; d = 1; }
+      if (k2 > m) { m = k2; d = 2; }
+      XM[ly][lx] = m;
+      XD[ly][lx] = d;
+    }
+    centre = K[wave + 1][lane + 1];
+    __syncthreads();
+    uint64_t m = XM[wave][lane];
+    int dy = 0;
+    const uint64_t m1 = XM[wave + 1][lane], m2 = XM[wave + 2][lane];
+    if (m1 > m) { m = m1; dy = 1; }
+    if (m2 > m) { m = m2; dy = 2; }
+    code = dy * 3 + XD[wave + dy][lane];
+    return m;
+  };
+
+  // ---- march: slices z0-1 .. z0+nzt, three slices of loads in flight ----
+  // Slice k is z0 - 1 + k. The buffers A, B, C are used in rotation without copies, so each
+  // load stays in flight until the slice that needs it.
+  const int last_k = nzt + 1;
+  float A0, A1 = 0.f, B0, B1 = 0.f, C0 = 0.f, C1 = 0.f;
+  load_slice(z0 - 1, A0, A1);
+  load_slice(z0, B0, B1);
+  load_slice(z0 + 1, C0, C1);
+  uint64_t c_prev, c_cur, c_next;
+  int d_prev, d_cur, d_next;
+  uint64_t m_prev = slice_max(z0 - 1, A0, A1, c_prev, d_prev);
+  load_slice(z0 + 2, A0, A1);
+  uint64_t m_cur = slice_max(z0, B0, B1, c_cur, d_cur);
+  load_slice(z0 + 3, B0, B1);
+  uint32_t pore_mask = 0;
+  // one slice of the march: needs slice k = zl + 2 (values va, vb)
+  auto march = [&](int zl, float va, float vb) {
+    const int z = z0 + zl;
+    const uint64_t m_next = slice_max(z + 1, va, vb, c_next, d_next);
+    const int l = (zl * TILE_Y + wave) * 64 + lane;
+    const int i = (z * ny + y) * nx + x;
+    const float cv = key_value(c_cur);
+    bool is_peak = false;
+    uint8_t dir = kNoDir;
+    if (col_in && !(cv <= 0.0f)) {
+      if (cv != cv) {
+        is_peak = true;  // nothing beats a NaN centre
+      } else {
+        uint64_t m = m_prev;
+        int dz = 0, cd = d_prev;
+        if (m_cur > m) { m = m_cur; dz = 1; cd = d_cur; }
+        if (m_next > m) { m = m_next; dz = 2; cd = d_next; }
+        is_peak = (int)(uint32_t)m == i;
+        if (cv > 0.0f) {
+          dir = is_peak ? (uint8_t)13 : (uint8_t)(dz * 9 + cd);
+          pore_mask |= 1u << zl;
+        }
+      }
+    }
+    const bool mk = col_in && (cv < thr);
+    const uint64_t mm = __ballot(mk);
+    const uint64_t starts = mm & ~(mm << 1);
+    const uint64_t smask = starts & (below | (1ull << lane));
+    const int st = smask ? 63 - __clzll((long long)smask) : lane;
+    LAB[l] = mk ? l - (lane - st) : -1;
+    const bool band = col_in && !(cv >= 0.0f) && !mk;
+    DIR[l] = dir | (is_peak ? kPeakBit : 0) | (band ? kBandBit : 0);
+    (void)i;
+    m_prev = m_cur;
+    m_cur = m_next;
+    d_prev = d_cur;
+    d_cur = d_next;
+    c_cur = c_next;
+  };
+  for (int zl = 0; zl < nzt; zl += 3) {  // slices zl+2 in C, zl+3 in A, zl+4 in B
+    march(zl, C0, C1);
+    load_slice(z0 + zl + 4, C0, C1);
+    if (zl + 1 >= nzt) break;
+    march(zl + 1, A0, A1);
+    load_slice(z0 + zl + 5, A0, A1);
+    if (zl + 2 >= nzt) break;
+    march(zl + 2, B0, B1);
+    load_slice(z0 + zl + 6, B0, B1);
+  }
+  (void)last_k;
+  __syncthreads();
+  // stage peaks and band voxels, in (slice, lane) order per wave
+  for (int zl = 0; zl < nzt; ++zl) {
+    const int l = (zl * TILE_Y + wave) * 64 + lane;
+    const int i = ((z0 + zl) * ny + y) * nx + x;
+    const uint8_t f = col_in ? DIR[l] : 0;
+    const bool is_peak = f & kPeakBit, band = f & kBandBit;
+    const uint64_t pm = __ballot(is_peak);
+    if (is_peak) o.peak_stage[seg_id * T_SEG + npk + __popcll(pm & below)] = (uint32_t)i;
+    npk += (unsigned)__popcll(pm);
+    const uint64_t bm = __ballot(band);
+    if (band) o.band_stage[seg_id * T_SEG + nbd + __popcll(bm & below)] = (uint32_t)i;
+    nbd += (unsigned)__popcll(bm);
+  }
+
+  // ---- CCL inside the tile: forward rows (y+1,z), (y-1,z+1), (y,z+1), (y+1,z+1) ----
+  if (y < ny) {
+    const bool yp_ok = wave + 1 < TILE_Y && y + 1 < ny, ym_ok = wave >= 1;
+    for (int zl = 0; zl < nzt; ++zl) {
+      const int l = (zl * TILE_Y + wave) * 64 + lane;
+      const int a = LAB[l];
+      const bool mk = a != -1;
+      const uint64_t m = __ballot(mk);
+      if (m == 0) continue;
+      const bool run_start = mk && lane > 0 && !((m >> (lane - 1)) & 1ull);
+      const bool run_end = mk && lane < 63 && x + 1 < nx && !((m >> (lane + 1)) & 1ull);
+      const bool zp_ok = zl + 1 < nzt;
+      const int rows[4] = {(zl * TILE_Y + wave + 1) * 64, (((zl + 1) * TILE_Y) + wave - 1) * 64,
+                           ((zl + 1) * TILE_Y + wave) * 64, ((zl + 1) * TILE_Y + wave + 1) * 64};
+      const bool ok[4] = {yp_ok, ym_ok && zp_ok, zp_ok, yp_ok && zp_ok};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (!ok[r]) continue;  // wave-uniform
+        const int c = LAB[rows[r] + lane];
+        const bool cm = c != -1;
+        const int prev_c = __shfl_up(c, 1, 64);
+        if (mk && cm) {
+          const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+          if (!dup) lds_unite(LAB, a, c);
+        }
+        if (mk && !cm) {
+          if (run_start) {
+            const int p = LAB[rows[r] + lane - 1];
+            if (p != -1) lds_unite(LAB, a, p);
+          }
+          if (run_end) {
+            const int q = LAB[rows[r] + lane + 1];
+            if (q != -1) lds_unite(LAB, a, q);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int zl = 0; zl < nzt; ++zl) {
+    const int l = (zl * TILE_Y + wave) * 64 + lane;
+    const int a = LAB[l];
+    if (a != -1) LAB[l] = lds_find(LAB, a);
+  }
+  __syncthreads();
+  if (col_in) {
+    for (int zl = 0; zl < nzt; ++zl) {
+      const int l = (zl * TILE_Y + wave) * 64 + lane;
+      const int a = LAB[l];
+      o.labels[((z0 + zl) * ny + y) * nx + x] = a == -1 ? -1 : gidx(a);
+    }
+  }
+  __syncthreads();
+
+  // ---- ascent walks inside the tile: pointer jumping on LAB (now the chase state) ----
+  auto in_tile = [&](int ax, int ay, int az) {
+    return ax >= 0 && ax < 64 && x0 + ax < nx && ay >= 0 && ay < TILE_Y && y0 + ay < ny &&
+           az >= 0 && az < nzt;
+  };
+  for (int zl = 0; zl < nzt; ++zl) {
+    if (!((pore_mask >> zl) & 1u)) continue;
+    const int l = (zl * TILE_Y + wave) * 64 + lane;
+    const int d = DIR[l] & kDirMask;
+    uint32_t w;
+    if (d == 13) {
+      w = cs_pack(l, 0, CS_FIX);
+    } else {
+      int dx, dy, dz;
+      dir_delta(d, dx, dy, dz);
+      const int ax = lane + dx, ay = wave + dy, az = zl + dz;
+      w = in_tile(ax, ay, az) ? cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT)
+                              : cs_pack(l, 0, CS_EXIT);
+    }
+    LAB[l] = (int)w;
+  }
+  __syncthreads();
+  while (true) {
+    bool active = false;
+    for (int zl = 0; zl < nzt; ++zl) {
+      if (!((pore_mask >> zl) & 1u)) continue;
+      const int l = (zl * TILE_Y + wave) * 64 + lane;
+      const uint32_t w = (uint32_t)LAB[l];
+      if (cs_st(w) != CS_ACT) continue;
+      const uint32_t wv = (uint32_t)LAB[cs_u(w)];
+      const int s2 = cs_s(w) + cs_s(wv);
+      const uint32_t sv = cs_st(wv);
+      if (sv == CS_LONG || s2 > kMaxSteps) {
+        LAB[l] = (int)cs_pack(cs_u(w), cs_s(w), CS_LONG);
+      } else {
+        LAB[l] = (int)cs_pack(cs_u(wv), s2, sv);
+        active |= sv == CS_ACT;
+      }
+    }
+    if (!__syncthreads_or(active)) break;
+  }
+
+  // ---- walk outputs; LAB at each fixed point collects the smallest local index whose walk
+  //      ends there (every other LAB word read below belongs to the reading thread) ----
+  bool capped_any = false;
+  for (int zl = 0; zl < nzt; ++zl) {
+    const int l = (zl * TILE_Y + wave) * 64 + lane;
+    const int i = ((z0 + zl) * ny + y) * nx + x;
+    const bool pore = (pore_mask >> zl) & 1u;
+    bool open = false;
+    if (pore) {
+      int target, steps, fin_u = -1;
+      bool fin = false, capped = false;
+      const bool root_here = (DIR[l] & kDirMask) == 13;
+      const uint32_t w = root_here ? cs_pack(l, 0, CS_FIX) : (uint32_t)LAB[l];
+      const int u = cs_u(w);
+      steps = cs_s(w);
+      if (cs_st(w) == CS_FIX) {
+        fin = true;
+        fin_u = u;
+        target = gidx(u);
+      } else if (cs_st(w) == CS_EXIT) {
+        if (steps == kMaxSteps) {  // the 512th step lands on u, whose successor is outside
+          fin = true;
+          capped = true;
+          target = gidx(u);
+        } else {
+          int dx, dy, dz;
+          dir_delta(DIR[u] & kDirMask, dx, dy, dz);
+          const int gx = wrap_near(x0 + (u & 63) + dx, nx);
+          const int gy = wrap_near(y0 + ((u >> 6) & 7) + dy, ny);
+          const int gz = wrap_near(z0 + (u >> 9) + dz, nz);
+          target = (gz * ny + gy) * nx + gx;
+          steps += 1;
+        }
+      } else {  // CS_LONG: walk the 512 steps one by one
+        int cx = lane, cy = wave, cz = zl, cl = l;
+        steps = 0;
+        while (true) {
+          const int d = DIR[cl] & kDirMask;
+          if (d == 13) {
+            fin = true;
+            fin_u = cl;
+            target = gidx(cl);
+            break;
+          }
+          int dx, dy, dz;
+          dir_delta(d, dx, dy, dz);
+          const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+          ++steps;
+          if (!in_tile(ax, ay, az)) {
+            target = (wrap_near(z0 + az, nz) * ny + wrap_near(y0 + ay, ny)) * nx +
+                     wrap_near(x0 + ax, nx);
+            break;
+          }
+          cx = ax;
+          cy = ay;
+          cz = az;
+          cl = (az * TILE_Y + ay) * 64 + ax;
+          if (steps == kMaxSteps) {
+            fin = true;
+            capped = (DIR[cl] & kDirMask) != 13;
+            if (!capped) fin_u = cl;
+            target = gidx(cl);
+            break;
+          }
+        }
+      }
+      if (fin) {
+        o.roots[i] = target;
+        if (!capped) atomicMin(&LAB[fin_u], l);
+        capped_any |= capped;
+      }
+      open = !fin;
+      const bool boundary = x == x0 || x == xlast || y == y0 || y == ylast || zl == 0 ||
+                            zl == nzt - 1;
+      if (open || boundary) o.walk[i] = walk_record(target, steps, fin, capped);
+    }
+    const uint64_t om = __ballot(open);
+    if (open) o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] = (uint32_t)i;
+    nop += (unsigned)__popcll(om);
+  }
+  if (__any(capped_any) && lane == 0) atomicOr(o.cap_flag, 1ull);
+  __syncthreads();
+  for (int zl = 0; zl < nzt; ++zl) {
+    if (!((pore_mask >> zl) & 1u)) continue;
+    const int l = (zl * TILE_Y + wave) * 64 + lane;
+    if ((DIR[l] & kDirMask) == 13) o.min_pore[gidx(l)] = gidx(LAB[l]);
+  }
+  if (lane == 0) {
+    o.peak_cnt[seg_id] = npk;
+    o.band_cnt[seg_id] = nbd;
+    o.open_cnt[seg_id] = nop;
+  }
+}
+
+// ---- cross-tile and wrapping marker relations ----
+// One wave per 64-voxel row chunk. The +x relation at the chunk start is always global; a
+// forward row in another tile (or reached by wrapping) gets the full run-union rule; a forward
+// row in the same tile only needs the two contacts the tile skipped: the chunk's first column
+// with column x-1 and its last column with column x+1.
+__global__ __launch_bounds__(256) void k_ccl_seams(int* L, int nx, int ny, int nz) {
+  const int lane = threadIdx.x;
+  const int x0 = blockIdx.x * 64, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+  if (y >= ny) return;  // wave-uniform
+  const int x = x0 + lane;
+  const bool in = x < nx;
+  const int xlast = min(x0 + 63, nx - 1);
+  const int i = in ? (z * ny + y) * nx + x : 0;
+  const int my = in ? L[i] : -1;
+  const bool mk = my != -1;
+  const uint64_t m = __ballot(mk);
+  if (m == 0) return;
+  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+  const bool run_end = mk && (x == xlast || !((m >> (lane + 1)) & 1ull));
+  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
+  if (mk && lane == 0) {
+    const int nl = L[(z * ny + y) * nx + xm];
+    if (nl != -1) uf_unite(L, my, nl);
+  }
+  const int yp = wp1(y, ny), ym = wm1(y, ny), zp = wp1(z, nz);
+  const bool ty_p = (y + 1 < ny) && ((y + 1) / TILE_Y == y / TILE_Y);
+  const bool ty_m = (y >= 1) && ((y - 1) / TILE_Y == y / TILE_Y);
+  const bool tz_p = (z + 1 < nz) && ((z + 1) / TILE_Z == z / TILE_Z);
+  const int rows[4] = {(z * ny + yp) * nx, (zp * ny + ym) * nx, (zp * ny + y) * nx,
+                       (zp * ny + yp) * nx};
+  const bool same[4] = {ty_p, ty_m && tz_p, tz_p, ty_p && tz_p};
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (same[r]) {
+      // edge contacts only: first column with x-1, last column with x+1
+      const bool edge_s = mk && lane == 0, edge_e = mk && x == xlast;
+      if (edge_s || edge_e) {
+        const int c = L[rows[r] + x];
+        if (c == -1) {
+          if (edge_s) {
+            const int a = L[rows[r] + xm];
+            if (a != -1) uf_unite(L, my, a);
+          }
+          if (edge_e) {
+            const int b = L[rows[r] + xp];
+            if (b != -1) uf_unite(L, my, b);
+          }
+        }
+      }
+      continue;
+    }
+    const int c = in ? L[rows[r] + x] : -1;
+    const bool cm = c != -1;
+    const int prev_c = __shfl_up(c, 1, 64);
+    if (mk && cm) {
+      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+      if (!dup) uf_unite(L, my, c);
+    }
+    if (mk && !cm) {
+      if (run_start) {
+        const int a = L[rows[r] + xm];
+        if (a != -1) uf_unite(L, my, a);
+      }
+      if (run_end) {
+        const int b = L[rows[r] + xp];
+        if (b != -1) uf_unite(L, my, b);
+      }
+    }
+  }
+}
+
+// ---- walks that left their tile ----
+// Hops along the walk records: from the voxel a walk entered a tile through, that voxel's own
+// record gives where its walk ends inside that tile (or leaves it again) and after how many
+// steps, which is where the original walk goes too. When the remaining step budget ends inside a
+// tile, the rest is walked step by step on the SDF.
+__global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__ sdf, int nx,
+                                                       int ny, int nz, int64_t n,
+                                                       const uint32_t* __restrict__ open,
+                                                       const uint64_t* __restrict__ count,
+                                                       const uint64_t* __restrict__ walk,
+                                                       int32_t* __restrict__ roots,
+                                                       int32_t* __restrict__ min_pore,
+                                                       unsigned long long* cap_flag) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int root = -1;
+  int i = 0;
+  if (j < (int64_t)*count) {
+    i = (int)open[j];
+    const uint64_t own = walk[i];
+    int w = walk_target(own);
+    int t = walk_steps(own);
+    while (true) {
+      // every hop target is an sdf>0 voxel on a tile boundary and carries a record; a target
+      // or record that breaks this is reported (bit 1) instead of followed
+      if ((uint64_t)(uint32_t)w >= (uint64_t)n) {
+        atomicOr(cap_flag, 2ull);
+        break;
+      }
+      const uint64_t rec = walk[w];
+      const int sw = walk_steps(rec);
+      if (!walk_final(rec) && (sw == 0 || sw > kMaxSteps)) {
+        atomicOr(cap_flag, 2ull);
+        break;
+      }
+      if (t == kMaxSteps) {  // the budget ran out exactly on entering this tile
+        root = w;
+        if (!(walk_final(rec) && sw == 0)) atomicOr(cap_flag, 1ull);
+        break;
+      }
+      if (t + sw <= kMaxSteps) {
+        t += sw;
+        if (walk_final(rec)) {
+          root = walk_target(rec);
+          if (walk_capped(rec)) atomicOr(cap_flag, 1ull);
+          break;
+        }
+        w = walk_target(rec);
+        continue;
+      }
+      // fewer steps left than this tile's walk takes: walk the remainder exactly
+      for (; t < kMaxSteps; ++t) {
+        const int wx = w % nx, wy = (w / nx) % ny, wz = w / (nx * ny);
+        const int b = ascent_best(sdf, wx, wy, wz, nx, ny, nz, w, sdf[w]);
+        if (b == w) break;
+        w = b;
+      }
+      root = w;
+      {
+        const int wx = w % nx, wy = (w / nx) % ny, wz = w / (nx * ny);
+        if (ascent_best(sdf, wx, wy, wz, nx, ny, nz, w, sdf[w]) != w) atomicOr(cap_flag, 1ull);
+      }
+      break;
+    }
+    if (root >= 0) roots[i] = root;
+  }
+  // of two neighbouring lanes with one root, the one with the larger voxel index skips
+  const int prev = __shfl_up(root, 1, 64), prev_i = __shfl_up(i, 1, 64);
+  if (root >= 0 && i < root && (lane == 0 || prev != root || prev_i > i))
+    atomicMin(min_pore + root, i);
+}
+
+// rootsOut convention for voxels that are not sdf>0: their own index (:383-384).
+__global__ void k_roots_self(const float* __restrict__ sdf, int64_t n, int32_t* roots) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(sdf[i] > 0.0f)) roots[i] = (int32_t)i;
+}
+
+void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots) {
+  PNM_LAUNCH(c, "k_roots_self", k_roots_self, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
+             roots);
+}
+
+static dim3 tile_grid(Vol v) {
+  return dim3(cdiv(v.nx, 64), cdiv(v.ny, TILE_Y), cdiv(v.nz, TILE_Z));
+}
+
+int64_t tile_segments(Vol v) {
+  const dim3 g = tile_grid(v);
+  return (int64_t)g.x * g.y * g.z * TILE_Y;
+}
+int64_t tile_segment_cap() { return T_SEG; }
+
+void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
+  PNM_LAUNCH(c, "k_tile", k_tile, tile_grid(v), dim3(64, TILE_Y), 0, sdf, v.nx, v.ny, v.nz, thr,
+             o);
+}
+
+void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels) {
+  PNM_LAUNCH(c, "k_ccl_seams", k_ccl_seams, dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz),
+             dim3(64, 4), 0, labels, v.nx, v.ny, v.nz);
+}
+
+void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,
+                          const uint64_t* count, int64_t grid_items, const uint64_t* walk,
+                          int32_t* roots, int32_t* min_pore, uint64_t* cap_flag) {
+  if (grid_items <= 0) return;
+  PNM_LAUNCH(c, "k_roots_resolve", k_roots_resolve, dim3(cdiv(grid_items, 256)), dim3(256), 0,
+             sdf, v.nx, v.ny, v.nz, v.n, open, count, walk, roots, min_pore,
+             reinterpret_cast<unsigned long long*>(cap_flag));
+}
+
+}  // namespace pnm

@@ -63,47 +63,61 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   const int64_t n = v.n;
   const float mn = std::min(spc[0], std::min(spc[1], spc[2]));
   const float thr = -1.5f * mn;  // pore_extraction.hpp:163-164
-  const int64_t nbp = pre_blocks(v), pcap = pre_stage_cap();
+  const int64_t nseg = tile_segments(v), scap = tile_segment_cap();
   const int64_t nb1 = cdiv(n, 256);
   int32_t* labels = labels_out ? labels_out : c->arena.get_n<int32_t>((size_t)n);
-  int32_t* next = c->arena.get_n<int32_t>((size_t)n);
   int32_t* roots = roots_out ? roots_out : c->arena.get_n<int32_t>((size_t)n);
+  uint64_t* walk = c->arena.get_n<uint64_t>((size_t)n);
   int32_t* min_pore = c->arena.get_n<int32_t>((size_t)n);
   int32_t* min_solid = c->arena.get_n<int32_t>((size_t)n);
-  // staging: k_pre uses nbp blocks of pcap slots; flatten and the flood use nb1 blocks of 256
-  const int64_t stage_words = std::max(nbp * pcap, nb1 * 256);
+  // staging: the tile pass fills nseg segments of scap slots; flatten and the flood use nb1
+  // blocks of 256
+  const int64_t stage_words = std::max(nseg * scap, nb1 * 256);
   uint32_t* stage_a = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)stage_words);
-  uint32_t* cnt_a = c->arena.get_n<uint32_t>((size_t)std::max(nbp, nb1));
-  uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nbp, nb1));
+  uint32_t* stage_c = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
+  uint32_t* cnt_a = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
+  uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
+  uint32_t* cnt_c = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)n);
+  uint32_t* open = c->arena.get_n<uint32_t>((size_t)n);
   int32_t* newlab = c->arena.get_n<int32_t>((size_t)n);
   uint32_t* skeys = c->arena.get_n<uint32_t>((size_t)n);
   uint64_t* ctr = c->arena.get_n<uint64_t>(64);
-  // ctr: [0] peaks [1] band [2] solid keys [3] cap flag [4] pores [5] solids [8..] band counts
+  // ctr: [0] peaks [1] open walks [2] solid keys [3] cap flag [4] pores [5] solids [6] full keys
+  //      [8..] band counts
   fill_u64(c, ctr, 0, 64);
 
-  PreOut o{};
+  TileOut o{};
   o.labels = labels;
-  o.next = next;
+  o.roots = roots;
+  o.walk = walk;
   o.min_pore = min_pore;
   o.peak_stage = stage_a;
   o.peak_cnt = cnt_a;
   o.band_stage = stage_b;
   o.band_cnt = cnt_b;
-  launch_pre(c, sdf, v, thr, o);
-  compact_staged(c, stage_a, cnt_a, nbp, pcap, peaks, ctr + 0);
-  compact_staged(c, stage_b, cnt_b, nbp, pcap, band, ctr + 8);
-  launch_ccl(c, v, labels, min_solid, stage_a, cnt_a);
+  o.open_stage = stage_c;
+  o.open_cnt = cnt_c;
+  o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
+  launch_tile(c, sdf, v, thr, o);
+  compact_staged(c, stage_a, cnt_a, nseg, scap, peaks, ctr + 0);
+  compact_staged(c, stage_b, cnt_b, nseg, scap, band, ctr + 8);
+  compact_staged(c, stage_c, cnt_c, nseg, scap, open, ctr + 1);
+  launch_ccl_seams(c, v, labels);
+  launch_ccl_flatten(c, v, labels, min_solid, stage_a, cnt_a);
   compact_staged(c, stage_a, cnt_a, nb1, 256, skeys, ctr + 2);
   if (ccl_out)
     PNM_HIP(hipMemcpyAsync(ccl_out, labels, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
                            c->stream));
   uint64_t h[9];
   readback(c, ctr, 9, h);
-  const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2], nband = (int64_t)h[8];
+  const int64_t npeaks = (int64_t)h[0], nopen = (int64_t)h[1], nskeys = (int64_t)h[2],
+                nband = (int64_t)h[8];
+  if (!labels_only)
+    launch_roots_resolve(c, sdf, v, open, ctr + 1, nopen, walk, roots, min_pore, ctr + 3);
 
   // Jacobi flood until a sweep labels nothing (pore_extraction.hpp:238-269). Band lengths live
   // on the device (ctr[8 + s]); the host checks them every two sweeps.
@@ -126,11 +140,11 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
 
   SegInternal out{};
   if (labels_only) return out;
-  launch_roots_min(c, sdf, v, next, roots, min_pore, ctr + 3, true);
   readback(c, ctr + 3, 1, h);
+  if (h[0] & 2u) fail(PNM_EHIP, "internal error: ascent walk record missing");
   const uint32_t* pkeys = peaks;
   int64_t npk = npeaks;
-  if (h[0]) {  // a walk stopped at the 512-step cap on a non-fixed point
+  if (h[0] & 1u) {  // a walk stopped at the 512-step cap on a non-fixed point
     launch_min_full(c, sdf, v, roots, min_pore);
     uint32_t* full = c->arena.get_n<uint32_t>((size_t)n);
     npk = collect_keys_full(c, sdf, v, min_pore, full, ctr + 6);
@@ -139,6 +153,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   assign_ids(c, sdf, pkeys, npk, min_pore, false, ctr + 4);
   assign_ids(c, sdf, skeys, nskeys, min_solid, true, ctr + 5);
   launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
+  if (roots_out) launch_roots_self(c, sdf, v, roots_out);
   readback(c, ctr + 4, 2, h);
   out.r.n_pores = (int32_t)h[0];
   out.r.n_solids = (int32_t)h[1];

@@ -130,12 +130,12 @@ struct PreOut {
   int32_t* labels;        // markers (x-run start index) or -1 [n], nullable
   int32_t* next;          // ascent successor                  [n], nullable
   int32_t* min_pore;      // set to the peak's own index at sdf>0 peaks [n], nullable
-  uint32_t* peak_stage;   // per-block staged peaks (extract_pores predicate), pre_stage_cap()
-  uint32_t* peak_cnt;     // per-block peak counts
-  uint32_t* band_stage;   // per-block staged flood candidates, nullable
-  uint32_t* band_cnt;     // per-block band counts, nullable
+  uint32_t* peak_stage;   // staged peaks (extract_pores predicate): pre_stage_cap() per segment
+  uint32_t* peak_cnt;     // per-segment peak counts
+  uint32_t* band_stage;   // staged flood candidates, nullable
+  uint32_t* band_cnt;     // per-segment band counts, nullable
 };
-// Number of blocks launch_pre uses for a volume, and the staging slots per block.
+// Number of staging segments launch_pre fills for a volume, and the slots per segment.
 int64_t pre_blocks(Vol v);
 int64_t pre_stage_cap();
 void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o);
@@ -144,6 +144,30 @@ void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const
 
 // CCL of markers: x-run unions + flatten; flatten stages the component roots (solid keys) per
 // 256-voxel block and sets min_solid[root] = root.
+// Segmentation tile pass (k_tile.hip).
+struct TileOut {
+  int32_t* labels;        // marker label = flat index of its tile-local CCL root, or -1 [n]
+  int32_t* roots;         // roots of sdf>0 voxels whose walk ends inside their tile    [n]
+  uint64_t* walk;         // walk records: sdf>0 voxels on tile boundaries or left open [n]
+  int32_t* min_pore;      // first-appearance minimum over the tile, at in-tile roots   [n]
+  uint32_t* peak_stage;   // staged peaks (extract_pores predicate): tile_segment_cap() per segment
+  uint32_t* peak_cnt;
+  uint32_t* band_stage;   // staged flood candidates
+  uint32_t* band_cnt;
+  uint32_t* open_stage;   // staged sdf>0 voxels whose walk left the tile
+  uint32_t* open_cnt;
+  unsigned long long* cap_flag;  // set when a walk stops at the cap on a non-fixed point
+};
+int64_t tile_segments(Vol v);
+int64_t tile_segment_cap();
+void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o);
+void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels);
+void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,
+                          const uint64_t* count, int64_t grid_items, const uint64_t* walk,
+                          int32_t* roots, int32_t* min_pore, uint64_t* cap_flag);
+void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots);
+void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
+                        uint32_t* key_cnt);
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                 uint32_t* key_cnt);
 // One Jacobi flood sweep over the band list whose length is the device word *count (grid sized
