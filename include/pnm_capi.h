@@ -196,6 +196,22 @@ pnm_status pnm_segment_batch_d(pnm_ctx_t ctx, int32_t nvol, const float* const* 
 pnm_status pnm_batch_config(pnm_ctx_t ctx, int32_t* block, int32_t* per_cu, int32_t* groups,
                             float* tuned_ms);
 
+/* ---- project MAC Stokes solver (data generation for extract_network_flow; the reference takes
+ * its fields from an external solver, README.md:56-100, whose MAC layout it states at
+ * pore_extraction.hpp:476-477) ----
+ * Solves mu*(-Lap u) + grad p = force, div u = 0 on the periodic MAC grid: u on the -x face of
+ * each cell, v on -y, w on -z, p at cell centres. Fluid = sdf > 0; a face is open iff both cells
+ * are fluid (binary openness); closed faces carry zero velocity. Preconditioned MINRES in FP64
+ * until the preconditioned residual falls below rtol times its initial value (or max_iter).
+ * dims/spacing/force are XYZ. u_d..p_d: device double[n] outputs. ox_d/oy_d/oz_d (nullable):
+ * device double[n] receiving the binary openness (1.0 / 0.0) in extract_network_flow's layout.
+ * iterations / rel_residual (nullable): MINRES iterations and ||b - Mx|| / ||b||. */
+pnm_status pnm_stokes_solve_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                              const float spacing[3], const double force[3], double mu,
+                              double rtol, int32_t max_iter, double* u_d, double* v_d,
+                              double* w_d, double* p_d, double* ox_d, double* oy_d, double* oz_d,
+                              int32_t* iterations, double* rel_residual);
+
 /* ---- stage-level device entry points (parity tests of single kernels; no reference
  * counterpart as separate calls: each is one step of segmentVolumeView) ---- */
 

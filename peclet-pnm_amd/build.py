@@ -50,6 +50,7 @@ SOURCES = [
     "k_topology.hip",
     "k_flow.hip",
     "k_tile.hip",
+    "k_stokes.hip",
     "k_batch.hip",
     "pipeline.hip",
     "capi.cpp",

@@ -486,6 +486,30 @@ pnm_status pnm_extract_network_flow_d(pnm_ctx_t ctx, const float* sdf_d, const i
   });
 }
 
+pnm_status pnm_stokes_solve_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                              const float spacing[3], const double force[3], double mu,
+                              double rtol, int32_t max_iter, double* u_d, double* v_d,
+                              double* w_d, double* p_d, double* ox_d, double* oy_d, double* oz_d,
+                              int32_t* iterations, double* rel_residual) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(spacing && force, "spacing and force are required");
+    need(v.n == 0 || (sdf_d && u_d && v_d && w_d && p_d), "sdf, u, v, w, p must not be NULL");
+    need(spacing[0] > 0.0f && spacing[1] > 0.0f && spacing[2] > 0.0f,
+         "spacing must be positive");
+    need(mu > 0.0 && rtol > 0.0 && max_iter >= 0, "mu and rtol must be positive, max_iter >= 0");
+    Ctx* c = check_ctx(ctx);
+    if (iterations) *iterations = 0;
+    if (rel_residual) *rel_residual = 0.0;
+    if (v.n == 0) return;
+    CallScope cs(c);
+    const StokesInfo si = stokes_device(c, sdf_d, v, spacing, force, mu, rtol, max_iter, u_d,
+                                        v_d, w_d, p_d, ox_d, oy_d, oz_d);
+    if (iterations) *iterations = si.iterations;
+    if (rel_residual) *rel_residual = si.rel_residual;
+  });
+}
+
 // ------------------------- host-pointer entry points -------------------------
 
 pnm_status pnm_extract_pores(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],

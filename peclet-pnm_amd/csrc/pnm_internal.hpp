@@ -239,4 +239,13 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
                  const double* ox, const double* oy, const double* oz, const double grad_p[3],
                  pnm_flow_result_t* out);
 
+// ---------- project MAC Stokes solver (k_stokes.hip) ----------
+struct StokesInfo {
+  int iterations;
+  double rel_residual;  // ||b - M x|| / ||b||
+};
+StokesInfo stokes_device(Ctx* c, const float* sdf, Vol v, const float spc[3],
+                         const double force[3], double mu, double rtol, int max_iter, double* u,
+                         double* vv, double* w, double* p, double* ox, double* oy, double* oz);
+
 }  // namespace pnm

@@ -27,6 +27,7 @@ EXPORTS = (
     "pnm_flow_result_free", "pnm_extract_pores_d", "pnm_segment_volume_d",
     "pnm_extract_topology_d", "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
     "pnm_stage_labels_d", "pnm_stage_roots_d", "pnm_segment_batch_d", "pnm_batch_config",
+    "pnm_stokes_solve_d",
 )
 
 PNM_BATCH_GRAPH = 1
@@ -106,6 +107,9 @@ def lib():
         L.pnm_segment_batch_d.restype = st
         L.pnm_batch_config.argtypes = [vp, vp, vp, vp, vp]
         L.pnm_batch_config.restype = st
+        L.pnm_stokes_solve_d.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.pnm_stokes_solve_d.restype = st
         L.pnm_stage_labels_d.argtypes = [vp, vp, vp, vp, vp, vp]
         L.pnm_stage_roots_d.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         for name in ("pnm_extract_pores_d", "pnm_segment_volume_d", "pnm_extract_topology_d",
@@ -248,6 +252,20 @@ class Context:
             self.h, _ptr(sdf), _i3(dims_xyz), _f3(origin_xyz), _f3(spacing_xyz), _ptr(u), _ptr(v),
             _ptr(w), _ptr(p), _ptr(ox), _ptr(oy), _ptr(oz), gp, ctypes.byref(r)))
         return _flow_to_numpy(r)
+
+
+    def stokes_solve_d(self, sdf, dims_xyz, spacing_xyz, force_xyz, u, v, w, p, ox=None, oy=None,
+                       oz=None, mu=1.0, rtol=1e-10, max_iter=20000):
+        """Project MAC Stokes solve into device float64 tensors u, v, w, p (and the binary
+        openness into ox, oy, oz when given). Returns (iterations, relative residual)."""
+        it = ctypes.c_int32()
+        rr = ctypes.c_double()
+        f = (ctypes.c_double * 3)(*[float(a) for a in force_xyz])
+        _check(lib().pnm_stokes_solve_d(
+            self.h, _ptr(sdf), _i3(dims_xyz), _f3(spacing_xyz), f, ctypes.c_double(mu),
+            ctypes.c_double(rtol), ctypes.c_int32(max_iter), _ptr(u), _ptr(v), _ptr(w), _ptr(p),
+            _ptr(ox), _ptr(oy), _ptr(oz), ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
 
 
 def _flow_to_numpy(r):
