@@ -7,11 +7,17 @@ timed region, the segmentation stays in HBM, the pore list and the unique connec
 written to device buffers. Prints ONE JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload polydisperse|boolean] [--size 512]
+                    [--no-batched] [--no-flow] [--no-cpu-baseline]
 
 `value` is voxels per second of the whole step (Gvox/s). `roofline` is the dominant kernel's
 algorithmic bytes per launch over its average duration, timed live with HIP events on the
 launch stream. `cpu_baseline` times the serial C oracle (1 core) on a bounded z-slab of the same
-volume.
+volume. Two more objects on the same line:
+  batched: 4096 cubic crops, edges U[48,160] at seeded offsets of the same 512^3 volume (periodic
+           wrap), segmented in one persistent launch replayed from a hipGraph with the autotuned
+           shape, against per-volume segment_volume calls on a sample of the same crops;
+  flow:    extract_network_flow on the 256^3 chamber lattice (5^3 chambers r=12 at pitch 51.2,
+           tubes r ~ U[3,8], seed 3001) with FP64 fields from the project Stokes solver.
 """
 from __future__ import annotations
 
@@ -29,11 +35,10 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic (compulsory, one streaming pass) bytes per voxel of each per-voxel kernel.
 ALGO_BYTES_PER_VOXEL = {
-    "k_pre": 12,          # read sdf 4; write marker label 4, ascent successor 4
+    "k_tile": 12,         # read sdf 4; write marker label 4, gradient-path root 4
     "k_pre_peaks": 4,     # read sdf 4
-    "k_ccl_runs": 4,      # read marker labels 4 (neighbour rows are cache hits; unions sparse)
+    "k_ccl_seams": 4,     # read marker labels 4 (neighbour rows are cache hits; unions sparse)
     "k_ccl_flatten": 8,   # read + write labels
-    "k_roots_min": 12,    # read sdf 4, successor 4; write roots 4
     "k_seg": 16,          # read sdf 4, roots 4, labels 4; write seg 4
     "k_topo_cand": 4,     # read seg 4
 }
@@ -74,6 +79,101 @@ def cpu_baseline(sdf, slab):
                       f"({crop.size} voxels) of the same SDF, {dt:.1f} s"}
 
 
+def bench_batched(ctx, src, reps, seed=4001, count=4096, lo=48, hi=160, sample=256):
+    """Batched segmentation of `count` periodic cubic crops of the device volume `src`."""
+    import numpy as np
+    import torch
+
+    dev = src.device
+    N = src.shape
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(lo, hi + 1, size=count)
+    offs = np.stack([rng.integers(0, k, size=count) for k in N], axis=1)
+    vols, segs, dims = [], [], []
+    for e, o in zip(edges, offs):
+        idx = [(torch.arange(int(e), device=dev) + int(o[a])) % N[a] for a in range(3)]
+        v = src.index_select(0, idx[0]).index_select(1, idx[1]).index_select(2, idx[2])
+        vols.append(v.contiguous().view(-1))
+        segs.append(torch.empty(int(e) ** 3, dtype=torch.int32, device=dev))
+        dims.append((int(e), int(e), int(e)))
+    total = int(sum(int(e) ** 3 for e in edges))
+    spc = (1.0, 1.0, 1.0)
+    stream = torch.cuda.current_stream(dev)
+    ctx.segment_batch_d(vols, segs, dims, spc, graph=True, autotune=True)  # tune + capture
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.segment_batch_d(vols, segs, dims, spc, graph=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # per-volume calls on a sample of the same crops; their labels must equal the batch's
+    pick = rng.choice(count, size=min(sample, count), replace=False)
+    one = [torch.empty_like(segs[k]) for k in pick]
+    torch.cuda.synchronize()
+    t = time.time()
+    for k, out in zip(pick, one):
+        ctx.segment_volume_d(vols[k], dims[k], spc, out)
+    torch.cuda.synchronize()
+    dt = time.time() - t
+    sample_vox = int(sum(int(edges[k]) ** 3 for k in pick))
+    same = all(torch.equal(segs[k], out) for k, out in zip(pick, one))
+    per_vol = sample_vox / dt / 1e9
+    value = total / (ms / 1e3) / 1e9
+    return {"workload": f"{count} periodic cubic crops, edge U[{lo},{hi}] (seed {seed}), of the "
+                        "512^3 volume above", "voxels": total, "ms": round(ms, 3),
+            "value": round(value, 3), "unit": "Gvox/s",
+            "hbm_frac": round(56 * total / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "per_volume_calls": {"value": round(per_vol, 3), "unit": "Gvox/s",
+                                 "sample_volumes": int(len(pick)), "sample_voxels": sample_vox},
+            "speedup_vs_per_volume": round(value / per_vol, 2),
+            "sample_labels_identical": bool(same), "launch": ctx.batch_config()}
+
+
+def bench_flow(ctx, reps, size=256):
+    """extract_network_flow on the chamber lattice with fields from the project Stokes solver."""
+    import numpy as np
+    import torch
+
+    from peclet.pnm import synth
+
+    dev = torch.device("cuda:0")
+    shape = (size, size, size)
+    sdf = synth.chamber_sdf(shape, m=5, chamber_r=12.0, tube_rmin=3.0, tube_rmax=8.0, seed=3001)
+    n = sdf.size
+    dims = (size, size, size)
+    s = torch.from_numpy(sdf).to(dev)
+    f = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(7)]
+    mu, fx = 1.0, 1.0
+    torch.cuda.synchronize()
+    t = time.time()
+    it, rr = ctx.stokes_solve_d(s, dims, (1, 1, 1), (fx, 0.0, 0.0), *f, mu=mu, rtol=1e-10,
+                                max_iter=40000)
+    torch.cuda.synchronize()
+    t_stokes = time.time() - t
+    args = (s, dims, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), *f[:4])
+    kw = dict(ox=f[4], oy=f[5], oz=f[6], grad_p_xyz=(-fx, 0.0, 0.0))
+    res = ctx.extract_network_flow_d(*args, **kw)  # warm-up
+    torch.cuda.synchronize()
+    t = time.time()
+    for _ in range(reps):
+        res = ctx.extract_network_flow_d(*args, **kw)
+    torch.cuda.synchronize()
+    ms = (time.time() - t) / reps * 1e3
+    q = np.abs(res["throat_flow"])
+    return {"workload": "256^3 chamber lattice: 5^3 chambers r=12 at pitch 51.2, capsule tubes "
+                        "r ~ U[3,8] (seed 3001); binary openness",
+            "ms": round(ms, 3), "value": round(n / (ms / 1e3) / 1e9, 3), "unit": "Gvox/s",
+            "hbm_frac": round(172 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pores": int(len(res["pores"])), "throats": int(len(q)),
+            "max_residual_over_max_q": float(np.abs(res["pore_residual"]).max() / q.max())
+            if len(q) else None,
+            "stokes": {"solver": "MAC, FP64 preconditioned MINRES", "iterations": it,
+                       "rel_residual": rr, "seconds": round(t_stokes, 2), "mu": mu,
+                       "body_force_x": fx}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +184,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2001)
     ap.add_argument("--cpu-slab", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-flow", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU only (the 512^3 volume is one GPU's block); "
@@ -169,6 +271,13 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sdf, min(args.cpu_slab, shape[0]))
     else:
         out["cpu_baseline"] = None
+    if not args.no_batched:
+        seg = pairs = pores = None
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(ctx, s.view(shape), reps=3)
+    if not args.no_flow:
+        torch.cuda.empty_cache()
+        out["flow"] = bench_flow(ctx, reps=3)
     print(json.dumps(out), flush=True)
 
 
