@@ -128,6 +128,10 @@ typedef struct {
   double* throat_flow;    /* [nt]  positive li -> lj */
   double* throat_area;    /* [nt]  sum of openness * face area */
   double* throat_dp;      /* [nt]  P_i - P_j including the macroscopic gradient */
+  double* throat_center;  /* [3*nt] X,Y,Z world position of the interface centroid: the lower
+                             pore's peak-voxel centre plus the min-image centroid offset C/A
+                             (the anchor throat_dp uses, :962-991); not part of the reference's
+                             NetworkFlow, returned for throat-level models */
 } pnm_flow_result_t;
 
 /* extract_network_flow_k (pore_extraction.hpp:493-997). u,v,w,p: float64[n] MAC fields
@@ -211,6 +215,34 @@ pnm_status pnm_stokes_solve_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t d
                               double rtol, int32_t max_iter, double* u_d, double* v_d,
                               double* w_d, double* p_d, double* ox_d, double* oy_d, double* oz_d,
                               int32_t* iterations, double* rel_residual);
+
+/* ---- throat-conductance surrogate (project extension, no reference counterpart: a learned
+ * g = Q/dp per throat, trained on extract_network_flow's own output, which stays its fallback)
+ * Network: 16^3 SDF patch -> three 3^3 convolutions (1->16->32->32, ReLU, 2x average pools,
+ * global average) + 4 throat scalars -> dense 36->64 (ReLU) -> 1 = log g. Convolutions run on
+ * the BF16 MFMA with FP32 accumulation; parameters are one flat FP32 vector.
+ * patches_d: float[batch][16][16][16] (z, y, x); scalars_d: float[batch][4]; out_d: [batch]. */
+int64_t pnm_sur_param_count(void);
+/* He-normal weights, zero biases, from a seed; params: host float[count]. */
+pnm_status pnm_sur_init(uint64_t seed, float* params, int64_t count);
+/* Forward pass; keeps its activations in the context for the next backward call. */
+pnm_status pnm_sur_forward_d(pnm_ctx_t ctx, int32_t batch, const float* patches_d,
+                             const float* scalars_d, const float* params_d, float* out_d);
+/* Gradients of sum_b dout[b] * out[b] w.r.t. the parameters, for the context's last forward
+ * pass (same batch); grads_d: float[pnm_sur_param_count()] (overwritten). */
+pnm_status pnm_sur_backward_d(pnm_ctx_t ctx, int32_t batch, const float* dout_d,
+                              const float* params_d, float* grads_d);
+/* Mean squared error over the batch; dpred = 2 (pred - target) / batch; loss_d nullable. */
+pnm_status pnm_sur_loss_d(pnm_ctx_t ctx, int32_t batch, const float* pred_d,
+                          const float* target_d, float* dpred_d, float* loss_d);
+/* One Adam step (bias-corrected, step >= 1) on the flat parameter vector. */
+pnm_status pnm_sur_adam_d(pnm_ctx_t ctx, float* params_d, const float* grads_d, float* m_d,
+                          float* v_d, int32_t step, float lr, float beta1, float beta2,
+                          float eps);
+/* Throat patches: out[b] = sdf / scale on the 16^3 voxels starting at floor(center_b) - 7
+ * (periodic); centers_d: float[batch][3], X,Y,Z in voxel units. */
+pnm_status pnm_sur_patches_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                             const float* centers_d, int32_t batch, float scale, float* out_d);
 
 /* ---- stage-level device entry points (parity tests of single kernels; no reference
  * counterpart as separate calls: each is one step of segmentVolumeView) ---- */

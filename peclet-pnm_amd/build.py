@@ -51,6 +51,7 @@ SOURCES = [
     "k_flow.hip",
     "k_tile.hip",
     "k_stokes.hip",
+    "k_surrogate.hip",
     "k_batch.hip",
     "pipeline.hip",
     "capi.cpp",

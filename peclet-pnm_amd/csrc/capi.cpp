@@ -218,6 +218,8 @@ void pnm_ctx_destroy(pnm_ctx_t ctx) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->arena.release();
+  c->sur.release();
+  surrogate_release(c);
   for (auto& r : c->recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -510,6 +512,73 @@ pnm_status pnm_stokes_solve_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t d
   });
 }
 
+// ------------------------- throat-conductance surrogate -------------------------
+
+int64_t pnm_sur_param_count(void) { return surrogate_param_count(); }
+
+pnm_status pnm_sur_init(uint64_t seed, float* params, int64_t count) {
+  return guarded([&] {
+    need(params != nullptr, "params must not be NULL");
+    need(count == surrogate_param_count(), "count must equal pnm_sur_param_count()");
+    surrogate_init(seed, params);
+  });
+}
+
+pnm_status pnm_sur_forward_d(pnm_ctx_t ctx, int32_t batch, const float* patches_d,
+                             const float* scalars_d, const float* params_d, float* out_d) {
+  return guarded([&] {
+    need(batch > 0 && patches_d && scalars_d && params_d && out_d,
+         "batch > 0 and all buffers are required");
+    Ctx* c = check_ctx(ctx);
+    CallScope cs(c);
+    surrogate_forward(c, batch, patches_d, scalars_d, params_d, out_d);
+  });
+}
+
+pnm_status pnm_sur_backward_d(pnm_ctx_t ctx, int32_t batch, const float* dout_d,
+                              const float* params_d, float* grads_d) {
+  return guarded([&] {
+    need(batch > 0 && dout_d && params_d && grads_d, "batch > 0 and all buffers are required");
+    Ctx* c = check_ctx(ctx);
+    CallScope cs(c);
+    surrogate_backward(c, batch, dout_d, params_d, grads_d);
+  });
+}
+
+pnm_status pnm_sur_loss_d(pnm_ctx_t ctx, int32_t batch, const float* pred_d,
+                          const float* target_d, float* dpred_d, float* loss_d) {
+  return guarded([&] {
+    need(batch > 0 && pred_d && target_d && dpred_d, "batch > 0 and buffers are required");
+    Ctx* c = check_ctx(ctx);
+    CallScope cs(c);
+    surrogate_loss(c, batch, pred_d, target_d, dpred_d, loss_d);
+  });
+}
+
+pnm_status pnm_sur_adam_d(pnm_ctx_t ctx, float* params_d, const float* grads_d, float* m_d,
+                          float* v_d, int32_t step, float lr, float beta1, float beta2,
+                          float eps) {
+  return guarded([&] {
+    need(params_d && grads_d && m_d && v_d && step >= 1, "buffers required, step >= 1");
+    Ctx* c = check_ctx(ctx);
+    CallScope cs(c);
+    surrogate_adam(c, params_d, grads_d, m_d, v_d, step, lr, beta1, beta2, eps);
+  });
+}
+
+pnm_status pnm_sur_patches_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                             const float* centers_d, int32_t batch, float scale, float* out_d) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(v.n > 0 && sdf_d && centers_d && out_d && batch >= 0 && scale > 0.0f,
+         "non-empty volume, buffers and scale > 0 are required");
+    Ctx* c = check_ctx(ctx);
+    if (batch == 0) return;
+    CallScope cs(c);
+    surrogate_patches(c, sdf_d, v, centers_d, batch, scale, out_d);
+  });
+}
+
 // ------------------------- host-pointer entry points -------------------------
 
 pnm_status pnm_extract_pores(pnm_ctx_t ctx, const float* sdf, const int32_t dims[3],
@@ -662,6 +731,7 @@ void pnm_flow_result_free(pnm_flow_result_t* r) {
   std::free(r->throat_flow);
   std::free(r->throat_area);
   std::free(r->throat_dp);
+  std::free(r->throat_center);
   std::memset(r, 0, sizeof(*r));
 }
 

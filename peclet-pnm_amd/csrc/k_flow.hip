@@ -442,7 +442,7 @@ __global__ void k_throat_dp(const int32_t* __restrict__ throats, int64_t nt,
                             const double* __restrict__ Cx, const double* __restrict__ Cy,
                             const double* __restrict__ Cz, Geo g, float ox, float oy, float oz,
                             float sx, float sy, float sz, double gpx, double gpy, double gpz,
-                            double* __restrict__ dp) {
+                            double* __restrict__ dp, double* __restrict__ tc) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const int li = throats[2 * t], lj = throats[2 * t + 1];
@@ -463,6 +463,7 @@ __global__ void k_throat_dp(const int32_t* __restrict__ throats, int64_t nt,
     const double anc = o[a] + pki[a] * (double)s[a];
     const double ancj = o[a] + pkj3[a] * (double)s[a];
     const double ct = (Cx[t] * (a == 0) + Cy[t] * (a == 1) + Cz[t] * (a == 2)) / aw;
+    tc[3 * t + a] = anc + ct;  // interface centroid: anchor + min-image offset
     const long long Dc = (long long)round((ancj - anc) / (double)s[a]) -
                          (long long)round(ct / (double)s[a] + 1e-6);
     const long long k = (long long)round((double)Dc / N[a]);
@@ -589,6 +590,7 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   double* Cy = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* Cz = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* dp = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* tc = c->arena.get_n<double>((size_t)std::max<int64_t>(3 * nt, 1));
   double* resid = c->arena.get_n<double>((size_t)np);
   FluxArgs fa{g, lab, u, vv, w, ox, oy, oz, (double)spc[1] * spc[2], (double)spc[0] * spc[2],
               (double)spc[0] * spc[1]};
@@ -629,7 +631,7 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
                fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], Q, Ar, Cx, Cy, Cz);
     PNM_LAUNCH(c, "k_throat_dp", k_throat_dp, dim3(cdiv(nt, 256)), dim3(256), 0, throats, nt,
                pores, pp, peak, Ar, Cx, Cy, Cz, g, org[0], org[1], org[2], spc[0], spc[1],
-               spc[2], grad_p[0], grad_p[1], grad_p[2], dp);
+               spc[2], grad_p[0], grad_p[1], grad_p[2], dp, tc);
   }
   // residual
   {
@@ -655,6 +657,7 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   out->throat_flow = host_array(c, Q, nt);
   out->throat_area = host_array(c, Ar, nt);
   out->throat_dp = host_array(c, dp, nt);
+  out->throat_center = host_array(c, tc, 3 * nt);
   PNM_HIP(hipStreamSynchronize(c->stream));
 }
 

@@ -1,0 +1,533 @@
+// k_surrogate.hip — throat-conductance surrogate: a small 3D CNN trained and run with the
+// project's own kernels. No reference counterpart: the plain bookkeeping of
+// extract_network_flow (pore_extraction.hpp:463-997) produces its targets g = Q/dp and stays
+// its fallback.
+//
+// Network (parameters FP32, ~44.5k):
+//   16^3 SDF patch -> conv3^3 1->16, ReLU, avg-pool 2 -> conv3^3 16->32, ReLU, avg-pool 2
+//   -> conv3^3 32->32, ReLU, global average -> [32 features, 4 throat scalars]
+//   -> dense 36->64, ReLU -> dense 64->1 = log g.
+// Convolutions (zero padding 1) are implicit GEMMs on the MFMA: activations and weights are
+// rounded to BF16, products accumulate in FP32 (v_mfma_f32_16x16x16_bf16). Forward
+//   z[p][co] = sum_k col[p][k] W[co][k]                 (col = im2col of the input, k = tap*CI+ci)
+// and backward
+//   dW[co][k] = sum_p dz[p][co] col[p][k]               (split over p, fixed-order reduction)
+//   dcol[p][k] = sum_co dz[p][co] W[co][k],  dX = col2im(dcol)  (gather: no atomics)
+// all as one "NT" GEMM C[M][N] = A[M][K] . Bt[N][K]^T with K-contiguous BF16 operands.
+// Activations are channels-last: [batch][z][y][x][channel]. The dense head, pooling, loss and
+// Adam run in FP32.
+#include "k_common.hpp"
+
+namespace pnm {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Round-to-nearest-even FP32 -> BF16 (bits), NaN kept quiet.
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ---------------- NT GEMM on the MFMA ----------------
+// C_part[s][m][n] = sum_{k in split s} A[m][k] * Bt[n][k]. M, N, K multiples of 16; each wave
+// owns one 16x16 tile of one split. Lane l feeds A[m0 + l%16][k + 4*(l/16) .. +3] and
+// Bt[n0 + l%16][k + 4*(l/16) .. +3] and receives C[m0 + 4*(l/16) + i][n0 + l%16], i = 0..3.
+__global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
+                                                 const uint16_t* __restrict__ Bt,
+                                                 float* __restrict__ Cp, int M, int N, int K,
+                                                 int kchunk) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tiles_n = N >> 4;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + wv;
+  const int64_t ntiles = (int64_t)(M >> 4) * tiles_n;
+  if (tile >= ntiles) return;  // whole wave
+  const int m0 = (int)(tile / tiles_n) << 4, n0 = (int)(tile % tiles_n) << 4;
+  const int s = blockIdx.y;
+  const int k_lo = s * kchunk, k_hi = min(K, k_lo + kchunk);
+  const int r = lane & 15, q = (lane >> 4) * 4;
+  const uint16_t* __restrict__ ap = A + (int64_t)(m0 + r) * K + q;
+  const uint16_t* __restrict__ bp = Bt + (int64_t)(n0 + r) * K + q;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k = k_lo; k < k_hi; k += 16) {
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(ap + k);
+    const bf16x4 b = *reinterpret_cast<const bf16x4*>(bp + k);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  float* __restrict__ cp = Cp + (int64_t)s * M * N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) cp[(int64_t)(m0 + q + i) * N + n0 + r] = acc[i];
+}
+
+// C[m][n] = sum_s Cp[s][m][n] (+ bias[n]) (ReLU), FP32; optional BF16 copy.
+__global__ void k_split_reduce(const float* __restrict__ Cp, int splits, int64_t MN, int N,
+                               const float* __restrict__ bias, int relu, float* __restrict__ C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MN) return;
+  float v = 0.f;
+  for (int s = 0; s < splits; ++s) v += Cp[(int64_t)s * MN + i];
+  if (bias) v += bias[i % N];
+  if (relu) v = v > 0.f ? v : 0.f;
+  C[i] = v;
+}
+
+// im2col with zero padding: col[p][k] (and colT[k][p]) for p = (b, z, y, x) of a D^3 grid,
+// k = tap*CI + ci (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)), zero for k >= 27*CI (K padding).
+__global__ void k_im2col(const float* __restrict__ act, int D, int CI, int Kp, int64_t P,
+                         uint16_t* __restrict__ col, uint16_t* __restrict__ colT) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * Kp) return;
+  const int64_t p = i / Kp;
+  const int k = (int)(i - p * Kp);
+  float v = 0.f;
+  if (k < 27 * CI) {
+    const int tap = k / CI, ci = k - tap * CI;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int D3 = D * D * D;
+    const int64_t b = p / D3;
+    const int r = (int)(p - b * D3);
+    const int x = r % D + dx, y = (r / D) % D + dy, z = r / (D * D) + dz;
+    if (x >= 0 && x < D && y >= 0 && y < D && z >= 0 && z < D)
+      v = act[(b * D3 + (int64_t)(z * D + y) * D + x) * CI + ci];
+  }
+  const uint16_t h = f2bf(v);
+  col[i] = h;
+  if (colT) colT[(int64_t)k * P + p] = h;
+}
+
+// dX[b][q][ci] = sum_tap dcol[b][q - off(tap)][tap*CI + ci] over in-range positions.
+__global__ void k_col2im(const float* __restrict__ dcol, int D, int CI, int Kp, int64_t P,
+                         float* __restrict__ dx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * CI) return;
+  const int64_t q = i / CI;
+  const int ci = (int)(i - q * CI);
+  const int D3 = D * D * D;
+  const int64_t b = q / D3;
+  const int r = (int)(q - b * D3);
+  const int x = r % D, y = (r / D) % D, z = r / (D * D);
+  float acc = 0.f;
+  for (int tap = 0; tap < 27; ++tap) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, ddx = tap % 3 - 1;
+    const int px = x - ddx, py = y - dy, pz = z - dz;
+    if (px < 0 || px >= D || py < 0 || py >= D || pz < 0 || pz >= D) continue;
+    const int64_t p = b * D3 + (int64_t)(pz * D + py) * D + px;
+    acc += dcol[p * Kp + tap * CI + ci];
+  }
+  dx[i] = acc;
+}
+
+// 2x2x2 average pooling, channels-last, D -> D/2.
+__global__ void k_pool_fwd(const float* __restrict__ a, int B, int D, int C, float* __restrict__ o) {
+  const int h = D / 2;
+  const int64_t n = (int64_t)B * h * h * h * C;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  int64_t t = i / C;
+  const int x = (int)(t % h), y = (int)((t / h) % h), z = (int)((t / ((int64_t)h * h)) % h);
+  const int64_t b = t / ((int64_t)h * h * h);
+  float s = 0.f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx)
+        s += a[((b * D + 2 * z + dz) * D * D + (int64_t)(2 * y + dy) * D + 2 * x + dx) * C + c];
+  o[i] = s * 0.125f;
+}
+
+// da = pool_bwd(dp) masked by ReLU (a > 0); writes dz in FP32, BF16 [P][C] and BF16 [C][P].
+__global__ void k_pool_bwd_relu(const float* __restrict__ dp, const float* __restrict__ a, int B,
+                                int D, int C, float* __restrict__ dz, uint16_t* __restrict__ dzb,
+                                uint16_t* __restrict__ dzT) {
+  const int64_t P = (int64_t)B * D * D * D;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * C) return;
+  const int64_t p = i / C;
+  const int c = (int)(i - p * C);
+  const int D3 = D * D * D, h = D / 2;
+  const int64_t b = p / D3;
+  const int r = (int)(p - b * D3);
+  const int x = r % D, y = (r / D) % D, z = r / (D * D);
+  const float g = dp[((b * h + z / 2) * h * h + (int64_t)(y / 2) * h + x / 2) * C + c] * 0.125f;
+  const float v = a[i] > 0.f ? g : 0.f;
+  dz[i] = v;
+  const uint16_t hb = f2bf(v);
+  dzb[i] = hb;
+  dzT[(int64_t)c * P + p] = hb;
+}
+
+// Global average of a [B][S][C] activation -> feat[B][36] (first C entries); the other 4 are
+// the throat scalars.
+__global__ void k_gap_concat(const float* __restrict__ a, int B, int S, int C,
+                             const float* __restrict__ scal, float* __restrict__ h) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * (C + 4)) return;
+  const int64_t b = i / (C + 4);
+  const int j = (int)(i - b * (C + 4));
+  if (j < C) {
+    float s = 0.f;
+    for (int k = 0; k < S; ++k) s += a[(b * S + k) * C + j];
+    h[i] = s / (float)S;
+  } else {
+    h[i] = scal[b * 4 + (j - C)];
+  }
+}
+
+// dz3 = (dg / S) masked by ReLU(a3 > 0).
+__global__ void k_gap_bwd_relu(const float* __restrict__ dh, const float* __restrict__ a, int B,
+                               int S, int C, float* __restrict__ dz, uint16_t* __restrict__ dzb,
+                               uint16_t* __restrict__ dzT) {
+  const int64_t P = (int64_t)B * S;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P * C) return;
+  const int64_t p = i / C;
+  const int c = (int)(i - p * C);
+  const int64_t b = p / S;
+  const float g = dh[b * (C + 4) + c] / (float)S;
+  const float v = a[i] > 0.f ? g : 0.f;
+  dz[i] = v;
+  const uint16_t hb = f2bf(v);
+  dzb[i] = hb;
+  dzT[(int64_t)c * P + p] = hb;
+}
+
+// Dense head forward: r4 = ReLU(W4 h + b4) [B][64]; y = W5 r4 + b5.
+__global__ void k_head_fwd(const float* __restrict__ h, const float* __restrict__ W4,
+                           const float* __restrict__ b4, const float* __restrict__ W5,
+                           const float* __restrict__ b5, int B, float* __restrict__ r4,
+                           float* __restrict__ y) {
+  __shared__ float sh[64];
+  const int b = blockIdx.x, j = threadIdx.x;  // 64 threads
+  float s = b4[j];
+  for (int k = 0; k < 36; ++k) s += W4[j * 36 + k] * h[(int64_t)b * 36 + k];
+  s = s > 0.f ? s : 0.f;
+  r4[(int64_t)b * 64 + j] = s;
+  sh[j] = W5[j] * s;
+  __syncthreads();
+  if (j == 0) {
+    float t = b5[0];
+    for (int k = 0; k < 64; ++k) t += sh[k];
+    y[b] = t;
+  }
+  (void)B;
+}
+
+// MSE loss over the batch and dy = 2 (y - t) / B; loss written by block 0 (fixed order).
+__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, int B,
+                      float* __restrict__ dy, float* __restrict__ loss) {
+  __shared__ float sh[256];
+  float acc = 0.f;
+  for (int b = threadIdx.x; b < B; b += blockDim.x) {
+    const float e = y[b] - t[b];
+    dy[b] = 2.f * e / (float)B;
+    acc += e * e;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss) *loss = sh[0] / (float)B;
+}
+
+// Dense head backward, part 1 (one block per sample): dz4 = (dy W5) masked, dh = W4^T dz4.
+__global__ void k_head_bwd_act(const float* __restrict__ dy, const float* __restrict__ r4,
+                               const float* __restrict__ W4, const float* __restrict__ W5,
+                               float* __restrict__ dz4, float* __restrict__ dh) {
+  __shared__ float sz[64];
+  const int b = blockIdx.x, j = threadIdx.x;  // 64 threads
+  const float g = r4[(int64_t)b * 64 + j] > 0.f ? dy[b] * W5[j] : 0.f;
+  dz4[(int64_t)b * 64 + j] = g;
+  sz[j] = g;
+  __syncthreads();
+  if (j < 36) {
+    float s = 0.f;
+    for (int k = 0; k < 64; ++k) s += W4[k * 36 + j] * sz[k];
+    dh[(int64_t)b * 36 + j] = s;
+  }
+}
+
+// Dense head backward, part 2 (weight gradients, summed over the batch in order):
+// threads 0..2303 -> dW4[j][k], 2304..2367 -> db4[j], 2368..2431 -> dW5[j], 2432 -> db5.
+__global__ void k_head_bwd_w(const float* __restrict__ dy, const float* __restrict__ r4,
+                             const float* __restrict__ h, const float* __restrict__ dz4, int B,
+                             float* __restrict__ gW4, float* __restrict__ gb4,
+                             float* __restrict__ gW5, float* __restrict__ gb5) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float s = 0.f;
+  if (i < 2304) {
+    const int j = i / 36, k = i % 36;
+    for (int b = 0; b < B; ++b) s += dz4[(int64_t)b * 64 + j] * h[(int64_t)b * 36 + k];
+    gW4[i] = s;
+  } else if (i < 2368) {
+    const int j = i - 2304;
+    for (int b = 0; b < B; ++b) s += dz4[(int64_t)b * 64 + j];
+    gb4[j] = s;
+  } else if (i < 2432) {
+    const int j = i - 2368;
+    for (int b = 0; b < B; ++b) s += dy[b] * r4[(int64_t)b * 64 + j];
+    gW5[j] = s;
+  } else if (i == 2432) {
+    for (int b = 0; b < B; ++b) s += dy[b];
+    gb5[0] = s;
+  }
+}
+
+// Bias gradient: db[c] = sum_p dz[p][c] (one block per channel, fixed order).
+__global__ void k_bias_grad(const float* __restrict__ dz, int64_t P, int C, float* __restrict__ db) {
+  __shared__ float sh[256];
+  const int c = blockIdx.x;
+  float acc = 0.f;
+  for (int64_t p = threadIdx.x; p < P; p += blockDim.x) acc += dz[p * C + c];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// BF16 copies of a conv weight Wt[co][Kp]: as is (forward Bt) and transposed [Kp][co].
+__global__ void k_weight_bf16(const float* __restrict__ W, int CO, int Kp,
+                              uint16_t* __restrict__ wb, uint16_t* __restrict__ wT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= CO * Kp) return;
+  const int co = i / Kp, k = i - co * Kp;
+  const uint16_t h = f2bf(W[i]);
+  wb[i] = h;
+  wT[k * CO + co] = h;
+}
+
+// Adam (Kingma & Ba 2015) with bias correction, FP32.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// 16^3 SDF patch (periodic, nearest voxel) centred on each throat centre (voxel coordinates,
+// xyz), scaled by 1/scale.
+__global__ void k_patches(const float* __restrict__ sdf, int nx, int ny, int nz,
+                          const float* __restrict__ centers, int B, float scale,
+                          float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * 4096) return;
+  const int64_t b = i >> 12;
+  const int r = (int)(i & 4095);
+  const int px = r & 15, py = (r >> 4) & 15, pz = r >> 8;
+  const int cx = (int)floorf(centers[b * 3 + 0]) - 7, cy = (int)floorf(centers[b * 3 + 1]) - 7,
+            cz = (int)floorf(centers[b * 3 + 2]) - 7;
+  const int x = wrap_any(cx + px, nx), y = wrap_any(cy + py, ny), z = wrap_any(cz + pz, nz);
+  out[i] = sdf[((int64_t)z * ny + y) * nx + x] / scale;
+}
+
+// ---------------- host orchestration ----------------
+struct ConvSpec {
+  int D, CI, CO, Kp;
+};
+static const ConvSpec kConv[3] = {{16, 1, 16, 32}, {8, 16, 32, 432}, {4, 32, 32, 864}};
+
+// parameter offsets in the flat FP32 vector
+struct SurLayout {
+  int64_t w[3], b[3], W4, b4, W5, b5, total;
+};
+static SurLayout sur_layout() {
+  SurLayout L{};
+  int64_t o = 0;
+  for (int l = 0; l < 3; ++l) {
+    L.w[l] = o;
+    o += (int64_t)kConv[l].CO * kConv[l].Kp;
+    L.b[l] = o;
+    o += kConv[l].CO;
+  }
+  L.W4 = o;
+  o += 64 * 36;
+  L.b4 = o;
+  o += 64;
+  L.W5 = o;
+  o += 64;
+  L.b5 = o;
+  o += 1;
+  L.total = o;
+  return L;
+}
+
+int64_t surrogate_param_count() { return sur_layout().total; }
+
+// He-normal initial weights (std::mt19937_64 + Box-Muller), zero biases; padded K columns zero.
+void surrogate_init(uint64_t seed, float* host) {
+  const SurLayout L = sur_layout();
+  std::fill(host, host + L.total, 0.0f);
+  uint64_t s = seed ? seed : 0x9e3779b97f4a7c15ull;
+  auto next = [&]() {  // splitmix64
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  auto normal = [&]() {
+    const double u1 = ((next() >> 11) + 1.0) / 9007199254740993.0;
+    const double u2 = (next() >> 11) / 9007199254740992.0;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  };
+  for (int l = 0; l < 3; ++l) {
+    const ConvSpec& c = kConv[l];
+    const double sd = std::sqrt(2.0 / (27.0 * c.CI));
+    for (int co = 0; co < c.CO; ++co)
+      for (int k = 0; k < 27 * c.CI; ++k) host[L.w[l] + (int64_t)co * c.Kp + k] = (float)(sd * normal());
+  }
+  for (int i = 0; i < 64 * 36; ++i) host[L.W4 + i] = (float)(std::sqrt(2.0 / 36.0) * normal());
+  for (int i = 0; i < 64; ++i) host[L.W5 + i] = (float)(std::sqrt(1.0 / 64.0) * normal());
+}
+
+// Activations of the last forward pass (kept for backward) live in the context's surrogate
+// workspace; their pointers are remembered in SurActs.
+struct SurActs {
+  int B = 0;
+  float* x0 = nullptr;
+  uint16_t* col[3] = {};
+  uint16_t* colT[3] = {};
+  float* a[3] = {};   // post-ReLU conv outputs [P][CO]
+  float* p[2] = {};   // pooled
+  float* h = nullptr;  // [B][36]
+  float* r4 = nullptr;
+  float* y = nullptr;
+};
+
+static SurActs& acts(Ctx* c) {
+  if (!c->sur_acts) c->sur_acts = new SurActs();
+  return *static_cast<SurActs*>(c->sur_acts);
+}
+
+static void gemm_nt(Ctx* c, const uint16_t* A, const uint16_t* Bt, float* C, int64_t M, int N,
+                    int K, const float* bias, bool relu) {
+  // split K so that a launch has at least ~2048 waves
+  const int64_t tiles = (M / 16) * (N / 16);
+  int splits = 1;
+  while (tiles * splits < 2048 && K / (16 * splits * 2) >= 4) splits *= 2;
+  const int kchunk = ((K / 16 + splits - 1) / splits) * 16;
+  splits = (K + kchunk - 1) / kchunk;
+  float* Cp = splits == 1 && !bias && !relu ? C : c->arena.get_n<float>((size_t)(splits * M * N));
+  PNM_LAUNCH(c, "k_gemm_nt", k_gemm_nt, dim3(cdiv(tiles, 4), splits), dim3(256), 0, A, Bt, Cp,
+             (int)M, N, K, kchunk);
+  if (Cp != C)
+    PNM_LAUNCH(c, "k_split_reduce", k_split_reduce, dim3(cdiv(M * N, 256)), dim3(256), 0, Cp,
+               splits, M * N, N, bias, relu ? 1 : 0, C);
+}
+
+void surrogate_forward(Ctx* c, int B, const float* patches, const float* scalars,
+                       const float* params, float* out) {
+  const SurLayout L = sur_layout();
+  SurActs& S = acts(c);
+  c->sur.reset();
+  S.B = B;
+  S.x0 = const_cast<float*>(patches);
+  const float* in = patches;
+  for (int l = 0; l < 3; ++l) {
+    const ConvSpec& cs = kConv[l];
+    const int64_t P = (int64_t)B * cs.D * cs.D * cs.D;
+    S.col[l] = c->sur.get_n<uint16_t>((size_t)(P * cs.Kp));
+    S.colT[l] = c->sur.get_n<uint16_t>((size_t)(P * cs.Kp));
+    S.a[l] = c->sur.get_n<float>((size_t)(P * cs.CO));
+    PNM_LAUNCH(c, "k_im2col", k_im2col, dim3(cdiv(P * cs.Kp, 256)), dim3(256), 0, in, cs.D,
+               cs.CI, cs.Kp, P, S.col[l], S.colT[l]);
+    uint16_t* wb = c->arena.get_n<uint16_t>((size_t)(cs.CO * cs.Kp));
+    uint16_t* wT = c->arena.get_n<uint16_t>((size_t)(cs.CO * cs.Kp));
+    PNM_LAUNCH(c, "k_weight_bf16", k_weight_bf16, dim3(cdiv(cs.CO * cs.Kp, 256)), dim3(256), 0,
+               params + L.w[l], cs.CO, cs.Kp, wb, wT);
+    gemm_nt(c, S.col[l], wb, S.a[l], P, cs.CO, cs.Kp, params + L.b[l], true);
+    if (l < 2) {
+      const int h = cs.D / 2;
+      S.p[l] = c->sur.get_n<float>((size_t)((int64_t)B * h * h * h * cs.CO));
+      PNM_LAUNCH(c, "k_pool_fwd", k_pool_fwd, dim3(cdiv((int64_t)B * h * h * h * cs.CO, 256)),
+                 dim3(256), 0, S.a[l], B, cs.D, cs.CO, S.p[l]);
+      in = S.p[l];
+    }
+  }
+  S.h = c->sur.get_n<float>((size_t)B * 36);
+  S.r4 = c->sur.get_n<float>((size_t)B * 64);
+  PNM_LAUNCH(c, "k_gap_concat", k_gap_concat, dim3(cdiv((int64_t)B * 36, 256)), dim3(256), 0,
+             S.a[2], B, 64, 32, scalars, S.h);
+  PNM_LAUNCH(c, "k_head_fwd", k_head_fwd, dim3(B), dim3(64), 0, S.h, params + L.W4,
+             params + L.b4, params + L.W5, params + L.b5, B, S.r4, out);
+  S.y = out;
+}
+
+void surrogate_backward(Ctx* c, int B, const float* dy, const float* params, float* grads) {
+  const SurLayout L = sur_layout();
+  SurActs& S = acts(c);
+  if (S.B != B || !S.h) fail(PNM_EINVAL, "backward needs the forward pass of the same batch");
+  float* dz4 = c->arena.get_n<float>((size_t)B * 64);
+  float* dh = c->arena.get_n<float>((size_t)B * 36);
+  PNM_LAUNCH(c, "k_head_bwd_act", k_head_bwd_act, dim3(B), dim3(64), 0, dy, S.r4, params + L.W4,
+             params + L.W5, dz4, dh);
+  PNM_LAUNCH(c, "k_head_bwd_w", k_head_bwd_w, dim3(cdiv(2433, 256)), dim3(256), 0, dy, S.r4, S.h,
+             dz4, B, grads + L.W4, grads + L.b4, grads + L.W5, grads + L.b5);
+  float* dnext = nullptr;  // gradient w.r.t. the pooled input of layer l+1
+  for (int l = 2; l >= 0; --l) {
+    const ConvSpec& cs = kConv[l];
+    const int64_t P = (int64_t)B * cs.D * cs.D * cs.D;
+    float* dz = c->arena.get_n<float>((size_t)(P * cs.CO));
+    uint16_t* dzb = c->arena.get_n<uint16_t>((size_t)(P * cs.CO));
+    uint16_t* dzT = c->arena.get_n<uint16_t>((size_t)(P * cs.CO));
+    if (l == 2)
+      PNM_LAUNCH(c, "k_gap_bwd_relu", k_gap_bwd_relu, dim3(cdiv(P * cs.CO, 256)), dim3(256), 0, dh,
+                 S.a[2], B, 64, cs.CO, dz, dzb, dzT);
+    else
+      PNM_LAUNCH(c, "k_pool_bwd_relu", k_pool_bwd_relu, dim3(cdiv(P * cs.CO, 256)), dim3(256), 0,
+                 dnext, S.a[l], B, cs.D, cs.CO, dz, dzb, dzT);
+    PNM_LAUNCH(c, "k_bias_grad", k_bias_grad, dim3(cs.CO), dim3(256), 0, dz, P, cs.CO,
+               grads + L.b[l]);
+    // dW[co][k] = sum_p dzT[co][p] colT[k][p]
+    gemm_nt(c, dzT, S.colT[l], grads + L.w[l], cs.CO, cs.Kp, (int)P, nullptr, false);
+    if (l > 0) {
+      uint16_t* wb = c->arena.get_n<uint16_t>((size_t)(cs.CO * cs.Kp));
+      uint16_t* wT = c->arena.get_n<uint16_t>((size_t)(cs.CO * cs.Kp));
+      PNM_LAUNCH(c, "k_weight_bf16", k_weight_bf16, dim3(cdiv(cs.CO * cs.Kp, 256)), dim3(256), 0,
+                 params + L.w[l], cs.CO, cs.Kp, wb, wT);
+      float* dcol = c->arena.get_n<float>((size_t)(P * cs.Kp));
+      gemm_nt(c, dzb, wT, dcol, P, cs.Kp, cs.CO, nullptr, false);
+      float* dx = c->arena.get_n<float>((size_t)(P * cs.CI));
+      PNM_LAUNCH(c, "k_col2im", k_col2im, dim3(cdiv(P * cs.CI, 256)), dim3(256), 0, dcol, cs.D,
+                 cs.CI, cs.Kp, P, dx);
+      dnext = dx;
+    }
+  }
+}
+
+void surrogate_loss(Ctx* c, int B, const float* y, const float* t, float* dy, float* loss) {
+  PNM_LAUNCH(c, "k_mse", k_mse, dim3(1), dim3(256), 0, y, t, B, dy, loss);
+}
+
+void surrogate_adam(Ctx* c, float* params, const float* grads, float* m, float* v, int step,
+                    float lr, float b1, float b2, float eps) {
+  const int64_t n = surrogate_param_count();
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  PNM_LAUNCH(c, "k_adam", k_adam, dim3(cdiv(n, 256)), dim3(256), 0, params, grads, m, v, n, lr,
+             b1, b2, eps, c1, c2);
+}
+
+void surrogate_patches(Ctx* c, const float* sdf, Vol v, const float* centers, int B, float scale,
+                       float* out) {
+  PNM_LAUNCH(c, "k_patches", k_patches, dim3(cdiv((int64_t)B * 4096, 256)), dim3(256), 0, sdf,
+             v.nx, v.ny, v.nz, centers, B, scale, out);
+}
+
+void surrogate_release(Ctx* c) {
+  delete static_cast<SurActs*>(c->sur_acts);
+  c->sur_acts = nullptr;
+}
+
+}  // namespace pnm

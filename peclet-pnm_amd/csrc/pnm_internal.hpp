@@ -72,6 +72,9 @@ struct Ctx {
   float batch_tuned_ms = 0.0f;
   void* batch_graph = nullptr;  // hipGraphExec_t
   uint64_t batch_key = 0;
+  // Throat surrogate: activations of the last forward pass, kept for its backward pass.
+  Arena sur;
+  void* sur_acts = nullptr;
 };
 
 struct KTimer {
@@ -247,5 +250,18 @@ struct StokesInfo {
 StokesInfo stokes_device(Ctx* c, const float* sdf, Vol v, const float spc[3],
                          const double force[3], double mu, double rtol, int max_iter, double* u,
                          double* vv, double* w, double* p, double* ox, double* oy, double* oz);
+
+// ---------- throat-conductance surrogate (k_surrogate.hip) ----------
+int64_t surrogate_param_count();
+void surrogate_init(uint64_t seed, float* host_params);
+void surrogate_forward(Ctx* c, int B, const float* patches, const float* scalars,
+                       const float* params, float* out);
+void surrogate_backward(Ctx* c, int B, const float* dy, const float* params, float* grads);
+void surrogate_loss(Ctx* c, int B, const float* y, const float* t, float* dy, float* loss);
+void surrogate_adam(Ctx* c, float* params, const float* grads, float* m, float* v, int step,
+                    float lr, float b1, float b2, float eps);
+void surrogate_patches(Ctx* c, const float* sdf, Vol v, const float* centers, int B, float scale,
+                       float* out);
+void surrogate_release(Ctx* c);
 
 }  // namespace pnm

@@ -27,7 +27,8 @@ EXPORTS = (
     "pnm_flow_result_free", "pnm_extract_pores_d", "pnm_segment_volume_d",
     "pnm_extract_topology_d", "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
     "pnm_stage_labels_d", "pnm_stage_roots_d", "pnm_segment_batch_d", "pnm_batch_config",
-    "pnm_stokes_solve_d",
+    "pnm_stokes_solve_d", "pnm_sur_param_count", "pnm_sur_init", "pnm_sur_forward_d",
+    "pnm_sur_backward_d", "pnm_sur_loss_d", "pnm_sur_adam_d", "pnm_sur_patches_d",
 )
 
 PNM_BATCH_GRAPH = 1
@@ -50,7 +51,8 @@ class FlowResult(ctypes.Structure):
                 ("throats", ctypes.POINTER(ctypes.c_int32)),
                 ("throat_flow", ctypes.POINTER(ctypes.c_double)),
                 ("throat_area", ctypes.POINTER(ctypes.c_double)),
-                ("throat_dp", ctypes.POINTER(ctypes.c_double))]
+                ("throat_dp", ctypes.POINTER(ctypes.c_double)),
+                ("throat_center", ctypes.POINTER(ctypes.c_double))]
 
 
 _lib = None
@@ -110,6 +112,21 @@ def lib():
         L.pnm_stokes_solve_d.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.pnm_stokes_solve_d.restype = st
+        L.pnm_sur_param_count.restype = ctypes.c_int64
+        L.pnm_sur_param_count.argtypes = []
+        L.pnm_sur_init.argtypes = [ctypes.c_uint64, vp, ctypes.c_int64]
+        L.pnm_sur_init.restype = st
+        L.pnm_sur_forward_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp]
+        L.pnm_sur_forward_d.restype = st
+        L.pnm_sur_backward_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
+        L.pnm_sur_backward_d.restype = st
+        L.pnm_sur_loss_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp]
+        L.pnm_sur_loss_d.restype = st
+        L.pnm_sur_adam_d.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.pnm_sur_adam_d.restype = st
+        L.pnm_sur_patches_d.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp]
+        L.pnm_sur_patches_d.restype = st
         L.pnm_stage_labels_d.argtypes = [vp, vp, vp, vp, vp, vp]
         L.pnm_stage_roots_d.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         for name in ("pnm_extract_pores_d", "pnm_segment_volume_d", "pnm_extract_topology_d",
@@ -268,6 +285,41 @@ class Context:
         return it.value, rr.value
 
 
+    # ---- throat surrogate ----
+    def sur_forward_d(self, batch, patches, scalars, params, out):
+        _check(lib().pnm_sur_forward_d(self.h, int(batch), _ptr(patches), _ptr(scalars),
+                                       _ptr(params), _ptr(out)))
+
+    def sur_backward_d(self, batch, dout, params, grads):
+        _check(lib().pnm_sur_backward_d(self.h, int(batch), _ptr(dout), _ptr(params),
+                                        _ptr(grads)))
+
+    def sur_loss_d(self, batch, pred, target, dpred, loss=None):
+        _check(lib().pnm_sur_loss_d(self.h, int(batch), _ptr(pred), _ptr(target), _ptr(dpred),
+                                    _ptr(loss)))
+
+    def sur_adam_d(self, params, grads, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999,
+                   eps=1e-8):
+        _check(lib().pnm_sur_adam_d(self.h, _ptr(params), _ptr(grads), _ptr(m), _ptr(v),
+                                    int(step), lr, beta1, beta2, eps))
+
+    def sur_patches_d(self, sdf, dims_xyz, centers, batch, scale, out):
+        _check(lib().pnm_sur_patches_d(self.h, _ptr(sdf), _i3(dims_xyz), _ptr(centers),
+                                       int(batch), float(scale), _ptr(out)))
+
+
+def sur_param_count():
+    return int(lib().pnm_sur_param_count())
+
+
+def sur_init(seed):
+    """Initial surrogate parameters (host float32 array)."""
+    n = sur_param_count()
+    out = np.zeros(n, dtype=np.float32)
+    _check(lib().pnm_sur_init(ctypes.c_uint64(seed), out.ctypes.data_as(ctypes.c_void_p), n))
+    return out
+
+
 def _flow_to_numpy(r):
     npores, nt = r.np, r.nt
 
@@ -286,6 +338,7 @@ def _flow_to_numpy(r):
         "throat_flow": arr(r.throat_flow, nt, np.float64),
         "throat_area": arr(r.throat_area, nt, np.float64),
         "throat_dp": arr(r.throat_dp, nt, np.float64),
+        "throat_center": arr(r.throat_center, 3 * nt, np.float64).reshape(-1, 3),
     }
     lib().pnm_flow_result_free(ctypes.byref(r))
     return out

@@ -1,0 +1,194 @@
+"""Throat-conductance surrogate: learns log g, g = Q/dp, from extract_network_flow's own output.
+
+The network and its training run on the GPU through the C-ABI (pnm_sur_*: BF16-MFMA convolutions,
+FP32 dense head, Adam). The plain bookkeeping (extract_network_flow on solved fields) produces
+the targets and remains the fallback and the judge of the surrogate's error.
+
+Per throat the inputs are
+  - a 16^3 SDF patch centred on the interface centroid (throat_center), scaled by 1/8;
+  - 4 scalars: log throat area, log radius of each pore (SDF at its peak), log min-image pore
+    separation, standardised with the training set's mean and spread.
+The "throat geometry" (throats, areas, centroids, pore positions and radii) comes from the same
+bookkeeping with zero fields, so inference needs only the SDF.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi
+
+PATCH_SCALE = 8.0
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def throat_inputs(net, shape_zyx, spacing_zyx=(1.0, 1.0, 1.0), origin_zyx=(0.0, 0.0, 0.0)):
+    """Host arrays (centers_vox [nt,3] xyz, raw scalars [nt,4]) from a flow-result dict."""
+    th = np.asarray(net["throats"], dtype=np.int64).reshape(-1, 2)
+    nt = th.shape[0]
+    pores = net["pores"]
+    pos = np.stack([pores["x"], pores["y"], pores["z"]], axis=1).astype(np.float64)
+    rad = np.asarray(pores["radius"], dtype=np.float64)
+    org = np.asarray(origin_zyx[::-1], dtype=np.float64)
+    spc = np.asarray(spacing_zyx[::-1], dtype=np.float64)
+    L = np.asarray(shape_zyx[::-1], dtype=np.float64) * spc
+    centers = (np.asarray(net["throat_center"], dtype=np.float64).reshape(-1, 3) - org) / spc
+    if nt == 0:
+        return centers.astype(np.float32), np.zeros((0, 4), np.float32)
+    d = pos[th[:, 1] - 1] - pos[th[:, 0] - 1]
+    d -= L * np.round(d / L)
+    sep = np.sqrt((d * d).sum(axis=1))
+    A = np.asarray(net["throat_area"], dtype=np.float64)
+    raw = np.stack([np.log(np.maximum(A, 1e-12)),
+                    np.log(np.maximum(rad[th[:, 0] - 1], 1e-6)),
+                    np.log(np.maximum(rad[th[:, 1] - 1], 1e-6)),
+                    np.log(np.maximum(sep, 1e-6))], axis=1)
+    return centers.astype(np.float32), raw.astype(np.float32)
+
+
+def conductance_targets(net):
+    """(log g, valid mask) with g = Q/dp; throats with Q*dp <= 0 are flagged invalid."""
+    q = np.asarray(net["throat_flow"], dtype=np.float64)
+    dp = np.asarray(net["throat_dp"], dtype=np.float64)
+    ok = (q * dp > 0) & np.isfinite(q) & np.isfinite(dp)
+    g = np.where(ok, q / np.where(dp == 0, 1.0, dp), 1.0)
+    return np.log(np.abs(g)).astype(np.float32), ok
+
+
+def geometric_baseline_fit(raw, logg):
+    """The method without the network: g = c * A^2 / L (log g = 2 log A - log L + log c)."""
+    base = 2.0 * raw[:, 0] - raw[:, 3]
+    return float(np.median(logg - base))
+
+
+def geometric_baseline_predict(raw, c):
+    return 2.0 * raw[:, 0] - raw[:, 3] + c
+
+
+class ThroatSurrogate:
+    """Parameters, Adam state and feature standardisation of one surrogate network."""
+
+    def __init__(self, ctx, seed=0):
+        torch = _torch()
+        self.ctx = ctx
+        self.params = torch.from_numpy(capi.sur_init(seed)).cuda()
+        n = self.params.numel()
+        self.m = torch.zeros(n, device="cuda")
+        self.v = torch.zeros(n, device="cuda")
+        self.grads = torch.zeros(n, device="cuda")
+        self.step = 0
+        self.mu = np.zeros(4, np.float32)
+        self.sd = np.ones(4, np.float32)
+        self.t_mean = 0.0
+
+    # ---- features ----
+    def standardize(self, raw):
+        return (raw - self.mu) / self.sd
+
+    def fit_standardization(self, raw, logg):
+        self.mu = raw.mean(axis=0).astype(np.float32)
+        self.sd = np.maximum(raw.std(axis=0), 1e-6).astype(np.float32)
+        self.t_mean = float(np.mean(logg))
+
+    def patches(self, sdf_dev, shape_zyx, centers):
+        """Device float32 [nt, 4096] patches for device SDF `sdf_dev` and host centres."""
+        torch = _torch()
+        nt = centers.shape[0]
+        out = torch.empty(max(nt, 1) * 4096, device="cuda")
+        if nt:
+            c = torch.from_numpy(np.ascontiguousarray(centers, dtype=np.float32)).cuda()
+            self.ctx.sur_patches_d(sdf_dev, shape_zyx[::-1], c, nt, PATCH_SCALE, out)
+        return out[: nt * 4096].view(nt, 4096)
+
+    # ---- network ----
+    def predict(self, patches, scalars_std, batch=4096):
+        """log g for device patches [N,4096] and host standardised scalars [N,4]."""
+        torch = _torch()
+        N = patches.shape[0]
+        out = torch.empty(max(N, 1), device="cuda")
+        s = torch.from_numpy(np.ascontiguousarray(scalars_std, dtype=np.float32)).cuda()
+        for a in range(0, N, batch):
+            b = min(N, a + batch)
+            self.ctx.sur_forward_d(b - a, patches[a:b].contiguous(), s[a:b].contiguous(),
+                                   self.params, out[a:b])
+        return out[:N] + self.t_mean
+
+    def train_step(self, patches, scalars_std, targets, lr):
+        """One Adam step on a device minibatch; returns the loss tensor (device)."""
+        torch = _torch()
+        B = patches.shape[0]
+        y = torch.empty(B, device="cuda")
+        dy = torch.empty(B, device="cuda")
+        loss = torch.empty(1, device="cuda")
+        self.ctx.sur_forward_d(B, patches, scalars_std, self.params, y)
+        self.ctx.sur_loss_d(B, y, targets - self.t_mean, dy, loss)
+        self.ctx.sur_backward_d(B, dy, self.params, self.grads)
+        self.step += 1
+        self.ctx.sur_adam_d(self.params, self.grads, self.m, self.v, self.step, lr=lr)
+        return loss
+
+    def fit(self, patches, scalars_raw, logg, steps=2000, batch=256, lr=2e-3, seed=0,
+            log_every=0):
+        """Train on device patches [N,4096], host raw scalars [N,4] and host log g [N]."""
+        torch = _torch()
+        self.fit_standardization(scalars_raw, logg)
+        s = torch.from_numpy(self.standardize(scalars_raw).astype(np.float32)).cuda()
+        t = torch.from_numpy(np.asarray(logg, dtype=np.float32)).cuda()
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        N = patches.shape[0]
+        hist = []
+        for k in range(steps):
+            idx = torch.randint(0, N, (batch,), device="cuda", generator=gen)
+            lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / steps))  # cosine decay
+            loss = self.train_step(patches.index_select(0, idx).contiguous(),
+                                   s.index_select(0, idx).contiguous(),
+                                   t.index_select(0, idx).contiguous(), lr_k)
+            if log_every and (k % log_every == 0 or k == steps - 1):
+                hist.append((k, float(loss.item())))
+        return hist
+
+    def state(self):
+        return {"params": self.params.cpu().numpy(), "mu": self.mu, "sd": self.sd,
+                "t_mean": np.float64(self.t_mean)}
+
+    def load(self, st):
+        torch = _torch()
+        self.params = torch.from_numpy(np.asarray(st["params"], np.float32)).cuda()
+        self.mu = np.asarray(st["mu"], np.float32)
+        self.sd = np.asarray(st["sd"], np.float32)
+        self.t_mean = float(st["t_mean"])
+
+
+def median_rel_error(logg_pred, logg_true):
+    """Median over throats of |g_pred - g| / |g|."""
+    return float(np.median(np.abs(np.expm1(np.asarray(logg_pred, np.float64)
+                                           - np.asarray(logg_true, np.float64)))))
+
+
+def flow_sample(ctx, sdf, mu=1.0, fx=1.0, rtol=1e-9, max_iter=40000, geometry_only=False):
+    """Stokes solve (unless geometry_only) + extract_network_flow on a host SDF (zyx).
+    Returns (device sdf, result dict, seconds of the Stokes solve)."""
+    import time
+
+    torch = _torch()
+    n = sdf.size
+    dims = sdf.shape[::-1]
+    s = torch.from_numpy(np.ascontiguousarray(sdf, dtype=np.float32)).cuda()
+    f = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(7)]
+    # geometry only: zero fields with the solver's binary openness (max_iter = 0), so the
+    # throats are exactly those of the solved case
+    torch.cuda.synchronize()
+    t0 = time.time()
+    ctx.stokes_solve_d(s, dims, (1, 1, 1), (fx, 0.0, 0.0), *f, mu=mu, rtol=rtol,
+                       max_iter=0 if geometry_only else max_iter)
+    torch.cuda.synchronize()
+    t_st = time.time() - t0
+    ox, oy, oz = f[4], f[5], f[6]
+    net = ctx.extract_network_flow_d(s, dims, (0, 0, 0), (1, 1, 1), *f[:4], ox=ox, oy=oy, oz=oz,
+                                     grad_p_xyz=(-fx, 0.0, 0.0))
+    return s, net, t_st

@@ -1,0 +1,103 @@
+"""Throat surrogate kernels on the GPU against the FP64 oracle (oracle/surrogate_oracle.py):
+BF16-MFMA convolutions forward, backward (weight and input gradients), dense head, loss, Adam and
+the patch gather."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import surrogate_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def group_errors(g_gpu, g_ref):
+    off, _ = so.layout()
+    out = {}
+    for name, (o, shape) in off.items():
+        n = int(np.prod(shape))
+        a, b = g_gpu[o:o + n], g_ref[o:o + n]
+        out[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    return out
+
+
+@pytest.mark.parametrize("B", [3, 16])
+def test_forward_backward_vs_oracle(capi_ctx, B):
+    from peclet.pnm import capi
+
+    rng = np.random.default_rng(100 + B)
+    p = capi.sur_init(11 + B)
+    P = so.unpack(p)
+    P["b0"] += rng.normal(0, 0.05, 16)
+    P["b1"] += rng.normal(0, 0.05, 32)
+    P["b4"] += rng.normal(0, 0.05, 64)
+    p = so.pack(P).astype(np.float32)
+    x = rng.normal(0, 1, size=(B, 16, 16, 16)).astype(np.float32)
+    s = rng.normal(0, 1, size=(B, 4)).astype(np.float32)
+    dy = rng.normal(0, 1, size=B).astype(np.float32)
+    pd, xd, sd, dyd = dev(p), dev(x), dev(s), dev(dy)
+    y = torch.empty(B, device="cuda")
+    g = torch.zeros(p.size, device="cuda")
+    capi_ctx.sur_forward_d(B, xd, sd, pd, y)
+    capi_ctx.sur_backward_d(B, dyd, pd, g)
+    y_ref, cache = so.forward(p.astype(np.float64), x, s, rounding=True)
+    g_ref = so.backward(cache, dy)
+    yg = y.cpu().numpy()
+    assert np.abs(yg - y_ref).max() <= 2e-3 * max(1.0, np.abs(y_ref).max()), (yg, y_ref)
+    errs = group_errors(g.cpu().numpy().astype(np.float64), g_ref)
+    assert max(errs.values()) < 2e-2, errs
+    # padded K columns of the conv weights get no gradient
+    gg = g.cpu().numpy()
+    off, _ = so.layout()
+    for l, (_, ci, co, kp) in enumerate(so.CONV):
+        o, _ = off[f"w{l}"]
+        assert np.all(gg[o:o + co * kp].reshape(co, kp)[:, 27 * ci:] == 0.0)
+
+
+def test_loss_and_adam(capi_ctx):
+    from peclet.pnm import capi
+
+    rng = np.random.default_rng(3)
+    B = 37
+    yv = rng.normal(size=B).astype(np.float32)
+    tv = rng.normal(size=B).astype(np.float32)
+    dy = torch.empty(B, device="cuda")
+    loss = torch.empty(1, device="cuda")
+    capi_ctx.sur_loss_d(B, dev(yv), dev(tv), dy, loss)
+    l_ref, dy_ref = so.mse(yv, tv)
+    assert abs(loss.item() - l_ref) < 1e-5 * max(1.0, l_ref)
+    np.testing.assert_allclose(dy.cpu().numpy(), dy_ref, rtol=1e-5, atol=1e-7)
+    n = capi.sur_param_count()
+    p0 = rng.normal(size=n).astype(np.float32)
+    gv = rng.normal(size=n).astype(np.float32)
+    p, m, v = dev(p0), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda")
+    for step in (1, 2):
+        capi_ctx.sur_adam_d(p, dev(gv), m, v, step, lr=1e-2)
+    pe, me, ve = p0.astype(np.float64), np.zeros(n), np.zeros(n)
+    for step in (1, 2):
+        me = 0.9 * me + 0.1 * gv
+        ve = 0.999 * ve + 0.001 * gv.astype(np.float64) ** 2
+        pe = pe - 1e-2 * (me / (1 - 0.9 ** step)) / (np.sqrt(ve / (1 - 0.999 ** step)) + 1e-8)
+    np.testing.assert_allclose(p.cpu().numpy(), pe, rtol=1e-5, atol=1e-6)
+
+
+def test_patches_gather(capi_ctx):
+    rng = np.random.default_rng(9)
+    shape = (20, 24, 28)  # z, y, x
+    sdf = rng.normal(size=shape).astype(np.float32)
+    centers = np.array([[0.2, 0.0, 0.0], [27.9, 23.5, 19.99], [13.0, 5.5, 2.0]], np.float32)
+    out = torch.empty(3 * 4096, device="cuda")
+    capi_ctx.sur_patches_d(dev(sdf), (28, 24, 20), dev(centers), 3, 4.0, out)
+    got = out.cpu().numpy().reshape(3, 16, 16, 16)
+    for b, (cx, cy, cz) in enumerate(centers):
+        ix = (np.floor(cx).astype(int) - 7 + np.arange(16)) % 28
+        iy = (np.floor(cy).astype(int) - 7 + np.arange(16)) % 24
+        iz = (np.floor(cz).astype(int) - 7 + np.arange(16)) % 20
+        want = sdf[np.ix_(iz, iy, ix)] / 4.0
+        np.testing.assert_array_equal(got[b], want)
