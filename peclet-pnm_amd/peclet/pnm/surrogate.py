@@ -69,6 +69,17 @@ def geometric_baseline_predict(raw, c):
     return 2.0 * raw[:, 0] - raw[:, 3] + c
 
 
+def gather_patches(ctx, sdf_dev, shape_zyx, centers):
+    """Device float32 [nt, 4096] patches of the device SDF around host centres [nt, 3] (xyz)."""
+    torch = _torch()
+    nt = centers.shape[0]
+    out = torch.empty(max(nt, 1) * 4096, device="cuda")
+    if nt:
+        c = torch.from_numpy(np.ascontiguousarray(centers, dtype=np.float32)).cuda()
+        ctx.sur_patches_d(sdf_dev, tuple(shape_zyx)[::-1], c, nt, PATCH_SCALE, out)
+    return out[: nt * 4096].view(nt, 4096)
+
+
 class ThroatSurrogate:
     """Parameters, Adam state and feature standardisation of one surrogate network."""
 
@@ -95,14 +106,7 @@ class ThroatSurrogate:
         self.t_mean = float(np.mean(logg))
 
     def patches(self, sdf_dev, shape_zyx, centers):
-        """Device float32 [nt, 4096] patches for device SDF `sdf_dev` and host centres."""
-        torch = _torch()
-        nt = centers.shape[0]
-        out = torch.empty(max(nt, 1) * 4096, device="cuda")
-        if nt:
-            c = torch.from_numpy(np.ascontiguousarray(centers, dtype=np.float32)).cuda()
-            self.ctx.sur_patches_d(sdf_dev, shape_zyx[::-1], c, nt, PATCH_SCALE, out)
-        return out[: nt * 4096].view(nt, 4096)
+        return gather_patches(self.ctx, sdf_dev, shape_zyx, centers)
 
     # ---- network ----
     def predict(self, patches, scalars_std, batch=4096):
@@ -192,3 +196,44 @@ def flow_sample(ctx, sdf, mu=1.0, fx=1.0, rtol=1e-9, max_iter=40000, geometry_on
     net = ctx.extract_network_flow_d(s, dims, (0, 0, 0), (1, 1, 1), *f[:4], ox=ox, oy=oy, oz=oz,
                                      grad_p_xyz=(-fx, 0.0, 0.0))
     return s, net, t_st
+
+
+def packing_params(seed):
+    """Grain-pack parameters of training/test packing `seed` (varied so the model generalises)."""
+    rng = np.random.default_rng(seed)
+    return {"median": float(rng.uniform(4.0, 8.0)), "sigma": float(rng.uniform(0.2, 0.35)),
+            "porosity": float(rng.uniform(0.33, 0.45))}
+
+
+def build_dataset(ctx, seeds, size, rtol=1e-9, log=None):
+    """Solve and extract every packing; returns device patches [N,4096] and host arrays
+    (raw scalars [N,4], log g [N], packing index [N]) of the throats with Q*dp > 0, plus a
+    per-packing summary."""
+    import time
+
+    torch = _torch()
+    from . import synth
+
+    P, R, T, K, info = [], [], [], [], []
+    for k, seed in enumerate(seeds):
+        prm = packing_params(seed)
+        t0 = time.time()
+        sdf, phi, _ = synth.polydisperse_sdf((size, size, size), prm["median"], prm["sigma"],
+                                             prm["porosity"], seed=int(seed))
+        s, net, t_st = flow_sample(ctx, sdf, rtol=rtol)
+        centers, raw = throat_inputs(net, sdf.shape)
+        logg, ok = conductance_targets(net)
+        P.append(gather_patches(ctx, s, sdf.shape, centers[ok]))
+        R.append(raw[ok])
+        T.append(logg[ok])
+        K.append(np.full(int(ok.sum()), k, np.int32))
+        info.append({"seed": int(seed), **prm, "porosity_achieved": round(phi, 4),
+                     "throats": int(len(ok)), "kept": int(ok.sum()),
+                     "stokes_s": round(t_st, 3), "total_s": round(time.time() - t0, 3)})
+        if log:
+            log(info[-1])
+        del s
+    return (torch.cat(P) if P else torch.empty(0, 4096, device="cuda"),
+            np.concatenate(R) if R else np.zeros((0, 4), np.float32),
+            np.concatenate(T) if T else np.zeros(0, np.float32),
+            np.concatenate(K) if K else np.zeros(0, np.int32), info)
