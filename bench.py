@@ -17,7 +17,10 @@ volume. Two more objects on the same line:
            wrap), segmented in one persistent launch replayed from a hipGraph with the autotuned
            shape, against per-volume segment_volume calls on a sample of the same crops;
   flow:    extract_network_flow on the 256^3 chamber lattice (5^3 chambers r=12 at pitch 51.2,
-           tubes r ~ U[3,8], seed 3001) with FP64 fields from the project Stokes solver.
+           tubes r ~ U[3,8], seed 3001) with FP64 fields from the project Stokes solver;
+  surrogate: throat conductance g = Q/dp on a held-out 256^3 grain pack: the plain path (Stokes
+           solve + extract_network_flow) against the surrogate path (throat geometry from the
+           SDF alone + 16^3 patches + CNN inference with the packaged trained weights).
 """
 from __future__ import annotations
 
@@ -36,8 +39,9 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic (compulsory, one streaming pass) bytes per voxel of each per-voxel kernel.
 ALGO_BYTES_PER_VOXEL = {
     "k_tile": 12,         # read sdf 4; write marker label 4, gradient-path root 4
+    "k_flood_tile": 12,   # read labels 4, sdf 4; write labels 4
+    "k_seg_topo": 16,     # read sdf 4, root 4, label 4; write seg 4
     "k_pre_peaks": 4,     # read sdf 4
-    "k_ccl_seams": 4,     # read marker labels 4 (neighbour rows are cache hits; unions sparse)
     "k_ccl_flatten": 8,   # read + write labels
     "k_seg": 16,          # read sdf 4, roots 4, labels 4; write seg 4
     "k_topo_cand": 4,     # read seg 4
@@ -174,6 +178,83 @@ def bench_flow(ctx, reps, size=256):
                        "body_force_x": fx}}
 
 
+def bench_surrogate(ctx, size=256, seed=9999):
+    """Plain (Stokes + bookkeeping) vs surrogate (geometry + CNN) throat conductance."""
+    import numpy as np
+    import torch
+
+    from peclet.pnm import surrogate as S
+    from peclet.pnm import synth
+
+    wpath = os.path.join(ROOT, "peclet-pnm_amd", "peclet", "pnm", "data",
+                         "surrogate_weights.npz")
+    if not os.path.exists(wpath):
+        return None
+    prm = S.packing_params(seed)
+    sdf, phi, _ = synth.polydisperse_sdf((size,) * 3, prm["median"], prm["sigma"],
+                                         prm["porosity"], seed=seed)
+    model = S.ThroatSurrogate(ctx)
+    model.load(np.load(wpath))
+    # plain path: Stokes solve + bookkeeping
+    torch.cuda.synchronize()
+    t0 = time.time()
+    s_dev, net, t_stokes = S.flow_sample(ctx, sdf)
+    torch.cuda.synchronize()
+    t_plain = time.time() - t0
+    logg, ok = S.conductance_targets(net)
+    # bookkeeping alone on the solved fields (second baseline)
+    # surrogate path: geometry from the SDF only, patches, inference (one warm-up, then timed)
+    def surrogate_path():
+        s2, geo, _ = S.flow_sample(ctx, sdf, geometry_only=True)
+        centers, raw = S.throat_inputs(geo, sdf.shape)
+        patches = S.gather_patches(ctx, s2, sdf.shape, centers)
+        pred = model.predict(patches, model.standardize(raw))
+        torch.cuda.synchronize()
+        return geo, pred
+    surrogate_path()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    geo, pred = surrogate_path()
+    t_sur = time.time() - t0
+    same = np.array_equal(np.asarray(geo["throats"]), np.asarray(net["throats"]))
+    pred = pred.cpu().numpy()
+    err = S.median_rel_error(pred[ok], logg[ok]) if same and ok.any() else None
+    c = float(np.load(wpath)["geometric_c"])
+    _, raw = S.throat_inputs(net, sdf.shape)
+    err_geo = S.median_rel_error(S.geometric_baseline_predict(raw[ok], c), logg[ok])
+    # inference alone, timed with events on the current stream
+    nt = len(pred)
+    _, raw = S.throat_inputs(geo, sdf.shape)
+    centers, _ = S.throat_inputs(geo, sdf.shape)
+    patches = S.gather_patches(ctx, s_dev, sdf.shape, centers)
+    sc = model.standardize(raw)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    model.predict(patches, sc)
+    e0.record(stream)
+    for _ in range(5):
+        model.predict(patches, sc)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_inf = e0.elapsed_time(e1) / 5 / 1e3
+    conv_flops = sum(2 * co * 27 * ci * d ** 3 for d, ci, co in ((16, 1, 16), (8, 16, 32),
+                                                                   (4, 32, 32)))
+    tflops = conv_flops * nt / t_inf / 1e12
+    return {"workload": f"held-out {size}^3 polydisperse grain pack (seed {seed}, median radius "
+                        f"{prm['median']:.2f}, porosity {phi:.3f})",
+            "throats": int(nt), "throats_with_q_dp_positive": int(ok.sum()),
+            "plain_s": round(t_plain, 3), "stokes_s": round(t_stokes, 3),
+            "surrogate_s": round(t_sur, 4),
+            "speedup_vs_plain": round(t_plain / t_sur, 1),
+            "speedup_vs_bookkeeping_only": round((t_plain - t_stokes) / t_sur, 2),
+            "median_rel_error_g": err, "median_rel_error_g_geometric_A2_over_L": err_geo,
+            "same_throats": bool(same),
+            "inference": {"seconds": round(t_inf, 5), "throats_per_s": round(nt / t_inf),
+                          "roofline": {"bound": "mfma", "achieved": round(tflops, 3),
+                                       "peak": 2500.0, "unit": "TFLOP/s",
+                                       "frac": round(tflops / 2500.0, 6)}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +267,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU only (the 512^3 volume is one GPU's block); "
@@ -278,6 +360,9 @@ def main():
     if not args.no_flow:
         torch.cuda.empty_cache()
         out["flow"] = bench_flow(ctx, reps=3)
+    if not args.no_surrogate:
+        torch.cuda.empty_cache()
+        out["surrogate"] = bench_surrogate(ctx)
     print(json.dumps(out), flush=True)
 
 

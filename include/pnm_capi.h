@@ -232,9 +232,10 @@ pnm_status pnm_sur_forward_d(pnm_ctx_t ctx, int32_t batch, const float* patches_
  * pass (same batch); grads_d: float[pnm_sur_param_count()] (overwritten). */
 pnm_status pnm_sur_backward_d(pnm_ctx_t ctx, int32_t batch, const float* dout_d,
                               const float* params_d, float* grads_d);
-/* Mean squared error over the batch; dpred = 2 (pred - target) / batch; loss_d nullable. */
+/* Mean loss over the batch: Huber with threshold delta (delta > 0) or squared error
+ * (delta <= 0) of pred - target; dpred = d(loss)/d(pred); loss_d nullable. */
 pnm_status pnm_sur_loss_d(pnm_ctx_t ctx, int32_t batch, const float* pred_d,
-                          const float* target_d, float* dpred_d, float* loss_d);
+                          const float* target_d, float delta, float* dpred_d, float* loss_d);
 /* One Adam step (bias-corrected, step >= 1) on the flat parameter vector. */
 pnm_status pnm_sur_adam_d(pnm_ctx_t ctx, float* params_d, const float* grads_d, float* m_d,
                           float* v_d, int32_t step, float lr, float beta1, float beta2,

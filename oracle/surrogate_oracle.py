@@ -148,3 +148,12 @@ def backward(cache, dy):
 def mse(y, t):
     e = np.asarray(y, dtype=np.float64) - np.asarray(t, dtype=np.float64)
     return float((e * e).mean()), 2.0 * e / e.size
+
+
+def huber(y, t, delta):
+    """Mean Huber loss and its gradient (threshold delta)."""
+    e = np.asarray(y, dtype=np.float64) - np.asarray(t, dtype=np.float64)
+    a = np.abs(e)
+    l = np.where(a <= delta, 0.5 * e * e, delta * (a - 0.5 * delta))
+    g = np.where(a <= delta, e, delta * np.sign(e))
+    return float(l.mean()), g / e.size

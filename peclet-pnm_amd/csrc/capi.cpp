@@ -546,12 +546,12 @@ pnm_status pnm_sur_backward_d(pnm_ctx_t ctx, int32_t batch, const float* dout_d,
 }
 
 pnm_status pnm_sur_loss_d(pnm_ctx_t ctx, int32_t batch, const float* pred_d,
-                          const float* target_d, float* dpred_d, float* loss_d) {
+                          const float* target_d, float delta, float* dpred_d, float* loss_d) {
   return guarded([&] {
     need(batch > 0 && pred_d && target_d && dpred_d, "batch > 0 and buffers are required");
     Ctx* c = check_ctx(ctx);
     CallScope cs(c);
-    surrogate_loss(c, batch, pred_d, target_d, dpred_d, loss_d);
+    surrogate_loss(c, batch, pred_d, target_d, delta, dpred_d, loss_d);
   });
 }
 

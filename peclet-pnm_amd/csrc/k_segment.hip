@@ -60,6 +60,181 @@ void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* ke
   launch_ccl_flatten(c, v, labels, min_solid, key_stage, key_cnt);
 }
 
+// ---------------- flatten + first flood sweeps, tile by tile ----------------
+// flatten (pore_extraction.hpp:183-194) and the first FLOOD_SWEEPS Jacobi sweeps of flood
+// (:237-269) in one pass. A 512-thread block loads its 64x8x16 tile plus a halo of
+// FLOOD_SWEEPS voxels (labels resolved to their component root, band candidates = unlabelled
+// !(sdf >= 0)) into LDS and sweeps there: sweep k is exact on the region shrunk by k voxels, so
+// after FLOOD_SWEEPS sweeps the tile holds exactly the global state after those sweeps. It writes
+// the tile's labels to a second array (neighbouring blocks keep reading the unflooded labels of
+// this tile as their halo), lowers the solid first-appearance minimum for voxels it labelled
+// (relabel_min :335-336; every marker of a label comes after its root, which k_tile seeded),
+// and stages the component roots (solid keys) and the voxels still unlabelled (the band of the
+// next global sweep).
+constexpr int FLOOD_SWEEPS = 2;
+constexpr int FR_X = 64 + 2 * FLOOD_SWEEPS, FR_Y = TILE_Y + 2 * FLOOD_SWEEPS,
+              FR_Z = TILE_Z + 2 * FLOOD_SWEEPS;
+constexpr int FR_N = FR_X * FR_Y * FR_Z;
+constexpr int FR_SEG = 64 * TILE_Z;  // staging slots per wave
+// region word: >= 0 label, -1 band candidate, -2 other unlabelled, <= -3 labelled this sweep
+
+__global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restrict__ sdf,
+                                                            const int* __restrict__ L,
+                                                            int* __restrict__ Lout, int nx, int ny,
+                                                            int nz, int32_t* __restrict__ min_solid,
+                                                            uint32_t* __restrict__ key_stage,
+                                                            uint32_t* __restrict__ key_cnt,
+                                                            uint32_t* __restrict__ band_stage,
+                                                            uint32_t* __restrict__ band_cnt) {
+  __shared__ int R[FR_N];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  const int tid = wave * 64 + lane;
+  constexpr int NT = 64 * TILE_Y;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TILE_Y, z0 = blockIdx.z * TILE_Z;
+  const int nzt = min(TILE_Z, nz - z0);
+  const int64_t seg_id =
+      (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * TILE_Y + wave;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  constexpr int H = FLOOD_SWEEPS;
+  // load: labels (a marker's component is L[L[i]]: its tile-local root points at the component
+  // root) and band candidates. Rows of the region are spread over the waves; a lane holds
+  // columns lane and (lanes < 2H) 64 + lane, whose wrapped x is fixed.
+  const int rows = FR_Y * (nzt + 2 * H);
+  const int gx0 = wrap_any(x0 - H + lane, nx);
+  const bool two = lane < 2 * H;
+  const int gx1 = two ? wrap_any(x0 - H + 64 + lane, nx) : gx0;
+  // pass 1: raw labels (tile-local root or -1) and band candidates; pass 2: local root ->
+  // component root. Loads are issued in branch-free batches of RB rows into registers (rows
+  // past the region re-read a valid row), so RB rows of loads are in flight per thread.
+  constexpr int RB = 15;
+  constexpr int MAXR = (FR_Y * FR_Z + TILE_Y - 1) / TILE_Y;  // rows per wave, full tile
+  const bool near = ny > TILE_Y && nz > 2;  // halo coordinates lie in [-N, 2N)
+  for (int j0 = 0; j0 < MAXR; j0 += RB) {
+    int la[RB], lb[RB];
+    float va[RB], vb[RB];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int row0 = wave + (j0 + j) * TILE_Y;
+      const int row = row0 < rows ? row0 : wave;
+      const int ry = row % FR_Y, rz = row / FR_Y;
+      const int gy = near ? wrap_near(y0 - H + ry, ny) : wrap_any(y0 - H + ry, ny);
+      const int gz = near ? wrap_near(z0 - H + rz, nz) : wrap_any(z0 - H + rz, nz);
+      const int64_t gb = ((int64_t)gz * ny + gy) * nx;
+      la[j] = L[gb + gx0];
+      va[j] = sdf[gb + gx0];
+      lb[j] = L[gb + gx1];
+      vb[j] = sdf[gb + gx1];
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int row = wave + (j0 + j) * TILE_Y;
+      if (row < rows) {
+        R[row * FR_X + lane] = la[j] >= 0 ? la[j] : (!(va[j] >= 0.0f) ? -1 : -2);
+        if (two) R[row * FR_X + 64 + lane] = lb[j] >= 0 ? lb[j] : (!(vb[j] >= 0.0f) ? -1 : -2);
+      }
+    }
+  }
+  for (int j0 = 0; j0 < MAXR; j0 += RB) {
+    int ka[RB], kb[RB], ra[RB], rb[RB];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int row0 = wave + (j0 + j) * TILE_Y;
+      const int row = row0 < rows ? row0 : wave;
+      ka[j] = R[row * FR_X + lane];
+      kb[j] = two ? R[row * FR_X + 64 + lane] : -2;
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      ra[j] = L[ka[j] >= 0 ? ka[j] : 0];
+      rb[j] = L[kb[j] >= 0 ? kb[j] : 0];
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int row = wave + (j0 + j) * TILE_Y;
+      if (row < rows) {
+        if (ka[j] >= 0) R[row * FR_X + lane] = ra[j];
+        if (two && kb[j] >= 0) R[row * FR_X + 64 + lane] = rb[j];
+      }
+    }
+  }
+  __syncthreads();
+  // Jacobi sweeps on the shrinking region
+  for (int k = 1; k <= H; ++k) {
+    for (int row = wave; row < rows; row += TILE_Y) {
+      const int ry = row % FR_Y, rz = row / FR_Y;
+      if (ry < k || ry >= FR_Y - k || rz < k || rz >= nzt + 2 * H - k) continue;  // wave-uniform
+      for (int rx = lane; rx < FR_X; rx += 64) {
+        if (rx < k || rx >= FR_X - k) continue;
+        const int e = row * FR_X + rx;
+        if (R[e] != -1) continue;
+        int best = -1;
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+          for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+              if (dx == 0 && dy == 0 && dz == 0) continue;
+              const int nl = R[e + (dz * FR_Y + dy) * FR_X + dx];
+              if (nl >= 0 && (best < 0 || nl < best)) best = nl;
+            }
+        if (best >= 0) R[e] = -3 - best;  // labelled this sweep: reads as unlabelled until the end
+      }
+    }
+    __syncthreads();
+    for (int row = wave; row < rows; row += TILE_Y)
+      for (int rx = lane; rx < FR_X; rx += 64) {
+        const int e = row * FR_X + rx;
+        const int v = R[e];
+        if (v <= -3) R[e] = -3 - v;
+      }
+    __syncthreads();
+  }
+  // tile outputs, staged per wave in (slice, lane) order
+  const int x = x0 + lane, y = y0 + wave;
+  const bool col_in = x < nx && y < ny;
+  unsigned nk = 0, nb = 0;
+  int before[TILE_Z];  // the CCL label of each of this column's voxels (all loads in flight)
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    const int zc = zl < nzt ? z0 + zl : z0;
+    before[zl] = L[((int64_t)zc * ny + (col_in ? y : 0)) * nx + (col_in ? x : 0)];
+  }
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    if (zl >= nzt) break;
+    const int e = ((zl + H) * FR_Y + wave + H) * FR_X + lane + H;
+    const int i = ((z0 + zl) * ny + y) * nx + x;
+    const int v = col_in ? R[e] : -2;
+    bool root = false;
+    if (col_in) {
+      Lout[i] = v >= 0 ? v : -1;
+      if (v >= 0 && before[zl] < 0 && i < v) atomicMin(min_solid + v, i);
+      root = v == i;
+    }
+    const bool band = v == -1;
+    const uint64_t km = __ballot(root);
+    if (root) key_stage[seg_id * FR_SEG + nk + __popcll(km & below)] = (uint32_t)i;
+    nk += (unsigned)__popcll(km);
+    const uint64_t bm = __ballot(band);
+    if (band) band_stage[seg_id * FR_SEG + nb + __popcll(bm & below)] = (uint32_t)i;
+    nb += (unsigned)__popcll(bm);
+  }
+  if (lane == 0) {
+    key_cnt[seg_id] = nk;
+    band_cnt[seg_id] = nb;
+  }
+}
+
+void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
+                       int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
+                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt) {
+  PNM_LAUNCH(c, "k_flood_tile", k_flood_tile,
+             dim3(cdiv(v.nx, 64), cdiv(v.ny, TILE_Y), cdiv(v.nz, TILE_Z)), dim3(64, TILE_Y), 0,
+             sdf, labels_ccl, labels, v.nx, v.ny, v.nz, min_solid, key_stage, key_cnt, band_stage,
+             band_cnt);
+}
+
 // ---------------- flood (pore_extraction.hpp:237-269) ----------------
 // One Jacobi sweep over the current band list (count read on the device): the new label is the
 // minimum labelled neighbour of the previous state; voxels still unlabelled are staged per block

@@ -215,15 +215,27 @@ __global__ void k_head_fwd(const float* __restrict__ h, const float* __restrict_
   (void)B;
 }
 
-// MSE loss over the batch and dy = 2 (y - t) / B; loss written by block 0 (fixed order).
-__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, int B,
-                      float* __restrict__ dy, float* __restrict__ loss) {
+// Loss over the batch, mean over samples, in log g: Huber with threshold delta (delta > 0) or
+// squared error (delta <= 0); dy = d(loss)/dy. Written by one block in a fixed order.
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, int B,
+                       float delta, float* __restrict__ dy, float* __restrict__ loss) {
   __shared__ float sh[256];
   float acc = 0.f;
   for (int b = threadIdx.x; b < B; b += blockDim.x) {
     const float e = y[b] - t[b];
-    dy[b] = 2.f * e / (float)B;
-    acc += e * e;
+    float l, g;
+    if (delta <= 0.f) {
+      l = e * e;
+      g = 2.f * e;
+    } else if (fabsf(e) <= delta) {
+      l = 0.5f * e * e;
+      g = e;
+    } else {
+      l = delta * (fabsf(e) - 0.5f * delta);
+      g = e > 0.f ? delta : -delta;
+    }
+    dy[b] = g / (float)B;
+    acc += l;
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -507,8 +519,9 @@ void surrogate_backward(Ctx* c, int B, const float* dy, const float* params, flo
   }
 }
 
-void surrogate_loss(Ctx* c, int B, const float* y, const float* t, float* dy, float* loss) {
-  PNM_LAUNCH(c, "k_mse", k_mse, dim3(1), dim3(256), 0, y, t, B, dy, loss);
+void surrogate_loss(Ctx* c, int B, const float* y, const float* t, float delta, float* dy,
+                    float* loss) {
+  PNM_LAUNCH(c, "k_loss", k_loss, dim3(1), dim3(256), 0, y, t, B, delta, dy, loss);
 }
 
 void surrogate_adam(Ctx* c, float* params, const float* grads, float* m, float* v, int step,

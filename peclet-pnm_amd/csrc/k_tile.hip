@@ -18,7 +18,8 @@
 //     voxels a walk can enter the tile through — so a resolver hops tile to tile;
 //   - the first-appearance minimum of each in-tile root over the tile's final walks (relabel_min
 //     :333-334), stored at the root: this seeds min_pore for every sdf>0 peak.
-// Flood candidates (:245-248) and the extract_pores peak predicate (:87-105) are staged per wave.
+// The extract_pores peak predicate (:87-105) is staged per wave; marker roots seed the solid
+// first-appearance minimum.
 #pragma clang fp contract(off)
 
 #include "k_common.hpp"
@@ -29,9 +30,9 @@ constexpr int T_LX = 66, T_LY = TILE_Y + 2;
 constexpr int T_VOX = 64 * TILE_Y * TILE_Z;
 constexpr int T_SEG = 64 * TILE_Z;  // staging slots per wave
 // DIR byte: bits 0-4 the ascent direction of an sdf>0 voxel (0..26, 13 = itself; kNoDir for
-// other voxels), bit 5 the extract_pores peak predicate, bit 6 flood-band membership.
+// other voxels), bit 5 the extract_pores peak predicate.
 constexpr uint8_t kNoDir = 31;
-constexpr uint8_t kDirMask = 31, kPeakBit = 32, kBandBit = 64;
+constexpr uint8_t kDirMask = 31, kPeakBit = 32;
 
 __device__ __forceinline__ int lds_find(int* L, int x) {
   int p = L[x];
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
       (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * TILE_Y + wave;
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int plane_n = ny * nx;
-  unsigned npk = 0, nbd = 0, nop = 0;
+  unsigned npk = 0, nop = 0;
   auto gidx = [&](int l) {
     return ((z0 + (l >> 9)) * ny + y0 + ((l >> 6) & 7)) * nx + x0 + (l & 63);
   };
@@ -198,8 +199,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     const uint64_t smask = starts & (below | (1ull << lane));
     const int st = smask ? 63 - __clzll((long long)smask) : lane;
     LAB[l] = mk ? l - (lane - st) : -1;
-    const bool band = col_in && !(cv >= 0.0f) && !mk;
-    DIR[l] = dir | (is_peak ? kPeakBit : 0) | (band ? kBandBit : 0);
+    DIR[l] = dir | (is_peak ? kPeakBit : 0);
     (void)i;
     m_prev = m_cur;
     m_cur = m_next;
@@ -219,18 +219,15 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   }
   (void)last_k;
   __syncthreads();
-  // stage peaks and band voxels, in (slice, lane) order per wave
+  // stage peaks in (slice, lane) order per wave
   for (int zl = 0; zl < nzt; ++zl) {
     const int l = (zl * TILE_Y + wave) * 64 + lane;
     const int i = ((z0 + zl) * ny + y) * nx + x;
     const uint8_t f = col_in ? DIR[l] : 0;
-    const bool is_peak = f & kPeakBit, band = f & kBandBit;
+    const bool is_peak = f & kPeakBit;
     const uint64_t pm = __ballot(is_peak);
     if (is_peak) o.peak_stage[seg_id * T_SEG + npk + __popcll(pm & below)] = (uint32_t)i;
     npk += (unsigned)__popcll(pm);
-    const uint64_t bm = __ballot(band);
-    if (band) o.band_stage[seg_id * T_SEG + nbd + __popcll(bm & below)] = (uint32_t)i;
-    nbd += (unsigned)__popcll(bm);
   }
 
   // ---- CCL inside the tile: forward rows (y+1,z), (y-1,z+1), (y,z+1), (y+1,z+1) ----
@@ -282,8 +279,23 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     for (int zl = 0; zl < nzt; ++zl) {
       const int l = (zl * TILE_Y + wave) * 64 + lane;
       const int a = LAB[l];
-      o.labels[((z0 + zl) * ny + y) * nx + x] = a == -1 ? -1 : gidx(a);
+      const int i = ((z0 + zl) * ny + y) * nx + x;
+      o.labels[i] = a == -1 ? -1 : gidx(a);
+      // a component's root is its smallest marker, one of the tile-local roots: seed the
+      // solid first-appearance minimum there before any flood can lower it
+      if (a == l) o.min_solid[i] = i;
     }
+  }
+  {  // stage the tile-local roots (k_lroot_flatten resolves them after the seams)
+    unsigned nr = 0;
+    for (int zl = 0; zl < nzt; ++zl) {
+      const int l = (zl * TILE_Y + wave) * 64 + lane;
+      const bool r = col_in && LAB[l] == l;
+      const uint64_t rm = __ballot(r);
+      if (r) o.lroot_stage[seg_id * T_SEG + nr + __popcll(rm & below)] = (uint32_t)gidx(l);
+      nr += (unsigned)__popcll(rm);
+    }
+    if (lane == 0) o.lroot_cnt[seg_id] = nr;
   }
   __syncthreads();
 
@@ -418,20 +430,68 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   }
   if (lane == 0) {
     o.peak_cnt[seg_id] = npk;
-    o.band_cnt[seg_id] = nbd;
     o.open_cnt[seg_id] = nop;
   }
 }
 
 // ---- cross-tile and wrapping marker relations ----
-// One wave per 64-voxel row chunk. The +x relation at the chunk start is always global; a
-// forward row in another tile (or reached by wrapping) gets the full run-union rule; a forward
-// row in the same tile only needs the two contacts the tile skipped: the chunk's first column
-// with column x-1 and its last column with column x+1.
-__global__ __launch_bounds__(256) void k_ccl_seams(int* L, int nx, int ny, int nz) {
+// After k_tile, the relations left are those between voxels of different tiles or across the
+// periodic wrap. For a row chunk (64 voxels of one (y, z) row) and its forward rows (y+1, z),
+// (y-1, z+1), (y, z+1), (y+1, z+1): a forward row in another tile (or reached by wrapping)
+// needs the full run-union rule; one in the same tile only the two contacts the tile skipped
+// (the chunk's first column with x-1, its last column with x+1); and the +x relation at the
+// chunk start is always cross-tile. Only rows at a tile boundary in y or z have crossing
+// forward rows: y % TILE_Y in {0, TILE_Y-1}, y = ny-1, z % TILE_Z = TILE_Z-1, z = nz-1.
+
+__device__ __forceinline__ void seam_rows(int y, int z, int nx, int ny, int nz, int rows[4],
+                                          bool same[4]) {
+  const int yp = wp1(y, ny), ym = wm1(y, ny), zp = wp1(z, nz);
+  const bool ty_p = (y + 1 < ny) && ((y + 1) / TILE_Y == y / TILE_Y);
+  const bool ty_m = (y >= 1) && ((y - 1) / TILE_Y == y / TILE_Y);
+  const bool tz_p = (z + 1 < nz) && ((z + 1) / TILE_Z == z / TILE_Z);
+  rows[0] = (z * ny + yp) * nx;
+  rows[1] = (zp * ny + ym) * nx;
+  rows[2] = (zp * ny + y) * nx;
+  rows[3] = (zp * ny + yp) * nx;
+  same[0] = ty_p;
+  same[1] = ty_m && tz_p;
+  same[2] = tz_p;
+  same[3] = ty_p && tz_p;
+}
+__device__ __forceinline__ bool y_boundary(int y, int ny) {
+  return y % TILE_Y == 0 || y % TILE_Y == TILE_Y - 1 || y == ny - 1;
+}
+__device__ __forceinline__ int y_boundary_row(int j, int ny) {  // j < 2 * ceil(ny / TILE_Y)
+  const int y = (j >> 1) * TILE_Y + ((j & 1) ? TILE_Y - 1 : 0);
+  return y < ny ? y : ny - 1;
+}
+__device__ __forceinline__ int z_boundary_slice(int j, int nz) {  // j < ceil(nz / TILE_Z)
+  const int z = j * TILE_Z + TILE_Z - 1;
+  return z < nz ? z : nz - 1;
+}
+
+// Full run-union rule on the crossing forward rows of one row chunk (one wave).
+// mode 0: rows at a y boundary (index blockIdx.y * 4 + wave into them), every z;
+// mode 1: slices at a z boundary (index blockIdx.z), rows not at a y boundary.
+__global__ __launch_bounds__(256) void k_ccl_seams_rows(int* L, int nx, int ny, int nz, int mode,
+                                                        int nsel) {
   const int lane = threadIdx.x;
-  const int x0 = blockIdx.x * 64, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
-  if (y >= ny) return;  // wave-uniform
+  const int x0 = blockIdx.x * 64;
+  int y, z;
+  if (mode == 0) {
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (j >= nsel) return;  // wave-uniform
+    y = y_boundary_row(j, ny);
+    z = blockIdx.z;
+  } else {
+    y = blockIdx.y * 4 + threadIdx.y;
+    if (y >= ny || y_boundary(y, ny)) return;  // wave-uniform
+    z = z_boundary_slice(blockIdx.z, nz);
+  }
+  int rows[4];
+  bool same[4];
+  seam_rows(y, z, nx, ny, nz, rows, same);
+  if (same[0] && same[1] && same[2] && same[3]) return;
   const int x = x0 + lane;
   const bool in = x < nx;
   const int xlast = min(x0 + 63, nx - 1);
@@ -443,37 +503,9 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, int nx, int ny, int n
   const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
   const bool run_end = mk && (x == xlast || !((m >> (lane + 1)) & 1ull));
   const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
-  if (mk && lane == 0) {
-    const int nl = L[(z * ny + y) * nx + xm];
-    if (nl != -1) uf_unite(L, my, nl);
-  }
-  const int yp = wp1(y, ny), ym = wm1(y, ny), zp = wp1(z, nz);
-  const bool ty_p = (y + 1 < ny) && ((y + 1) / TILE_Y == y / TILE_Y);
-  const bool ty_m = (y >= 1) && ((y - 1) / TILE_Y == y / TILE_Y);
-  const bool tz_p = (z + 1 < nz) && ((z + 1) / TILE_Z == z / TILE_Z);
-  const int rows[4] = {(z * ny + yp) * nx, (zp * ny + ym) * nx, (zp * ny + y) * nx,
-                       (zp * ny + yp) * nx};
-  const bool same[4] = {ty_p, ty_m && tz_p, tz_p, ty_p && tz_p};
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    if (same[r]) {
-      // edge contacts only: first column with x-1, last column with x+1
-      const bool edge_s = mk && lane == 0, edge_e = mk && x == xlast;
-      if (edge_s || edge_e) {
-        const int c = L[rows[r] + x];
-        if (c == -1) {
-          if (edge_s) {
-            const int a = L[rows[r] + xm];
-            if (a != -1) uf_unite(L, my, a);
-          }
-          if (edge_e) {
-            const int b = L[rows[r] + xp];
-            if (b != -1) uf_unite(L, my, b);
-          }
-        }
-      }
-      continue;
-    }
+    if (same[r]) continue;  // wave-uniform; edges of same-tile rows: k_ccl_seams_edges
     const int c = in ? L[rows[r] + x] : -1;
     const bool cm = c != -1;
     const int prev_c = __shfl_up(c, 1, 64);
@@ -492,6 +524,71 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, int nx, int ny, int n
       }
     }
   }
+}
+
+// One thread per row chunk: the +x relation at the chunk start, and for forward rows in the
+// same tile the chunk's first column with column x-1 and its last column with column x+1.
+__global__ __launch_bounds__(256) void k_ccl_seams_edges(int* L, int nx, int ny, int nz) {
+  const int cx = (nx + 63) >> 6;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)cx * ny * nz) return;
+  const int c = (int)(t % cx);
+  const int64_t r = t / cx;
+  const int y = (int)(r % ny), z = (int)(r / ny);
+  const int x0 = c * 64, xlast = min(x0 + 63, nx - 1);
+  const int row = (z * ny + y) * nx;
+  const int m0 = L[row + x0], mL = L[row + xlast];
+  if (m0 == -1 && mL == -1) return;
+  const int xm = wm1(x0, nx), xp = wp1(xlast, nx);
+  if (m0 != -1) {
+    const int nl = L[row + xm];
+    if (nl != -1) uf_unite(L, m0, nl);
+  }
+  int rows[4];
+  bool same[4];
+  seam_rows(y, z, nx, ny, nz, rows, same);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!same[q]) continue;
+    if (m0 != -1 && L[rows[q] + x0] == -1) {
+      const int a = L[rows[q] + xm];
+      if (a != -1) uf_unite(L, m0, a);
+    }
+    if (mL != -1 && L[rows[q] + xlast] == -1) {
+      const int b = L[rows[q] + xp];
+      if (b != -1) uf_unite(L, mL, b);
+    }
+  }
+}
+
+// ---- tile-local roots -> component roots ----
+// One wave per staging segment of k_tile: after the seams, every tile-local root points at its
+// component root, so a marker's component is L[L[i]].
+__global__ __launch_bounds__(256) void k_lroot_flatten(int* L, const uint32_t* __restrict__ stage,
+                                                       const uint32_t* __restrict__ cnt,
+                                                       int64_t nseg) {
+  const int64_t seg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (seg >= nseg) return;
+  const int lane = threadIdx.x & 63;
+  const unsigned c = cnt[seg];
+  for (unsigned j = lane; j < c; j += 64) {
+    // read-only chase, one final store: a path-halving store here could overwrite the root
+    // another thread has just stored for its own local root with a lesser ancestor
+    const int r = (int)stage[seg * T_SEG + j];
+    int x = r, p = L[r];
+    while (p != x) {
+      x = p;
+      p = L[x];
+    }
+    L[r] = x;
+  }
+}
+
+void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
+                          const uint32_t* cnt) {
+  const int64_t nseg = tile_segments(v);
+  PNM_LAUNCH(c, "k_lroot_flatten", k_lroot_flatten, dim3(cdiv(nseg, 4)), dim3(256), 0, labels,
+             stage, cnt, nseg);
 }
 
 // ---- walks that left their tile ----
@@ -593,8 +690,14 @@ void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
 }
 
 void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels) {
-  PNM_LAUNCH(c, "k_ccl_seams", k_ccl_seams, dim3(cdiv(v.nx, 64), cdiv(v.ny, 4), (unsigned)v.nz),
-             dim3(64, 4), 0, labels, v.nx, v.ny, v.nz);
+  const unsigned cx = cdiv(v.nx, 64);
+  const int ysel = 2 * (int)cdiv(v.ny, TILE_Y), zsel = (int)cdiv(v.nz, TILE_Z);
+  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows, dim3(cx, cdiv(ysel, 4), (unsigned)v.nz),
+             dim3(64, 4), 0, labels, v.nx, v.ny, v.nz, 0, ysel);
+  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows, dim3(cx, cdiv(v.ny, 4), (unsigned)zsel),
+             dim3(64, 4), 0, labels, v.nx, v.ny, v.nz, 1, zsel);
+  PNM_LAUNCH(c, "k_ccl_seams_edges", k_ccl_seams_edges,
+             dim3(cdiv((int64_t)cx * v.ny * v.nz, 256)), dim3(256), 0, labels, v.nx, v.ny, v.nz);
 }
 
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,

@@ -27,34 +27,108 @@ __device__ __forceinline__ uint32_t hash64(uint64_t k) {
 // previous lane's key for the same direction skips (rows along a boundary repeat the pair), the
 // rest go through an LDS hash set; the set is staged per block (no global counter). Keys that
 // find no slot within the probe limit are appended to a small overflow list.
-__global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ seg, int nx,
-                                                   int ny, int nz, int32_t lo, int bits,
+// Source of the labels: a finished seg array, or (SegSrc::ids) the segmentation's last step
+// fused in — relabel_seg (:378-381) computed for the brick and its +x/+y/+z halo into LDS, the
+// brick's part written out as seg.
+struct SegIds {
+  const float* sdf;
+  const int32_t* roots;
+  const int32_t* labels;
+  const int32_t* pore_id;
+  const int32_t* solid_id;
+  int32_t* seg_out;
+};
+constexpr int TB_X = 65, TB_Y = 5, TB_Z = TOPO_ZB + 1;  // brick + forward halo
+
+template <bool FUSED>
+__global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ seg, SegIds ids,
+                                                   int nx, int ny, int nz, int32_t lo, int bits,
                                                    uint64_t* __restrict__ stage,
                                                    uint32_t* __restrict__ scnt,
                                                    uint64_t* __restrict__ overflow,
                                                    unsigned long long* ocount) {
   __shared__ unsigned long long table[TOPO_TABLE];
+  __shared__ int S[FUSED ? TB_X * TB_Y * TB_Z : 1];
   __shared__ unsigned wc[4];
   __shared__ unsigned s_base;
   const int tid = threadIdx.y * 64 + threadIdx.x;
   const int lane = threadIdx.x, wave = threadIdx.y;
   for (int s = tid; s < TOPO_TABLE; s += 256) table[s] = kEmpty;
   if (tid == 0) s_base = 0;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, z0 = blockIdx.z * TOPO_ZB;
+  if (FUSED) {
+    // Rows of the brick + halo over the 4 waves; lane covers column lane (and lane 0 the halo
+    // column 64). Positions past the volume hold the wrapped voxel: the +x/+y/+z neighbour of
+    // the volume's last voxel is read there. Brick voxels inside the volume also write seg.
+    const int gx0 = wrap_any(x0 + lane, nx), gx1 = wrap_any(x0 + 64, nx);
+    // pass 1: the value's sign picks the label source (root of an sdf>0 voxel, flagged in the
+    // top bit, or the solid label); pass 2: the id. Loads are issued in branch-free batches of
+    // RB rows into registers (rows past the region re-read a valid row).
+    constexpr int NROW = TB_Y * TB_Z, RB = 11, MAXR = (NROW + 3) / 4;
+    const bool two = lane == 0;
+    for (int j0 = 0; j0 < MAXR; j0 += RB) {
+      int k0[RB], k1[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        const int row0 = wave + (j0 + j) * 4;
+        const int row = row0 < NROW ? row0 : wave;
+        const int ry = row % TB_Y, rz = row / TB_Y;
+        const int64_t base =
+            ((int64_t)wrap_any(z0 + rz, nz) * ny + wrap_any(y0 + ry, ny)) * nx;
+        const int64_t g = base + gx0, g1 = base + gx1;
+        const float v = ids.sdf[g], v1 = ids.sdf[g1];
+        const int rt = ids.roots[g], lb = ids.labels[g];
+        const int rt1 = ids.roots[g1], lb1 = ids.labels[g1];
+        k0[j] = v > 0.0f ? (int)((uint32_t)rt | 0x80000000u) : lb;
+        k1[j] = v1 > 0.0f ? (int)((uint32_t)rt1 | 0x80000000u) : lb1;
+      }
+      int s0[RB], s1[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        const int a0 = k0[j] < -1 ? ids.pore_id[(uint32_t)k0[j] & 0x7fffffffu]
+                                  : ids.solid_id[k0[j] >= 0 ? k0[j] : 0];
+        const int a1 = k1[j] < -1 ? ids.pore_id[(uint32_t)k1[j] & 0x7fffffffu]
+                                  : ids.solid_id[k1[j] >= 0 ? k1[j] : 0];
+        s0[j] = k0[j] == -1 ? 0 : a0;
+        s1[j] = k1[j] == -1 ? 0 : a1;
+      }
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        const int row = wave + (j0 + j) * 4;
+        if (row < NROW) {
+          const int ry = row % TB_Y, rz = row / TB_Y;
+          S[row * TB_X + lane] = s0[j];
+          if (two) S[row * TB_X + 64] = s1[j];
+          if (ry < 4 && rz < TOPO_ZB && x0 + lane < nx && y0 + ry < ny && z0 + rz < nz)
+            ids.seg_out[((int64_t)(z0 + rz) * ny + (y0 + ry)) * nx + gx0] = s0[j];
+        }
+      }
+    }
+  }
   __syncthreads();
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int x = x0 + threadIdx.x;
+  const int y = y0 + threadIdx.y;
   const bool in = x < nx && y < ny;
   const int xp = in ? wp1(x, nx) : 0, yp = in ? wp1(y, ny) : 0;
   for (int zz = 0; zz < TOPO_ZB; ++zz) {
-    const int z = blockIdx.z * TOPO_ZB + zz;
+    const int z = z0 + zz;
     const bool act = in && z < nz;
     uint64_t key[3] = {kEmpty, kEmpty, kEmpty};
     if (act) {
-      const int zp = wp1(z, nz);
-      const int i = (z * ny + y) * nx + x;
-      const int my = seg[i];
-      const int nb[3] = {seg[(z * ny + y) * nx + xp], seg[(z * ny + yp) * nx + x],
-                         seg[(zp * ny + y) * nx + x]};
+      int my, nb[3];
+      if (FUSED) {
+        const int e = (zz * TB_Y + wave) * TB_X + lane;
+        my = S[e];
+        nb[0] = S[e + 1];
+        nb[1] = S[e + TB_X];
+        nb[2] = S[e + TB_X * TB_Y];
+      } else {
+        const int zp = wp1(z, nz);
+        my = seg[(z * ny + y) * nx + x];
+        nb[0] = seg[(z * ny + y) * nx + xp];
+        nb[1] = seg[(z * ny + yp) * nx + x];
+        nb[2] = seg[(zp * ny + y) * nx + x];
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const int nl = nb[k];
@@ -122,8 +196,9 @@ static int nbits(uint64_t v) {
   return b;
 }
 
-int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi_label,
-                 bool labels_known, int32_t* pairs_d, int64_t cap, int64_t* needed) {
+static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol v,
+                             int32_t lo_label, int32_t hi_label, bool labels_known,
+                             int32_t* pairs_d, int64_t cap, int64_t* needed) {
   int32_t lo = INT_MIN;
   int bits = 32;
   if (labels_known) {
@@ -141,8 +216,14 @@ int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi
   fill_u64(c, cnt, 0, 2);
   // overflow keys go to the tail of `cand`: the staged tables never exceed the brick's 3*voxels
   // faces, and together both never exceed 3n
-  PNM_LAUNCH(c, "k_topo_cand", k_topo_cand, grid, dim3(64, 4, 1), 0, seg, v.nx, v.ny, v.nz, lo,
-             bits, stage, scnt, cand, reinterpret_cast<unsigned long long*>(cnt + 1));
+  if (ids)
+    PNM_LAUNCH(c, "k_seg_topo", k_topo_cand<true>, grid, dim3(64, 4, 1), 0, seg, *ids, v.nx,
+               v.ny, v.nz, lo, bits, stage, scnt, cand,
+               reinterpret_cast<unsigned long long*>(cnt + 1));
+  else
+    PNM_LAUNCH(c, "k_topo_cand", k_topo_cand<false>, grid, dim3(64, 4, 1), 0, seg, SegIds{},
+               v.nx, v.ny, v.nz, lo, bits, stage, scnt, cand,
+               reinterpret_cast<unsigned long long*>(cnt + 1));
   uint64_t h[2];
   readback(c, cnt, 2, h);
   const int64_t nover = (int64_t)h[1];
@@ -161,6 +242,20 @@ int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi
   PNM_LAUNCH(c, "k_topo_decode", k_topo_decode, dim3(cdiv(nu, 256)), dim3(256), 0, uniq, nu, lo,
              bits, pairs_d);
   return nu;
+}
+
+int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi_label,
+                 bool labels_known, int32_t* pairs_d, int64_t cap, int64_t* needed) {
+  return topology_impl(c, seg, nullptr, v, lo_label, hi_label, labels_known, pairs_d, cap,
+                       needed);
+}
+
+int64_t seg_topology(Ctx* c, const float* sdf, const int32_t* roots, const int32_t* labels,
+                     const int32_t* pore_id, const int32_t* solid_id, int32_t* seg, Vol v,
+                     int32_t lo_label, int32_t hi_label, int32_t* pairs_d, int64_t cap,
+                     int64_t* needed) {
+  const SegIds ids{sdf, roots, labels, pore_id, solid_id, seg};
+  return topology_impl(c, nullptr, &ids, v, lo_label, hi_label, true, pairs_d, cap, needed);
 }
 
 }  // namespace pnm

@@ -53,10 +53,16 @@ struct SegInternal {
   SegResult r;
   uint32_t* peaks;  // unsorted peak list (extract_pores predicate)
   int64_t npeaks;
+  // inputs of relabel_seg, for a caller that fuses it (seg_topology)
+  const int32_t* roots;
+  const int32_t* labels;
+  const int32_t* pore_id;
+  const int32_t* solid_id;
 };
 
 // labels_out / ccl_out (nullable): caller buffers receiving the marker labels after the flood /
 // after the CCL; labels_only stops after the flood (stage-level tests).
+// seg == nullptr: everything up to the id assignment; the caller writes seg (fused).
 static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc[3],
                                 int32_t* seg, int32_t* roots_out, int32_t* labels_out = nullptr,
                                 int32_t* ccl_out = nullptr, bool labels_only = false) {
@@ -79,6 +85,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   uint32_t* cnt_a = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
   uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
   uint32_t* cnt_c = c->arena.get_n<uint32_t>((size_t)nseg);
+  uint32_t* stage_r = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
+  uint32_t* cnt_r = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)n);
@@ -90,28 +98,34 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   //      [8..] band counts
   fill_u64(c, ctr, 0, 64);
 
+  int32_t* lab_ccl = c->arena.get_n<int32_t>((size_t)n);  // CCL labels (tile-local roots)
   TileOut o{};
-  o.labels = labels;
+  o.labels = lab_ccl;
   o.roots = roots;
   o.walk = walk;
   o.min_pore = min_pore;
+  o.min_solid = min_solid;
   o.peak_stage = stage_a;
   o.peak_cnt = cnt_a;
-  o.band_stage = stage_b;
-  o.band_cnt = cnt_b;
+  o.lroot_stage = stage_r;
+  o.lroot_cnt = cnt_r;
   o.open_stage = stage_c;
   o.open_cnt = cnt_c;
   o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
   launch_tile(c, sdf, v, thr, o);
   compact_staged(c, stage_a, cnt_a, nseg, scap, peaks, ctr + 0);
-  compact_staged(c, stage_b, cnt_b, nseg, scap, band, ctr + 8);
   compact_staged(c, stage_c, cnt_c, nseg, scap, open, ctr + 1);
-  launch_ccl_seams(c, v, labels);
-  launch_ccl_flatten(c, v, labels, min_solid, stage_a, cnt_a);
-  compact_staged(c, stage_a, cnt_a, nb1, 256, skeys, ctr + 2);
-  if (ccl_out)
-    PNM_HIP(hipMemcpyAsync(ccl_out, labels, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
+  launch_ccl_seams(c, v, lab_ccl);
+  launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
+  if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
+    launch_ccl_flatten(c, v, lab_ccl, min_solid, stage_b, cnt_b);
+    PNM_HIP(hipMemcpyAsync(ccl_out, lab_ccl, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
                            c->stream));
+  }
+  // flatten + the first flood sweeps per tile; the band left over goes to the global sweeps
+  launch_flood_tile(c, sdf, v, lab_ccl, labels, min_solid, stage_b, cnt_b, stage_c, cnt_c);
+  compact_staged(c, stage_b, cnt_b, nseg, scap, skeys, ctr + 2);
+  compact_staged(c, stage_c, cnt_c, nseg, scap, band, ctr + 8);
   uint64_t h[9];
   readback(c, ctr, 9, h);
   const int64_t npeaks = (int64_t)h[0], nopen = (int64_t)h[1], nskeys = (int64_t)h[2],
@@ -119,14 +133,14 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   if (!labels_only)
     launch_roots_resolve(c, sdf, v, open, ctr + 1, nopen, walk, roots, min_pore, ctr + 3);
 
-  // Jacobi flood until a sweep labels nothing (pore_extraction.hpp:238-269). Band lengths live
-  // on the device (ctr[8 + s]); the host checks them every two sweeps.
+  // Remaining Jacobi sweeps until a sweep labels nothing (pore_extraction.hpp:238-269). Band
+  // lengths live on the device (ctr[8 + s]); the host checks them every two sweeps.
   if (nband > 0) {
     uint32_t* cur = band;
     uint32_t* nxt = band2;
     int s = 0;
     while (true) {
-      if (s + 2 >= 56) fail(PNM_ENOCONVERGE, "flood did not converge within 54 sweeps");
+      if (s + 2 >= 54) fail(PNM_ENOCONVERGE, "flood did not converge within 54 sweeps");
       for (int k = 0; k < 2; ++k, ++s) {
         flood_sweep(c, v, labels, cur, ctr + 8 + s, nband, nxt, ctr + 8 + s + 1, newlab, stage_b,
                     cnt_b, min_solid);
@@ -152,13 +166,17 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   }
   assign_ids(c, sdf, pkeys, npk, min_pore, false, ctr + 4);
   assign_ids(c, sdf, skeys, nskeys, min_solid, true, ctr + 5);
-  launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
+  if (seg) launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
   if (roots_out) launch_roots_self(c, sdf, v, roots_out);
   readback(c, ctr + 4, 2, h);
   out.r.n_pores = (int32_t)h[0];
   out.r.n_solids = (int32_t)h[1];
   out.peaks = peaks;
   out.npeaks = npeaks;
+  out.roots = roots;
+  out.labels = labels;
+  out.pore_id = min_pore;
+  out.solid_id = min_solid;
   return out;
 }
 
@@ -183,12 +201,12 @@ int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], cons
                        pnm_pore_t* pores_d, int64_t cap_pores, int64_t* total_pores,
                        int32_t* seg, int32_t* pairs_d, int64_t cap_pairs, int64_t* npairs,
                        bool* pairs_overflow) {
-  SegInternal s = segment_core(c, sdf, v, spc, seg, nullptr);
+  SegInternal s = segment_core(c, sdf, v, spc, nullptr, nullptr);
   const int64_t np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, pores_d, cap_pores,
                                   total_pores);
   int64_t needed = 0;
-  const int64_t nt =
-      topology(c, seg, v, -s.r.n_solids, s.r.n_pores, true, pairs_d, cap_pairs, &needed);
+  const int64_t nt = seg_topology(c, sdf, s.roots, s.labels, s.pore_id, s.solid_id, seg, v,
+                                  -s.r.n_solids, s.r.n_pores, pairs_d, cap_pairs, &needed);
   *npairs = needed;
   *pairs_overflow = nt < 0;
   return np;

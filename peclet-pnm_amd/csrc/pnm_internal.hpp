@@ -153,10 +153,11 @@ struct TileOut {
   int32_t* roots;         // roots of sdf>0 voxels whose walk ends inside their tile    [n]
   uint64_t* walk;         // walk records: sdf>0 voxels on tile boundaries or left open [n]
   int32_t* min_pore;      // first-appearance minimum over the tile, at in-tile roots   [n]
+  int32_t* min_solid;     // seeded with its own index at every tile-local marker root  [n]
   uint32_t* peak_stage;   // staged peaks (extract_pores predicate): tile_segment_cap() per segment
   uint32_t* peak_cnt;
-  uint32_t* band_stage;   // staged flood candidates
-  uint32_t* band_cnt;
+  uint32_t* lroot_stage;  // staged tile-local marker roots
+  uint32_t* lroot_cnt;
   uint32_t* open_stage;   // staged sdf>0 voxels whose walk left the tile
   uint32_t* open_cnt;
   unsigned long long* cap_flag;  // set when a walk stops at the cap on a non-fixed point
@@ -165,12 +166,21 @@ int64_t tile_segments(Vol v);
 int64_t tile_segment_cap();
 void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o);
 void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels);
+// After the seams: every tile-local root (staged by the tile pass) -> its component root.
+void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
+                          const uint32_t* cnt);
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,
                           const uint64_t* count, int64_t grid_items, const uint64_t* walk,
                           int32_t* roots, int32_t* min_pore, uint64_t* cap_flag);
 void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots);
 void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                         uint32_t* key_cnt);
+// Flatten + the first two flood sweeps per tile: reads the CCL labels (tile-local roots whose
+// own entries already hold their component root), writes the flooded labels to a second array
+// (segments as tile_segments / tile_segment_cap).
+void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
+                       int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
+                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt);
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                 uint32_t* key_cnt);
 // One Jacobi flood sweep over the band list whose length is the device word *count (grid sized
@@ -201,6 +211,12 @@ void launch_chase(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t*
 // (l2 - lo). Returns count; pairs_d receives (l1,l2) int32 pairs.
 int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi_label,
                  bool labels_known, int32_t* pairs_d, int64_t cap, int64_t* needed);
+// relabel_seg fused into the topology candidates: writes seg and returns the pair count as
+// topology() does.
+int64_t seg_topology(Ctx* c, const float* sdf, const int32_t* roots, const int32_t* labels,
+                     const int32_t* pore_id, const int32_t* solid_id, int32_t* seg, Vol v,
+                     int32_t lo_label, int32_t hi_label, int32_t* pairs_d, int64_t cap,
+                     int64_t* needed);
 
 // ---------- pipelines (pipeline.hip) ----------
 struct SegResult {
@@ -257,7 +273,8 @@ void surrogate_init(uint64_t seed, float* host_params);
 void surrogate_forward(Ctx* c, int B, const float* patches, const float* scalars,
                        const float* params, float* out);
 void surrogate_backward(Ctx* c, int B, const float* dy, const float* params, float* grads);
-void surrogate_loss(Ctx* c, int B, const float* y, const float* t, float* dy, float* loss);
+void surrogate_loss(Ctx* c, int B, const float* y, const float* t, float delta, float* dy,
+                    float* loss);
 void surrogate_adam(Ctx* c, float* params, const float* grads, float* m, float* v, int step,
                     float lr, float b1, float b2, float eps);
 void surrogate_patches(Ctx* c, const float* sdf, Vol v, const float* centers, int B, float scale,

@@ -120,7 +120,7 @@ def lib():
         L.pnm_sur_forward_d.restype = st
         L.pnm_sur_backward_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         L.pnm_sur_backward_d.restype = st
-        L.pnm_sur_loss_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp]
+        L.pnm_sur_loss_d.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_float, vp, vp]
         L.pnm_sur_loss_d.restype = st
         L.pnm_sur_adam_d.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float]
@@ -294,9 +294,9 @@ class Context:
         _check(lib().pnm_sur_backward_d(self.h, int(batch), _ptr(dout), _ptr(params),
                                         _ptr(grads)))
 
-    def sur_loss_d(self, batch, pred, target, dpred, loss=None):
-        _check(lib().pnm_sur_loss_d(self.h, int(batch), _ptr(pred), _ptr(target), _ptr(dpred),
-                                    _ptr(loss)))
+    def sur_loss_d(self, batch, pred, target, dpred, loss=None, delta=0.0):
+        _check(lib().pnm_sur_loss_d(self.h, int(batch), _ptr(pred), _ptr(target),
+                                    float(delta), _ptr(dpred), _ptr(loss)))
 
     def sur_adam_d(self, params, grads, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999,
                    eps=1e-8):

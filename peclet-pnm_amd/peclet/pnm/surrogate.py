@@ -103,7 +103,7 @@ class ThroatSurrogate:
     def fit_standardization(self, raw, logg):
         self.mu = raw.mean(axis=0).astype(np.float32)
         self.sd = np.maximum(raw.std(axis=0), 1e-6).astype(np.float32)
-        self.t_mean = float(np.mean(logg))
+        self.t_mean = float(np.median(logg))
 
     def patches(self, sdf_dev, shape_zyx, centers):
         return gather_patches(self.ctx, sdf_dev, shape_zyx, centers)
@@ -121,7 +121,7 @@ class ThroatSurrogate:
                                    self.params, out[a:b])
         return out[:N] + self.t_mean
 
-    def train_step(self, patches, scalars_std, targets, lr):
+    def train_step(self, patches, scalars_std, targets, lr, delta=0.25):
         """One Adam step on a device minibatch; returns the loss tensor (device)."""
         torch = _torch()
         B = patches.shape[0]
@@ -129,15 +129,17 @@ class ThroatSurrogate:
         dy = torch.empty(B, device="cuda")
         loss = torch.empty(1, device="cuda")
         self.ctx.sur_forward_d(B, patches, scalars_std, self.params, y)
-        self.ctx.sur_loss_d(B, y, targets - self.t_mean, dy, loss)
+        self.ctx.sur_loss_d(B, y, targets - self.t_mean, dy, loss, delta=delta)
         self.ctx.sur_backward_d(B, dy, self.params, self.grads)
         self.step += 1
         self.ctx.sur_adam_d(self.params, self.grads, self.m, self.v, self.step, lr=lr)
         return loss
 
     def fit(self, patches, scalars_raw, logg, steps=2000, batch=256, lr=2e-3, seed=0,
-            log_every=0):
-        """Train on device patches [N,4096], host raw scalars [N,4] and host log g [N]."""
+            log_every=0, delta=0.25):
+        """Train on device patches [N,4096], host raw scalars [N,4] and host log g [N] with a
+        Huber loss in log g (threshold delta; a few throats with near-zero Q or dp have very
+        large |log g| errors, and the error measure is a median)."""
         torch = _torch()
         self.fit_standardization(scalars_raw, logg)
         s = torch.from_numpy(self.standardize(scalars_raw).astype(np.float32)).cuda()
@@ -151,7 +153,7 @@ class ThroatSurrogate:
             lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / steps))  # cosine decay
             loss = self.train_step(patches.index_select(0, idx).contiguous(),
                                    s.index_select(0, idx).contiguous(),
-                                   t.index_select(0, idx).contiguous(), lr_k)
+                                   t.index_select(0, idx).contiguous(), lr_k, delta)
             if log_every and (k % log_every == 0 or k == steps - 1):
                 hist.append((k, float(loss.item())))
         return hist

@@ -28,6 +28,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--delta", type=float, default=0.25, help="Huber threshold (<= 0: MSE)")
     ap.add_argument("--out", required=True)
     ap.add_argument("--weights", default=None)
     args = ap.parse_args(argv)
@@ -64,7 +65,7 @@ def main(argv=None):
     torch.cuda.synchronize()
     t1 = time.time()
     hist = model.fit(Ptr, Rtr, Ttr, steps=args.steps, batch=args.batch, lr=args.lr,
-                     seed=args.seed, log_every=max(1, args.steps // 20))
+                     seed=args.seed, log_every=max(1, args.steps // 20), delta=args.delta)
     torch.cuda.synchronize()
     t_train = time.time() - t1
     pred_te = model.predict(Pte, model.standardize(Rte)).cpu().numpy()
@@ -78,7 +79,7 @@ def main(argv=None):
     report = {
         "train_packings": args.train, "test_packings": args.test, "size": args.size,
         "train_throats": int(len(Ttr)), "test_throats": int(len(Tte)),
-        "steps": args.steps, "batch": args.batch, "lr": args.lr,
+        "steps": args.steps, "batch": args.batch, "lr": args.lr, "huber_delta": args.delta,
         "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 2),
         "train_tflops": round(3 * fwd_flops * args.batch * args.steps / t_train / 1e12, 3),
         "median_rel_error_test": err, "median_rel_error_train": err_train,

@@ -73,6 +73,10 @@ def test_loss_and_adam(capi_ctx):
     l_ref, dy_ref = so.mse(yv, tv)
     assert abs(loss.item() - l_ref) < 1e-5 * max(1.0, l_ref)
     np.testing.assert_allclose(dy.cpu().numpy(), dy_ref, rtol=1e-5, atol=1e-7)
+    capi_ctx.sur_loss_d(B, dev(yv), dev(tv), dy, loss, delta=0.5)
+    l_ref, dy_ref = so.huber(yv, tv, 0.5)
+    assert abs(loss.item() - l_ref) < 1e-5 * max(1.0, l_ref)
+    np.testing.assert_allclose(dy.cpu().numpy(), dy_ref, rtol=1e-5, atol=1e-7)
     n = capi.sur_param_count()
     p0 = rng.normal(size=n).astype(np.float32)
     gv = rng.normal(size=n).astype(np.float32)
