@@ -76,6 +76,7 @@ constexpr int FR_X = 64 + 2 * FLOOD_SWEEPS, FR_Y = TILE_Y + 2 * FLOOD_SWEEPS,
               FR_Z = TILE_Z + 2 * FLOOD_SWEEPS;
 constexpr int FR_N = FR_X * FR_Y * FR_Z;
 constexpr int FR_SEG = 64 * TILE_Z;  // staging slots per wave
+constexpr int FLOOD_LIST = 4096;     // LDS list of band candidates per tile
 // region word: >= 0 label, -1 band candidate, -2 other unlabelled, <= -3 labelled this sweep
 
 __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restrict__ sdf,
@@ -87,6 +88,8 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
                                                             uint32_t* __restrict__ band_stage,
                                                             uint32_t* __restrict__ band_cnt) {
   __shared__ int R[FR_N];
+  __shared__ uint16_t cand_list[FLOOD_LIST];
+  __shared__ unsigned s_ncand;
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int tid = wave * 64 + lane;
   constexpr int NT = 64 * TILE_Y;
@@ -158,36 +161,73 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
     }
   }
   __syncthreads();
+  // Band candidates of the sweep-1 region, compacted into an LDS list so that the sweeps touch
+  // only them (a row-by-row scan costs 26 LDS reads per row whenever one lane is a candidate).
+  // A tile with more candidates than the list holds scans the region instead.
+  if (tid == 0) s_ncand = 0;
+  __syncthreads();
+  for (int row = wave; row < rows; row += TILE_Y) {
+    const int ry = row % FR_Y, rz = row / FR_Y;
+    const bool row_ok = ry >= 1 && ry < FR_Y - 1 && rz >= 1 && rz < nzt + 2 * H - 1;
+    for (int rx0 = 0; rx0 < FR_X; rx0 += 64) {
+      const int rx = rx0 + lane;
+      const int e = row * FR_X + rx;
+      const bool cand = row_ok && rx >= 1 && rx < FR_X - 1 && R[e] == -1;
+      const uint64_t m = __ballot(cand);
+      if (m == 0) continue;
+      unsigned base = 0;
+      if (lane == 0) base = atomicAdd(&s_ncand, (unsigned)__popcll(m));
+      base = __shfl(base, 0, 64);
+      const unsigned pos = base + (unsigned)__popcll(m & below);
+      if (cand && pos < FLOOD_LIST) cand_list[pos] = (uint16_t)e;
+    }
+  }
+  __syncthreads();
+  const unsigned ncand = s_ncand;
+  const bool listed = ncand <= FLOOD_LIST;
+  auto sweep_one = [&](int e, int k) {
+    const int rz = e / (FR_X * FR_Y), rem = e - rz * (FR_X * FR_Y);
+    const int ry = rem / FR_X, rx = rem - ry * FR_X;
+    if (rx < k || rx >= FR_X - k || ry < k || ry >= FR_Y - k || rz < k ||
+        rz >= nzt + 2 * H - k)
+      return;
+    if (R[e] != -1) return;
+    int best = -1;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (dx == 0 && dy == 0 && dz == 0) continue;
+          const int nl = R[e + (dz * FR_Y + dy) * FR_X + dx];
+          if (nl >= 0 && (best < 0 || nl < best)) best = nl;
+        }
+    if (best >= 0) R[e] = -3 - best;  // labelled this sweep: reads as unlabelled until the end
+  };
   // Jacobi sweeps on the shrinking region
   for (int k = 1; k <= H; ++k) {
-    for (int row = wave; row < rows; row += TILE_Y) {
-      const int ry = row % FR_Y, rz = row / FR_Y;
-      if (ry < k || ry >= FR_Y - k || rz < k || rz >= nzt + 2 * H - k) continue;  // wave-uniform
-      for (int rx = lane; rx < FR_X; rx += 64) {
-        if (rx < k || rx >= FR_X - k) continue;
-        const int e = row * FR_X + rx;
-        if (R[e] != -1) continue;
-        int best = -1;
-#pragma unroll
-        for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-          for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-              if (dx == 0 && dy == 0 && dz == 0) continue;
-              const int nl = R[e + (dz * FR_Y + dy) * FR_X + dx];
-              if (nl >= 0 && (best < 0 || nl < best)) best = nl;
-            }
-        if (best >= 0) R[e] = -3 - best;  // labelled this sweep: reads as unlabelled until the end
-      }
+    if (listed) {
+      for (unsigned j = tid; j < ncand; j += NT) sweep_one(cand_list[j], k);
+    } else {
+      for (int row = wave; row < rows; row += TILE_Y)
+        for (int rx = lane; rx < FR_X; rx += 64) sweep_one(row * FR_X + rx, k);
     }
     __syncthreads();
-    for (int row = wave; row < rows; row += TILE_Y)
-      for (int rx = lane; rx < FR_X; rx += 64) {
-        const int e = row * FR_X + rx;
+    if (listed) {
+      for (unsigned j = tid; j < ncand; j += NT) {
+        const int e = cand_list[j];
         const int v = R[e];
         if (v <= -3) R[e] = -3 - v;
       }
+    } else {
+      for (int row = wave; row < rows; row += TILE_Y)
+        for (int rx = lane; rx < FR_X; rx += 64) {
+          const int e = row * FR_X + rx;
+          const int v = R[e];
+          if (v <= -3) R[e] = -3 - v;
+        }
+    }
     __syncthreads();
   }
   // tile outputs, staged per wave in (slice, lane) order
