@@ -161,6 +161,10 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
     }
   }
   __syncthreads();
+  // which of this thread's tile voxels start unlabelled (flooded voxels lower the minimum)
+  uint32_t cand0 = 0;
+  for (int zl = 0; zl < nzt; ++zl)
+    if (R[((zl + H) * FR_Y + wave + H) * FR_X + lane + H] == -1) cand0 |= 1u << zl;
   // Band candidates of the sweep-1 region, compacted into an LDS list so that the sweeps touch
   // only them (a row-by-row scan costs 26 LDS reads per row whenever one lane is a candidate).
   // A tile with more candidates than the list holds scans the region instead.
@@ -234,12 +238,6 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   const int x = x0 + lane, y = y0 + wave;
   const bool col_in = x < nx && y < ny;
   unsigned nk = 0, nb = 0;
-  int before[TILE_Z];  // the CCL label of each of this column's voxels (all loads in flight)
-#pragma unroll
-  for (int zl = 0; zl < TILE_Z; ++zl) {
-    const int zc = zl < nzt ? z0 + zl : z0;
-    before[zl] = L[((int64_t)zc * ny + (col_in ? y : 0)) * nx + (col_in ? x : 0)];
-  }
 #pragma unroll
   for (int zl = 0; zl < TILE_Z; ++zl) {
     if (zl >= nzt) break;
@@ -249,7 +247,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
     bool root = false;
     if (col_in) {
       Lout[i] = v >= 0 ? v : -1;
-      if (v >= 0 && before[zl] < 0 && i < v) atomicMin(min_solid + v, i);
+      if (v >= 0 && ((cand0 >> zl) & 1u) && i < v) atomicMin(min_solid + v, i);
       root = v == i;
     }
     const bool band = v == -1;

@@ -180,6 +180,15 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   if (tid == 0) scnt[bid] = s_base;
 }
 
+// Appends the overflow keys after the staged ones (both counts on the device).
+__global__ void k_topo_append(uint64_t* __restrict__ cand, const uint64_t* __restrict__ cnt,
+                              const uint64_t* __restrict__ ovf) {
+  const uint64_t base = cnt[0], nov = cnt[1];
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nov;
+       j += (uint64_t)gridDim.x * blockDim.x)
+    cand[base + j] = ovf[j];
+}
+
 __global__ void k_topo_decode(const uint64_t* __restrict__ keys, int64_t count, int32_t lo,
                               int bits, int32_t* __restrict__ pairs) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,24 +221,24 @@ static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol 
   uint64_t* stage = c->arena.get_n<uint64_t>((size_t)(nblk * TOPO_TABLE));
   uint32_t* scnt = c->arena.get_n<uint32_t>((size_t)nblk);
   uint64_t* cand = c->arena.get_n<uint64_t>((size_t)maxc);
+  uint64_t* ovf = c->arena.get_n<uint64_t>((size_t)maxc);
   uint64_t* cnt = c->arena.get_n<uint64_t>(2);
   fill_u64(c, cnt, 0, 2);
-  // overflow keys go to the tail of `cand`: the staged tables never exceed the brick's 3*voxels
-  // faces, and together both never exceed 3n
+  // keys that find no slot in a brick's table go to `ovf`; the staged tables are compacted to
+  // the head of `cand` and the overflow appended after them, on the device
   if (ids)
     PNM_LAUNCH(c, "k_seg_topo", k_topo_cand<true>, grid, dim3(64, 4, 1), 0, seg, *ids, v.nx,
-               v.ny, v.nz, lo, bits, stage, scnt, cand,
+               v.ny, v.nz, lo, bits, stage, scnt, ovf,
                reinterpret_cast<unsigned long long*>(cnt + 1));
   else
     PNM_LAUNCH(c, "k_topo_cand", k_topo_cand<false>, grid, dim3(64, 4, 1), 0, seg, SegIds{},
-               v.nx, v.ny, v.nz, lo, bits, stage, scnt, cand,
+               v.nx, v.ny, v.nz, lo, bits, stage, scnt, ovf,
                reinterpret_cast<unsigned long long*>(cnt + 1));
+  compact_staged64(c, stage, scnt, nblk, TOPO_TABLE, cand, cnt);
+  PNM_LAUNCH(c, "k_topo_append", k_topo_append, dim3(64), dim3(256), 0, cand, cnt, ovf);
   uint64_t h[2];
   readback(c, cnt, 2, h);
-  const int64_t nover = (int64_t)h[1];
-  compact_staged64(c, stage, scnt, nblk, TOPO_TABLE, cand + nover, cnt);
-  readback(c, cnt, 1, h);
-  const int64_t ncand = nover + (int64_t)h[0];
+  const int64_t ncand = (int64_t)h[0] + (int64_t)h[1];
   if (ncand == 0) {
     *needed = 0;
     return 0;

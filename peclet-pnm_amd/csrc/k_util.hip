@@ -281,6 +281,53 @@ void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64
              total_dev);
 }
 
+// Several staged lists with the same segment layout in one pass: their counts are contiguous
+// (list k at counts + k * nseg), one exclusive scan covers all of them, one gather launch moves
+// every segment to its list (offsets relative to the list's first segment).
+struct MultiLists {
+  const uint32_t* stage[4];
+  uint32_t* out[4];
+  uint64_t* total[4];
+};
+__global__ void k_gather_multi(MultiLists L, const uint32_t* __restrict__ counts,
+                               const uint32_t* __restrict__ offs, int64_t nseg, int64_t cap) {
+  const int64_t b = blockIdx.x;
+  const int k = (int)(b / nseg);
+  const int64_t sgm = b - (int64_t)k * nseg;
+  const uint32_t cnt = counts[b];
+  const uint32_t base = offs[b] - offs[(int64_t)k * nseg];
+  const uint32_t* __restrict__ st = L.stage[k] + sgm * cap;
+  uint32_t* __restrict__ o = L.out[k];
+  for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) o[base + j] = st[j];
+}
+__global__ void k_multi_totals(MultiLists L, const uint32_t* __restrict__ counts,
+                               const uint32_t* __restrict__ offs, int64_t nseg, int K) {
+  const int k = threadIdx.x;
+  if (k >= K) return;
+  const int64_t last = (int64_t)(k + 1) * nseg - 1;
+  *L.total[k] = (uint64_t)(offs[last] + counts[last] - offs[(int64_t)k * nseg]);
+}
+
+void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uint32_t* counts,
+                          int64_t nseg, int64_t cap, uint32_t* const* out,
+                          uint64_t* const* total_dev) {
+  MultiLists L{};
+  for (int k = 0; k < K; ++k) {
+    L.stage[k] = stage[k];
+    L.out[k] = out[k];
+    L.total[k] = total_dev[k];
+  }
+  if (nseg <= 0) {
+    for (int k = 0; k < K; ++k) fill_u64(c, total_dev[k], 0, 1);
+    return;
+  }
+  uint32_t* offs = c->arena.get_n<uint32_t>((size_t)(K * nseg));
+  scan_exclusive_u32(c, counts, offs, K * nseg);
+  PNM_LAUNCH(c, "k_gather_staged", k_gather_multi, dim3((unsigned)(K * nseg)), dim3(64), 0, L,
+             counts, offs, nseg, cap);
+  PNM_LAUNCH(c, "k_staged_total", k_multi_totals, dim3(1), dim3(64), 0, L, counts, offs, nseg, K);
+}
+
 void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
                       int64_t cap, uint64_t* out, uint64_t* total_dev) {
   if (nblocks <= 0) {

@@ -82,9 +82,13 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   uint32_t* stage_a = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_c = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
-  uint32_t* cnt_a = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
+  // counts: [peaks | open] from the tile pass, [solid keys | band] from the flood tile, each
+  // pair contiguous for one multi-list compaction
+  uint32_t* cnt_pair1 = c->arena.get_n<uint32_t>((size_t)(2 * nseg));
+  uint32_t* cnt_pair2 = c->arena.get_n<uint32_t>((size_t)(2 * nseg));
+  uint32_t* cnt_a = cnt_pair1;
+  uint32_t* cnt_c = cnt_pair1 + nseg;
   uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
-  uint32_t* cnt_c = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* stage_r = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
   uint32_t* cnt_r = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
@@ -113,8 +117,12 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   o.open_cnt = cnt_c;
   o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
   launch_tile(c, sdf, v, thr, o);
-  compact_staged(c, stage_a, cnt_a, nseg, scap, peaks, ctr + 0);
-  compact_staged(c, stage_c, cnt_c, nseg, scap, open, ctr + 1);
+  {
+    const uint32_t* st[2] = {stage_a, stage_c};
+    uint32_t* outs[2] = {peaks, open};
+    uint64_t* tots[2] = {ctr + 0, ctr + 1};
+    compact_staged_multi(c, 2, st, cnt_pair1, nseg, scap, outs, tots);
+  }
   launch_ccl_seams(c, v, lab_ccl);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
   if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
@@ -123,9 +131,14 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
                            c->stream));
   }
   // flatten + the first flood sweeps per tile; the band left over goes to the global sweeps
-  launch_flood_tile(c, sdf, v, lab_ccl, labels, min_solid, stage_b, cnt_b, stage_c, cnt_c);
-  compact_staged(c, stage_b, cnt_b, nseg, scap, skeys, ctr + 2);
-  compact_staged(c, stage_c, cnt_c, nseg, scap, band, ctr + 8);
+  launch_flood_tile(c, sdf, v, lab_ccl, labels, min_solid, stage_b, cnt_pair2, stage_c,
+                    cnt_pair2 + nseg);
+  {
+    const uint32_t* st[2] = {stage_b, stage_c};
+    uint32_t* outs[2] = {skeys, band};
+    uint64_t* tots[2] = {ctr + 2, ctr + 8};
+    compact_staged_multi(c, 2, st, cnt_pair2, nseg, scap, outs, tots);
+  }
   uint64_t h[9];
   readback(c, ctr, 9, h);
   const int64_t npeaks = (int64_t)h[0], nopen = (int64_t)h[1], nskeys = (int64_t)h[2],
@@ -135,6 +148,9 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
 
   // Remaining Jacobi sweeps until a sweep labels nothing (pore_extraction.hpp:238-269). Band
   // lengths live on the device (ctr[8 + s]); the host checks them every two sweeps.
+  // The counter block is read whole, so the walk-cap flag (set by the resolver, launched
+  // before the sweeps) comes back with the last band counts.
+  bool have_flag = false;
   if (nband > 0) {
     uint32_t* cur = band;
     uint32_t* nxt = band2;
@@ -146,15 +162,17 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
                     cnt_b, min_solid);
         std::swap(cur, nxt);
       }
-      uint64_t hh[2];
-      readback(c, ctr + 8 + s - 1, 2, hh);
-      if (hh[1] == hh[0] || hh[1] == 0) break;
+      uint64_t hh[64];
+      readback(c, ctr, 8 + s + 1, hh);
+      h[0] = hh[3];
+      have_flag = true;
+      if (hh[8 + s] == hh[8 + s - 1] || hh[8 + s] == 0) break;
     }
   }
 
   SegInternal out{};
   if (labels_only) return out;
-  readback(c, ctr + 3, 1, h);
+  if (!have_flag) readback(c, ctr + 3, 1, h);
   if (h[0] & 2u) fail(PNM_EHIP, "internal error: ascent walk record missing");
   const uint32_t* pkeys = peaks;
   int64_t npk = npeaks;

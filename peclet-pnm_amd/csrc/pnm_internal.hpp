@@ -123,6 +123,11 @@ void radix_sort_u32(Ctx* c, uint32_t* keys, uint32_t* vals, int64_t n, int begin
 // total lands in *total_dev (device word).
 void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64_t nblocks,
                     int64_t cap, uint32_t* out, uint64_t* total_dev);
+// K (<= 4) staged lists sharing one segment layout, counts contiguous (list k at
+// counts + k * nseg), compacted with one scan and one gather; totals written per list.
+void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uint32_t* counts,
+                          int64_t nseg, int64_t cap, uint32_t* const* out,
+                          uint64_t* const* total_dev);
 void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
                       int64_t cap, uint64_t* out, uint64_t* total_dev);
 // Writes keys[i] with keys[i] != keys[i-1] to out; returns the count (synchronises).
