@@ -89,7 +89,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
                                                             uint32_t* __restrict__ band_cnt) {
   __shared__ int R[FR_N];
   __shared__ uint16_t cand_list[FLOOD_LIST];
-  __shared__ unsigned s_ncand;
+  __shared__ unsigned s_wcnt[TILE_Y];
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int tid = wave * 64 + lane;
   constexpr int NT = 64 * TILE_Y;
@@ -165,11 +165,12 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   uint32_t cand0 = 0;
   for (int zl = 0; zl < nzt; ++zl)
     if (R[((zl + H) * FR_Y + wave + H) * FR_X + lane + H] == -1) cand0 |= 1u << zl;
-  // Band candidates of the sweep-1 region, compacted into an LDS list so that the sweeps touch
-  // only them (a row-by-row scan costs 26 LDS reads per row whenever one lane is a candidate).
-  // A tile with more candidates than the list holds scans the region instead.
-  if (tid == 0) s_ncand = 0;
-  __syncthreads();
+  // Band candidates of the sweep-1 region, compacted into an LDS list (one segment per wave,
+  // filled by ballot without atomics) so that the sweeps touch only them (a row-by-row scan
+  // costs 26 LDS reads per row whenever one lane is a candidate). If a wave's rows hold more
+  // candidates than its segment, the tile scans the region instead.
+  constexpr int WL = FLOOD_LIST / TILE_Y;
+  unsigned nw = 0;
   for (int row = wave; row < rows; row += TILE_Y) {
     const int ry = row % FR_Y, rz = row / FR_Y;
     const bool row_ok = ry >= 1 && ry < FR_Y - 1 && rz >= 1 && rz < nzt + 2 * H - 1;
@@ -178,17 +179,16 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
       const int e = row * FR_X + rx;
       const bool cand = row_ok && rx >= 1 && rx < FR_X - 1 && R[e] == -1;
       const uint64_t m = __ballot(cand);
-      if (m == 0) continue;
-      unsigned base = 0;
-      if (lane == 0) base = atomicAdd(&s_ncand, (unsigned)__popcll(m));
-      base = __shfl(base, 0, 64);
-      const unsigned pos = base + (unsigned)__popcll(m & below);
-      if (cand && pos < FLOOD_LIST) cand_list[pos] = (uint16_t)e;
+      const unsigned pos = nw + (unsigned)__popcll(m & below);
+      if (cand && pos < (unsigned)WL) cand_list[wave * WL + pos] = (uint16_t)e;
+      nw += (unsigned)__popcll(m);
     }
   }
+  if (lane == 0) s_wcnt[wave] = nw;
   __syncthreads();
-  const unsigned ncand = s_ncand;
-  const bool listed = ncand <= FLOOD_LIST;
+  bool listed = true;
+#pragma unroll
+  for (int w = 0; w < TILE_Y; ++w) listed &= s_wcnt[w] <= (unsigned)WL;
   auto sweep_one = [&](int e, int k) {
     const int rz = e / (FR_X * FR_Y), rem = e - rz * (FR_X * FR_Y);
     const int ry = rem / FR_X, rx = rem - ry * FR_X;
@@ -212,15 +212,15 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   // Jacobi sweeps on the shrinking region
   for (int k = 1; k <= H; ++k) {
     if (listed) {
-      for (unsigned j = tid; j < ncand; j += NT) sweep_one(cand_list[j], k);
+      for (unsigned j = lane; j < nw; j += 64) sweep_one(cand_list[wave * WL + j], k);
     } else {
       for (int row = wave; row < rows; row += TILE_Y)
         for (int rx = lane; rx < FR_X; rx += 64) sweep_one(row * FR_X + rx, k);
     }
     __syncthreads();
     if (listed) {
-      for (unsigned j = tid; j < ncand; j += NT) {
-        const int e = cand_list[j];
+      for (unsigned j = lane; j < nw; j += 64) {
+        const int e = cand_list[wave * WL + j];
         const int v = R[e];
         if (v <= -3) R[e] = -3 - v;
       }

@@ -61,6 +61,32 @@ __device__ __forceinline__ void lds_unite(int* L, int a, int b) {
   }
 }
 
+// Unites every x-run of row mask `a` with every x-run of row mask `b` it touches under
+// 26-adjacency (columns x-1, x, x+1 inside the 64-column chunk), each run named by the local
+// index of its first voxel (row bases base_a / base_b). The runs and their contacts come from
+// the masks alone (bit scans), so one lane handles a whole pair of rows.
+__device__ __forceinline__ void unite_row_runs(int* LAB, uint64_t a, uint64_t b, int base_a,
+                                               int base_b) {
+  while (a) {
+    const int s = __ffsll((unsigned long long)a) - 1;
+    const uint64_t from_s = ~(a >> s);
+    const int e = from_s ? s + __ffsll((unsigned long long)from_s) - 2 : 63;
+    const int lo = s > 0 ? s - 1 : 0, hi = e < 63 ? e + 1 : 63;
+    const uint64_t win = (hi == 63 ? ~0ull : ((2ull << hi) - 1)) & ~((1ull << lo) - 1);
+    uint64_t bw = b & win;
+    while (bw) {
+      const int t = __ffsll((unsigned long long)bw) - 1;
+      const uint64_t zeros_below = t ? (~b & (~0ull >> (64 - t))) : 0ull;
+      const int sb = zeros_below ? 64 - __clzll((long long)zeros_below) : 0;
+      const uint64_t from_t = ~(b >> t);
+      const int eb = from_t ? t + __ffsll((unsigned long long)from_t) - 2 : 63;
+      lds_unite(LAB, base_a + s, base_b + sb);
+      bw = eb >= 63 ? 0ull : (bw & ~((2ull << eb) - 1));
+    }
+    a = e >= 63 ? 0ull : (a & ~((2ull << e) - 1));
+  }
+}
+
 // Chase state of an sdf>0 voxel during pointer jumping: u (local index, 13 bits), steps to u
 // (10 bits), status of u: ACT = u's own walk not yet known, FIX = u is a fixed point, EXIT = u's
 // successor lies outside the tile, LONG = the walk takes more than 512 steps inside the tile.
@@ -86,6 +112,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   __shared__ uint8_t XD[T_LY][64];
   __shared__ int LAB[T_VOX];
   __shared__ uint8_t DIR[T_VOX];
+  __shared__ uint64_t MASK[TILE_Z * TILE_Y];  // marker mask of each tile row
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int tid = wave * 64 + lane;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TILE_Y, z0 = blockIdx.z * TILE_Z;
@@ -199,6 +226,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     const uint64_t smask = starts & (below | (1ull << lane));
     const int st = smask ? 63 - __clzll((long long)smask) : lane;
     LAB[l] = mk ? l - (lane - st) : -1;
+    if (lane == 0) MASK[zl * TILE_Y + wave] = mm;
     DIR[l] = dir | (is_peak ? kPeakBit : 0);
     (void)i;
     m_prev = m_cur;
@@ -231,41 +259,20 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   }
 
   // ---- CCL inside the tile: forward rows (y+1,z), (y-1,z+1), (y,z+1), (y+1,z+1) ----
-  if (y < ny) {
-    const bool yp_ok = wave + 1 < TILE_Y && y + 1 < ny, ym_ok = wave >= 1;
-    for (int zl = 0; zl < nzt; ++zl) {
-      const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const int a = LAB[l];
-      const bool mk = a != -1;
-      const uint64_t m = __ballot(mk);
-      if (m == 0) continue;
-      const bool run_start = mk && lane > 0 && !((m >> (lane - 1)) & 1ull);
-      const bool run_end = mk && lane < 63 && x + 1 < nx && !((m >> (lane + 1)) & 1ull);
-      const bool zp_ok = zl + 1 < nzt;
-      const int rows[4] = {(zl * TILE_Y + wave + 1) * 64, (((zl + 1) * TILE_Y) + wave - 1) * 64,
-                           ((zl + 1) * TILE_Y + wave) * 64, ((zl + 1) * TILE_Y + wave + 1) * 64};
-      const bool ok[4] = {yp_ok, ym_ok && zp_ok, zp_ok, yp_ok && zp_ok};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (!ok[r]) continue;  // wave-uniform
-        const int c = LAB[rows[r] + lane];
-        const bool cm = c != -1;
-        const int prev_c = __shfl_up(c, 1, 64);
-        if (mk && cm) {
-          const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
-          if (!dup) lds_unite(LAB, a, c);
-        }
-        if (mk && !cm) {
-          if (run_start) {
-            const int p = LAB[rows[r] + lane - 1];
-            if (p != -1) lds_unite(LAB, a, p);
-          }
-          if (run_end) {
-            const int q = LAB[rows[r] + lane + 1];
-            if (q != -1) lds_unite(LAB, a, q);
-          }
-        }
-      }
+  // Lane j of wave y takes slice j/4 and forward row j%4 of its row: all 64 row pairs of a wave
+  // run side by side.
+  {
+    const int zl = lane >> 2, r = lane & 3;
+    const bool zp_ok = zl + 1 < nzt, yp_ok = wave + 1 < TILE_Y && y + 1 < ny,
+               ym_ok = wave >= 1;
+    const int yb = r == 0 ? wave + 1 : (r == 1 ? wave - 1 : (r == 2 ? wave : wave + 1));
+    const int zb = r == 0 ? zl : zl + 1;
+    const bool ok = y < ny && zl < nzt &&
+                    (r == 0 ? yp_ok : (r == 1 ? ym_ok && zp_ok : (r == 2 ? zp_ok : yp_ok && zp_ok)));
+    if (ok) {
+      const uint64_t ma = MASK[zl * TILE_Y + wave];
+      const uint64_t mb = MASK[zb * TILE_Y + yb];
+      if (ma && mb) unite_row_runs(LAB, ma, mb, (zl * TILE_Y + wave) * 64, (zb * TILE_Y + yb) * 64);
     }
   }
   __syncthreads();
