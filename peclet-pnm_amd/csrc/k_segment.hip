@@ -487,6 +487,96 @@ void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, i
              id_arr, reinterpret_cast<unsigned long long*>(count_dev));
 }
 
+// ---------------- ids by first-appearance bitmaps ----------------
+// relabel_first + the two exclusive scans + assign (:340-374) without sorting the keys: every
+// key sets the bit of its first-appearance voxel in an n-bit map (pores and solids in separate
+// maps), an exclusive scan of the per-word popcounts turns a bit into its rank, and a key's id
+// is its rank + 1 (pores) or -(rank + 1) (solids whose first voxel has sdf <= 0; other solid
+// keys get id 0, and pore keys that are not sdf>0 — NaN peaks of the extract_pores predicate —
+// get no id).
+__global__ void k_mark_first(const float* __restrict__ sdf, const uint32_t* __restrict__ pkeys,
+                             int64_t npk, const int32_t* __restrict__ pmin,
+                             const uint32_t* __restrict__ skeys, int64_t nsk,
+                             const int32_t* __restrict__ smin, uint32_t* __restrict__ pmap,
+                             uint32_t* __restrict__ smap) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < npk) {
+    const uint32_t r = pkeys[j];
+    if (sdf[r] > 0.0f) {
+      const uint32_t f = (uint32_t)pmin[r];
+      atomicOr(pmap + (f >> 5), 1u << (f & 31));
+    }
+  } else if (j - npk < nsk) {
+    const uint32_t r = skeys[j - npk];
+    const uint32_t f = (uint32_t)smin[r];
+    if (sdf[f] <= 0.0f) atomicOr(smap + (f >> 5), 1u << (f & 31));
+  }
+}
+
+__global__ void k_map_popc(const uint32_t* __restrict__ map, int64_t words,
+                           uint32_t* __restrict__ pc) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < words) pc[w] = (uint32_t)__popc(map[w]);
+}
+
+__global__ void k_assign_ranks(const float* __restrict__ sdf, const uint32_t* __restrict__ pkeys,
+                               int64_t npk, const uint32_t* __restrict__ skeys, int64_t nsk,
+                               int32_t* __restrict__ pid, int32_t* __restrict__ sid,
+                               const uint32_t* __restrict__ pmap,
+                               const uint32_t* __restrict__ prank,
+                               const uint32_t* __restrict__ smap,
+                               const uint32_t* __restrict__ srank, int64_t words,
+                               unsigned long long* __restrict__ counts) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0) {  // totals: rank past the last word
+    counts[0] = (unsigned long long)(prank[words - 1] + __popc(pmap[words - 1]));
+    counts[1] = (unsigned long long)(srank[words - 1] + __popc(smap[words - 1]));
+  }
+  if (j < npk) {
+    const uint32_t r = pkeys[j];
+    if (sdf[r] > 0.0f) {
+      const uint32_t f = (uint32_t)pid[r];  // still the first-appearance voxel
+      const uint32_t rk = prank[f >> 5] + __popc(pmap[f >> 5] & ((1u << (f & 31)) - 1u));
+      pid[r] = (int32_t)(rk + 1);
+    }
+  } else if (j - npk < nsk) {
+    const uint32_t r = skeys[j - npk];
+    const uint32_t f = (uint32_t)sid[r];
+    if (sdf[f] <= 0.0f) {
+      const uint32_t rk = srank[f >> 5] + __popc(smap[f >> 5] & ((1u << (f & 31)) - 1u));
+      sid[r] = -(int32_t)(rk + 1);
+    } else {
+      sid[r] = 0;
+    }
+  }
+}
+
+// Both key sets at once; counts_dev[0] = number of pore ids, counts_dev[1] = solid ids.
+void assign_ids_bitmap(Ctx* c, const float* sdf, Vol v, const uint32_t* pkeys, int64_t npk,
+                       int32_t* pore_ids, const uint32_t* skeys, int64_t nsk, int32_t* solid_ids,
+                       uint64_t* counts_dev) {
+  const int64_t words = (v.n + 31) / 32;
+  uint32_t* pmap = c->arena.get_n<uint32_t>((size_t)words);
+  uint32_t* smap = c->arena.get_n<uint32_t>((size_t)words);
+  uint32_t* pc = c->arena.get_n<uint32_t>((size_t)(2 * words));
+  uint32_t* rank = c->arena.get_n<uint32_t>((size_t)(2 * words));
+  fill_u32(c, pmap, 0u, words);
+  fill_u32(c, smap, 0u, words);
+  const int64_t nk = npk + nsk;
+  if (nk > 0)
+    PNM_LAUNCH(c, "k_mark_first", k_mark_first, dim3(cdiv(nk, 256)), dim3(256), 0, sdf, pkeys,
+               npk, (const int32_t*)pore_ids, skeys, nsk, (const int32_t*)solid_ids, pmap, smap);
+  // popcounts of both maps side by side, one scan (the solid ranks restart at their map start)
+  PNM_LAUNCH(c, "k_map_popc", k_map_popc, dim3(cdiv(words, 256)), dim3(256), 0, pmap, words, pc);
+  PNM_LAUNCH(c, "k_map_popc", k_map_popc, dim3(cdiv(words, 256)), dim3(256), 0, smap, words,
+             pc + words);
+  scan_exclusive_u32(c, pc, rank, words);
+  scan_exclusive_u32(c, pc + words, rank + words, words);
+  PNM_LAUNCH(c, "k_assign_ranks", k_assign_ranks, dim3(cdiv(std::max<int64_t>(nk, 1), 256)),
+             dim3(256), 0, sdf, pkeys, npk, skeys, nsk, pore_ids, solid_ids, pmap, rank, smap,
+             rank + words, words, reinterpret_cast<unsigned long long*>(counts_dev));
+}
+
 // relabel_seg (pore_extraction.hpp:378-381).
 __global__ __launch_bounds__(256) void k_seg(const float* __restrict__ sdf, int64_t n,
                                              const int32_t* __restrict__ roots,

@@ -110,6 +110,9 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   const int y = y0 + threadIdx.y;
   const bool in = x < nx && y < ny;
   const int xp = in ? wp1(x, nx) : 0, yp = in ? wp1(y, ny) : 0;
+  // a face pair repeats along x (the previous lane) and along z (this lane's previous slice):
+  // both repeats skip the hash set
+  uint64_t last[3] = {kEmpty, kEmpty, kEmpty};
   for (int zz = 0; zz < TOPO_ZB; ++zz) {
     const int z = z0 + zz;
     const bool act = in && z < nz;
@@ -141,7 +144,9 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const uint64_t prev = __shfl_up(key[k], 1, 64);
-      if (key[k] == kEmpty || (lane > 0 && prev == key[k])) continue;
+      const bool rep = key[k] == last[k];
+      last[k] = key[k];
+      if (key[k] == kEmpty || rep || (lane > 0 && prev == key[k])) continue;
       uint32_t h = hash64(key[k]) & (TOPO_TABLE - 1);
       bool placed = false;
       for (int p = 0; p < TOPO_PROBES; ++p) {

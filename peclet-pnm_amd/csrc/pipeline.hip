@@ -182,8 +182,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
     npk = collect_keys_full(c, sdf, v, min_pore, full, ctr + 6);
     pkeys = full;
   }
-  assign_ids(c, sdf, pkeys, npk, min_pore, false, ctr + 4);
-  assign_ids(c, sdf, skeys, nskeys, min_solid, true, ctr + 5);
+  assign_ids_bitmap(c, sdf, v, pkeys, npk, min_pore, skeys, nskeys, min_solid, ctr + 4);
   if (seg) launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
   if (roots_out) launch_roots_self(c, sdf, v, roots_out);
   readback(c, ctr + 4, 2, h);
