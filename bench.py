@@ -268,6 +268,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes 5 times (a "
+                         "known-size kernel to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU only (the 512^3 volume is one GPU's block); "
@@ -279,6 +282,14 @@ def main():
     from peclet.pnm import capi
 
     torch.zeros(1, device="cuda:0")  # bring up the GPU context before the OpenMP generators
+    if args.copy_bytes:
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4, "reps": 5}), flush=True)
+        return
     sdf, desc = make_sdf(args.workload, args.size, args.seed)
     shape = sdf.shape
     n = sdf.size

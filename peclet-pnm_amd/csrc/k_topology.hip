@@ -66,21 +66,33 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
     // RB rows into registers (rows past the region re-read a valid row).
     constexpr int NROW = TB_Y * TB_Z, RB = 11, MAXR = (NROW + 3) / 4;
     const bool two = lane == 0;
+    const bool near = ny >= TB_Y && nz >= TB_Z;  // row coordinates lie in [0, 2N)
     for (int j0 = 0; j0 < MAXR; j0 += RB) {
-      int k0[RB], k1[RB];
+      // the sign of the value picks the one array to read: the root of an sdf>0 voxel (flagged
+      // in the top bit) or the solid label
+      int64_t g0[RB], g1[RB];
+      float v0[RB], v1[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         const int row0 = wave + (j0 + j) * 4;
         const int row = row0 < NROW ? row0 : wave;
         const int ry = row % TB_Y, rz = row / TB_Y;
-        const int64_t base =
-            ((int64_t)wrap_any(z0 + rz, nz) * ny + wrap_any(y0 + ry, ny)) * nx;
-        const int64_t g = base + gx0, g1 = base + gx1;
-        const float v = ids.sdf[g], v1 = ids.sdf[g1];
-        const int rt = ids.roots[g], lb = ids.labels[g];
-        const int rt1 = ids.roots[g1], lb1 = ids.labels[g1];
-        k0[j] = v > 0.0f ? (int)((uint32_t)rt | 0x80000000u) : lb;
-        k1[j] = v1 > 0.0f ? (int)((uint32_t)rt1 | 0x80000000u) : lb1;
+        const int wy = near ? wrap_near(y0 + ry, ny) : wrap_any(y0 + ry, ny);
+        const int wz = near ? wrap_near(z0 + rz, nz) : wrap_any(z0 + rz, nz);
+        const int64_t base = ((int64_t)wz * ny + wy) * nx;
+        g0[j] = base + gx0;
+        g1[j] = base + gx1;
+        v0[j] = ids.sdf[g0[j]];
+        v1[j] = ids.sdf[g1[j]];
+      }
+      int k0[RB], k1[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        const int32_t* src0 = v0[j] > 0.0f ? ids.roots : ids.labels;
+        const int32_t* src1 = v1[j] > 0.0f ? ids.roots : ids.labels;
+        const int r0 = src0[g0[j]], r1 = src1[g1[j]];
+        k0[j] = v0[j] > 0.0f ? (int)((uint32_t)r0 | 0x80000000u) : r0;
+        k1[j] = v1[j] > 0.0f ? (int)((uint32_t)r1 | 0x80000000u) : r1;
       }
       int s0[RB], s1[RB];
 #pragma unroll
