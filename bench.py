@@ -40,13 +40,18 @@ HBM_PEAK_GBS = 8000.0
 ALGO_BYTES_PER_VOXEL = {
     "k_tile": 12,         # read sdf 4; write marker label 4, gradient-path root 4
     "k_flood_tile": 12,   # read labels 4, sdf 4; write labels 4
-    "k_seg_topo": 16,     # read sdf 4, root 4, label 4; write seg 4
+    "k_seg_topo": 12,     # read sdf 4, then root (pore) or label (solid) 4; write seg 4
     "k_pre_peaks": 4,     # read sdf 4
     "k_ccl_flatten": 8,   # read + write labels
     "k_seg": 16,          # read sdf 4, roots 4, labels 4; write seg 4
     "k_topo_cand": 4,     # read seg 4
 }
 PIPELINE_BYTES_PER_VOXEL = 64  # S1..S6 compulsory traffic of extract_pore_network
+
+# HBM traffic per launch measured with rocprofv3 counters on this workload (written by
+# --pmc-summary); rocprof kernel symbols differ from the library's timing names for one kernel.
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_extract_pore_network_512_polydisperse.md")
+PROFILER_NAMES = {"k_seg_topo": "k_topo_cand<true>"}
 
 
 def make_sdf(workload, size, seed):
@@ -255,6 +260,97 @@ def bench_surrogate(ctx, size=256, seed=9999):
                                        "frac": round(tflops / 2500.0, 6)}}}
 
 
+def _kernel_symbol(name):
+    """'void pnm::k_topo_cand<true>(int const*, ...)' -> 'k_topo_cand<true>'."""
+    head = name[5:] if name.startswith("void ") else name
+    head = head.split("(", 1)[0]
+    return head.split("::")[-1] if "<" not in head else head.split("<", 1)[0].split("::")[-1] + \
+        "<" + head.split("<", 1)[1]
+
+
+def _counter_rows(path):
+    import csv
+
+    with open(path) as f:
+        return [(_kernel_symbol(r["Kernel_Name"]), float(r["Counter_Value"]))
+                for r in csv.DictReader(f)]
+
+
+def pmc_summary(root, copy_bytes, voxels):
+    """Markdown summary of four rocprofv3 --pmc runs under `root` (pmc_copy_fetch,
+    pmc_copy_write: the calibration kernel; pmc_net_fetch, pmc_net_write: the pipeline), each with
+    run_counter_collection.csv. FETCH_SIZE and WRITE_SIZE are KiB per dispatch; each counter is
+    scaled by (true bytes of the calibration kernel) / (what the counter reads for it)."""
+    from collections import defaultdict
+
+    def calib(name, counter):
+        vals = [v for k, v in _counter_rows(os.path.join(root, name, "run_counter_collection.csv"))
+                if "neg_kernel" in k]
+        if not vals:
+            raise SystemExit(f"no calibration dispatches in {name}")
+        reading = sorted(vals)[len(vals) // 2] * 1024.0
+        return reading, copy_bytes / reading
+
+    rf, kf = calib("pmc_copy_fetch", "FETCH_SIZE")
+    rw, kw = calib("pmc_copy_write", "WRITE_SIZE")
+    per = {}
+    for name, scale, key in (("pmc_net_fetch", kf, "fetch"), ("pmc_net_write", kw, "write")):
+        acc = defaultdict(list)
+        for k, v in _counter_rows(os.path.join(root, name, "run_counter_collection.csv")):
+            acc[k].append(v * 1024.0 * scale)
+        for k, vals in acc.items():
+            per.setdefault(k, {})[key] = vals
+    steps = len(per.get("k_tile", {}).get("fetch", [])) or 1
+    rows = []
+    for k, d in per.items():
+        if k.startswith(("vectorized_elementwise", "reduce_kernel", "distribution", "elementwise")):
+            continue  # torch's own set-up kernels outside the pipeline
+        f, w = d.get("fetch", []), d.get("write", [])
+        rows.append((sum(f) / steps + sum(w) / steps, k, len(f) / steps,
+                     sum(f) / max(1, len(f)), sum(w) / max(1, len(w)),
+                     sum(f) / steps, sum(w) / steps))
+    rows.sort(reverse=True)
+    lines = ["| kernel | launches/step | fetch B/launch | write B/launch | fetch B/voxel/step "
+             "| write B/voxel/step | algorithmic B/voxel |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    inv = {v: k for k, v in PROFILER_NAMES.items()}
+    machine = {}
+    for _, k, nl, fa, wa, fs, ws in rows:
+        alg = ALGO_BYTES_PER_VOXEL.get(inv.get(k, k))
+        lines.append(f"| `{k}` | {nl:g} | {fa:.4g} | {wa:.4g} | {fs / voxels:.3f} | "
+                     f"{ws / voxels:.3f} | {alg if alg is not None else ''} |")
+        machine[k] = {"fetch_bytes": round(fa), "write_bytes": round(wa),
+                      "launches_per_step": nl}
+    tf = sum(r[5] for r in rows) / voxels
+    tw = sum(r[6] for r in rows) / voxels
+    out = ["## Calibration", "",
+           f"Elementwise negation of {copy_bytes} bytes (reads and writes that many bytes):", "",
+           "| counter | reads (KiB) | true (KiB) | scale factor |", "|---|---:|---:|---:|",
+           f"| FETCH_SIZE | {rf / 1024:.1f} | {copy_bytes / 1024:.1f} | {kf:.4f} |",
+           f"| WRITE_SIZE | {rw / 1024:.1f} | {copy_bytes / 1024:.1f} | {kw:.4f} |", "",
+           f"## Pipeline ({steps} passes of extract_pore_network over {voxels} voxels; "
+           "bytes are scaled)", ""] + lines + [
+           "", f"Total per step: fetch {tf:.2f} B/voxel, write {tw:.2f} B/voxel "
+               f"({tf + tw:.2f} B/voxel; {(tf + tw) * voxels / 1e9:.3f} GB).", "",
+           "## Machine-readable (bench.py reads `traffic` from this block)", "", "```json",
+           json.dumps({"voxels": voxels, "fetch_scale": kf, "write_scale": kw,
+                       "kernels": machine}), "```"]
+    return "\n".join(out)
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch (fetch + write) of `kernel` from the committed counter summary, or
+    None when it is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            text = f.read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        d = json.loads(block)["kernels"][PROFILER_NAMES.get(kernel, kernel)]
+        return d["fetch_bytes"] + d["write_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,15 +364,23 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--pmc-summary", default=None, metavar="DIR",
+                    help="print the markdown HBM traffic summary of the rocprofv3 --pmc runs under "
+                         "DIR (pmc_copy_fetch, pmc_copy_write, pmc_net_fetch, pmc_net_write; "
+                         "--copy-bytes gives the calibration size) and exit")
     ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="only run a device-to-device copy of this many bytes 5 times (a "
-                         "known-size kernel to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+                    help="only run an elementwise negation that reads and writes this many bytes, "
+                         "5 times (a known-size copy kernel to calibrate rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU only (the 512^3 volume is one GPU's block); "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary, args.copy_bytes, args.size ** 3))
+        return
     import torch
 
     from peclet.pnm import capi
@@ -286,7 +390,7 @@ def main():
         src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
         for _ in range(5):
-            dst.copy_(src)
+            torch.neg(src, out=dst)  # a shader kernel; copy_ may go to a DMA engine
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 4, "reps": 5}), flush=True)
         return
@@ -334,8 +438,12 @@ def main():
         bpl = ALGO_BYTES_PER_VOXEL[dom] * n
         avg_s = kernels[dom]["avg_ms"] / 1e3
         ach = bpl / avg_s / 1e9
+        measured = (args.workload == "polydisperse" and args.size == 512 and args.seed == 2001)
+        traffic = load_traffic(dom) if measured else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": (os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+                                   if traffic is not None else None),
                 "algorithmic_bytes_per_launch": bpl,
                 "share_of_step": round(kernels[dom]["ms_per_step"] / ms, 3)}
     gvox = n / (ms / 1e3) / 1e9
