@@ -199,8 +199,13 @@ pnm_status pnm_ctx_create(pnm_ctx_t* out, int device) {
       PNM_HIP(hipStreamCreate(&c->own_stream));
       PNM_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->pinned),
                             sizeof(uint64_t) * Ctx::kPinnedWords, hipHostMallocDefault));
+      PNM_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->err_host), 256, hipHostMallocMapped));
+      *c->err_host = 0u;
+      PNM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->err_dev), c->err_host, 0));
       if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
     } catch (...) {
+      if (c->err_host) (void)hipHostFree(c->err_host);
+      if (c->pinned) (void)hipHostFree(c->pinned);
       if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
       delete c;
       throw;
@@ -227,6 +232,9 @@ void pnm_ctx_destroy(pnm_ctx_t ctx) {
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->batch_graph) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(c->batch_graph));
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->err_host) (void)hipHostFree(c->err_host);
+  if (c->lb_state) (void)hipFree(c->lb_state);
+  if (c->lb_ctl) (void)hipFree(c->lb_ctl);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete c;

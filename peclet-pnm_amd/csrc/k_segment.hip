@@ -513,37 +513,42 @@ __global__ void k_mark_first(const float* __restrict__ sdf, const uint32_t* __re
   }
 }
 
-__global__ void k_map_popc(const uint32_t* __restrict__ map, int64_t words,
-                           uint32_t* __restrict__ pc) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w < words) pc[w] = (uint32_t)__popc(map[w]);
+// Rank of bit f in a bitmap with a rank directory (map_rank): the chunk's prefix, the words of
+// the chunk before f's word, the bits below f in its word.
+__device__ __forceinline__ uint32_t map_rank_of(const uint32_t* __restrict__ map,
+                                                const uint32_t* __restrict__ crank, int64_t f) {
+  const int64_t w = f >> 5, c = w / MAP_CHUNK_WORDS;
+  uint32_t r = crank[c];
+  for (int64_t q = c * MAP_CHUNK_WORDS; q < w; ++q) r += (uint32_t)__popc(map[q]);
+  return r + (uint32_t)__popc(map[w] & ((1u << (f & 31)) - 1u));
 }
 
+// Ranks both key sets; the solid map starts at word wpad (a chunk boundary), so its ranks are
+// offset by the number of pore bits, crank[wpad / MAP_CHUNK_WORDS].
 __global__ void k_assign_ranks(const float* __restrict__ sdf, const uint32_t* __restrict__ pkeys,
                                int64_t npk, const uint32_t* __restrict__ skeys, int64_t nsk,
                                int32_t* __restrict__ pid, int32_t* __restrict__ sid,
-                               const uint32_t* __restrict__ pmap,
-                               const uint32_t* __restrict__ prank,
-                               const uint32_t* __restrict__ smap,
-                               const uint32_t* __restrict__ srank, int64_t words,
+                               const uint32_t* __restrict__ map,
+                               const uint32_t* __restrict__ crank, int64_t wpad,
                                unsigned long long* __restrict__ counts) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) {  // totals: rank past the last word
-    counts[0] = (unsigned long long)(prank[words - 1] + __popc(pmap[words - 1]));
-    counts[1] = (unsigned long long)(srank[words - 1] + __popc(smap[words - 1]));
+  const uint32_t npore = crank[wpad / MAP_CHUNK_WORDS];
+  if (j == 0) {  // counts[1] holds the bit total of both maps
+    const unsigned long long all = counts[1];
+    counts[0] = npore;
+    counts[1] = all - npore;
   }
   if (j < npk) {
     const uint32_t r = pkeys[j];
     if (sdf[r] > 0.0f) {
       const uint32_t f = (uint32_t)pid[r];  // still the first-appearance voxel
-      const uint32_t rk = prank[f >> 5] + __popc(pmap[f >> 5] & ((1u << (f & 31)) - 1u));
-      pid[r] = (int32_t)(rk + 1);
+      pid[r] = (int32_t)(map_rank_of(map, crank, f) + 1);
     }
   } else if (j - npk < nsk) {
     const uint32_t r = skeys[j - npk];
     const uint32_t f = (uint32_t)sid[r];
     if (sdf[f] <= 0.0f) {
-      const uint32_t rk = srank[f >> 5] + __popc(smap[f >> 5] & ((1u << (f & 31)) - 1u));
+      const uint32_t rk = map_rank_of(map, crank, (wpad << 5) + f) - npore;
       sid[r] = -(int32_t)(rk + 1);
     } else {
       sid[r] = 0;
@@ -556,25 +561,19 @@ void assign_ids_bitmap(Ctx* c, const float* sdf, Vol v, const uint32_t* pkeys, i
                        int32_t* pore_ids, const uint32_t* skeys, int64_t nsk, int32_t* solid_ids,
                        uint64_t* counts_dev) {
   const int64_t words = (v.n + 31) / 32;
-  uint32_t* pmap = c->arena.get_n<uint32_t>((size_t)words);
-  uint32_t* smap = c->arena.get_n<uint32_t>((size_t)words);
-  uint32_t* pc = c->arena.get_n<uint32_t>((size_t)(2 * words));
-  uint32_t* rank = c->arena.get_n<uint32_t>((size_t)(2 * words));
-  fill_u32(c, pmap, 0u, words);
-  fill_u32(c, smap, 0u, words);
+  const int64_t wpad = (words + MAP_CHUNK_WORDS - 1) / MAP_CHUNK_WORDS * MAP_CHUNK_WORDS;
+  uint32_t* map = c->arena.get_n<uint32_t>((size_t)(2 * wpad));  // pore map | solid map
+  uint32_t* crank = c->arena.get_n<uint32_t>((size_t)(2 * wpad / MAP_CHUNK_WORDS));
+  fill_u32(c, map, 0u, 2 * wpad);
   const int64_t nk = npk + nsk;
   if (nk > 0)
     PNM_LAUNCH(c, "k_mark_first", k_mark_first, dim3(cdiv(nk, 256)), dim3(256), 0, sdf, pkeys,
-               npk, (const int32_t*)pore_ids, skeys, nsk, (const int32_t*)solid_ids, pmap, smap);
-  // popcounts of both maps side by side, one scan (the solid ranks restart at their map start)
-  PNM_LAUNCH(c, "k_map_popc", k_map_popc, dim3(cdiv(words, 256)), dim3(256), 0, pmap, words, pc);
-  PNM_LAUNCH(c, "k_map_popc", k_map_popc, dim3(cdiv(words, 256)), dim3(256), 0, smap, words,
-             pc + words);
-  scan_exclusive_u32(c, pc, rank, words);
-  scan_exclusive_u32(c, pc + words, rank + words, words);
+               npk, (const int32_t*)pore_ids, skeys, nsk, (const int32_t*)solid_ids, map,
+               map + wpad);
+  map_rank(c, map, 2 * wpad, crank, counts_dev + 1);
   PNM_LAUNCH(c, "k_assign_ranks", k_assign_ranks, dim3(cdiv(std::max<int64_t>(nk, 1), 256)),
-             dim3(256), 0, sdf, pkeys, npk, skeys, nsk, pore_ids, solid_ids, pmap, rank, smap,
-             rank + words, words, reinterpret_cast<unsigned long long*>(counts_dev));
+             dim3(256), 0, sdf, pkeys, npk, skeys, nsk, pore_ids, solid_ids, map, crank, wpad,
+             reinterpret_cast<unsigned long long*>(counts_dev));
 }
 
 // relabel_seg (pore_extraction.hpp:378-381).

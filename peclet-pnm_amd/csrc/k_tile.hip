@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
       (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * TILE_Y + wave;
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int plane_n = ny * nx;
-  unsigned npk = 0, nop = 0;
+  unsigned nop = 0;
   auto gidx = [&](int l) {
     return ((z0 + (l >> 9)) * ny + y0 + ((l >> 6) & 7)) * nx + x0 + (l & 63);
   };
@@ -247,15 +247,16 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   }
   (void)last_k;
   __syncthreads();
-  // stage peaks in (slice, lane) order per wave
-  for (int zl = 0; zl < nzt; ++zl) {
-    const int l = (zl * TILE_Y + wave) * 64 + lane;
-    const int i = ((z0 + zl) * ny + y) * nx + x;
-    const uint8_t f = col_in ? DIR[l] : 0;
-    const bool is_peak = f & kPeakBit;
-    const uint64_t pm = __ballot(is_peak);
-    if (is_peak) o.peak_stage[seg_id * T_SEG + npk + __popcll(pm & below)] = (uint32_t)i;
-    npk += (unsigned)__popcll(pm);
+  // peak masks: one word per (row, 64-column chunk), rows in flat order, so the set bits of the
+  // whole map in word order are the peaks in ascending voxel order
+  if (y < ny) {
+    const int64_t nwx = gridDim.x;
+    for (int zl = 0; zl < nzt; ++zl) {
+      const int l = (zl * TILE_Y + wave) * 64 + lane;
+      const bool is_peak = col_in && (DIR[l] & kPeakBit);
+      const uint64_t pm = __ballot(is_peak);
+      if (lane == 0) o.peak_map[((int64_t)(z0 + zl) * ny + y) * nwx + blockIdx.x] = pm;
+    }
   }
 
   // ---- CCL inside the tile: forward rows (y+1,z), (y-1,z+1), (y,z+1), (y+1,z+1) ----
@@ -435,10 +436,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     const int l = (zl * TILE_Y + wave) * 64 + lane;
     if ((DIR[l] & kDirMask) == 13) o.min_pore[gidx(l)] = gidx(LAB[l]);
   }
-  if (lane == 0) {
-    o.peak_cnt[seg_id] = npk;
-    o.open_cnt[seg_id] = nop;
-  }
+  if (lane == 0) o.open_cnt[seg_id] = nop;
 }
 
 // ---- cross-tile and wrapping marker relations ----

@@ -35,6 +35,8 @@ void readback(Ctx* c, const void* dev, int words, uint64_t* host_out) {
                          c->stream));
   PNM_HIP(hipStreamSynchronize(c->stream));
   std::memcpy(host_out, c->pinned, sizeof(uint64_t) * words);
+  if (c->err_host && *c->err_host)
+    fail(PNM_EHIP, "internal error: a scan's look-back wait timed out");
 }
 
 // ---------------- exclusive scan (uint32) ----------------
@@ -71,25 +73,251 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
   return off + inc - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in,
-                                                              int64_t n,
-                                                              uint32_t* __restrict__ sums) {
-  __shared__ uint32_t lds[SCAN_THREADS / 64];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k)
-    if (base + k < n) s += in[base + k];
-  uint32_t tot;
-  block_excl_scan(s, lds, &tot);
-  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+// Single-launch exclusive scan with decoupled look-back. Each block publishes its aggregate, then
+// its inclusive prefix, in one 64-bit status word per block: value in the low half, (epoch << 2 |
+// state) in the high half, so a reader gets value and state in one load and words left from an
+// earlier call (another epoch) read as "not ready" — nothing is cleared between calls.
+// Block order comes from a ticket taken when the block starts, so a block only waits for blocks
+// that are already running and never for one that waits on it; the wait is still bounded (a
+// timed-out wait sets the context's error word, which the next readback reports).
+// The block that finishes last resets the ticket and done counters and advances the epoch for the
+// next call on the same stream.
+constexpr uint32_t LB_AGG = 1, LB_INC = 2;
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 18;
+
+__device__ __forceinline__ uint64_t lb_word(uint32_t epoch, uint32_t st, uint32_t v) {
+  return ((uint64_t)((epoch << 2) | st) << 32) | v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* in, int64_t n,
-                                                            const uint32_t* __restrict__ offs,
-                                                            uint32_t* out) {
+// Thread 0 takes the block's ticket and reads the epoch into LDS; every thread returns after
+// the barrier.
+__device__ __forceinline__ void lb_begin(uint32_t* ctl, uint32_t& bid, uint32_t& epoch,
+                                         uint32_t* s_bid, uint32_t* s_epoch) {
+  if (threadIdx.x == 0) {
+    *s_bid = atomicAdd(ctl + 0, 1u);
+    *s_epoch = __hip_atomic_load(ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  bid = *s_bid;
+  epoch = *s_epoch;
+}
+
+// Wave 0 (all 64 lanes): publishes the block total `tot`, looks back over its predecessors and
+// publishes the inclusive prefix; returns the block's exclusive prefix (in every lane).
+__device__ __forceinline__ uint32_t lb_lookback(uint64_t* state, uint32_t bid, uint32_t epoch,
+                                                uint32_t tot, uint32_t* err) {
+  const int lane = threadIdx.x & 63;
+  if (bid == 0) {
+    if (lane == 0)
+      __hip_atomic_store(state, lb_word(epoch, LB_INC, tot), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    return 0u;
+  }
+  if (lane == 0)
+    __hip_atomic_store(state + bid, lb_word(epoch, LB_AGG, tot), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t excl = 0;
+  int64_t j = (int64_t)bid - 1;  // nearest predecessor not yet added
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t idx = j - lane;
+    uint32_t st = LB_INC, v = 0;  // before block 0: an inclusive zero
+    if (idx >= 0) {
+      const uint64_t w = __hip_atomic_load(state + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t hi = (uint32_t)(w >> 32);
+      st = (hi >> 2) == epoch ? (hi & 3u) : 0u;
+      v = (uint32_t)w;
+    }
+    const uint64_t notready = __ballot(st == 0u);
+    const uint64_t incl = __ballot(st == LB_INC);
+    // lanes up to and including the nearest inclusive prefix (all 64 if there is none)
+    const uint64_t upto = incl ? (((incl & (0ull - incl)) << 1) - 1ull) : ~0ull;
+    if (notready & upto) {
+      if (++spins > LB_SPIN_LIMIT) {
+        if (lane == 0) *(volatile uint32_t*)err = 1u;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t add = ((upto >> lane) & 1ull) ? v : 0u;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) add += __shfl_xor(add, o, 64);
+    excl += add;
+    if (incl) break;
+    j -= 64;
+  }
+  if (lane == 0)
+    __hip_atomic_store(state + bid, lb_word(epoch, LB_INC, excl + tot), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// Thread 0 of every block, at its very end: the block that finishes last resets the ticket and
+// done counters and advances the epoch for the next call on the same stream.
+__device__ __forceinline__ void lb_end(uint32_t* ctl, uint32_t epoch, unsigned nblocks) {
+  const uint32_t d = atomicAdd(ctl + 1, 1u);
+  if (d == nblocks - 1) {
+    const uint32_t ne = (epoch + 1u) & 0x3fffffffu;
+    __hip_atomic_store(ctl + 0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ctl + 2, ne ? ne : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ITEMS consecutive values of one thread; 16-byte vector accesses when the run is whole and
+// aligned (only a tail block takes the scalar path).
+template <int ITEMS, class T>
+__device__ __forceinline__ void load_run(const T* in, int64_t base, int64_t n, T (&v)[ITEMS]) {
+  constexpr int PER = 16 / sizeof(T);
+  if (base + ITEMS <= n && ((reinterpret_cast<uintptr_t>(in + base) & 15u) == 0)) {
+    const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int q = 0; q < ITEMS / PER; ++q) {
+      const uint4 t = p[q];
+      T u[PER];
+      __builtin_memcpy(u, &t, 16);
+#pragma unroll
+      for (int e = 0; e < PER; ++e) v[q * PER + e] = u[e];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) v[k] = (base + k < n) ? in[base + k] : (T)0;
+  }
+}
+
+// Exclusive scan of in[0..n) (in == out allowed): per-thread runs of ITEMS values, block scan,
+// look-back across blocks; the last block writes the total.
+template <int ITEMS>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_lookback(
+    const uint32_t* in, int64_t n, uint32_t* out, uint64_t* __restrict__ total_out,
+    uint32_t* __restrict__ ctl, uint64_t* __restrict__ state, unsigned nblocks,
+    uint32_t* __restrict__ err) {
   __shared__ uint32_t lds[SCAN_THREADS / 64];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  __shared__ uint32_t s_bid, s_epoch, s_excl;
+  uint32_t bid, epoch;
+  lb_begin(ctl, bid, epoch, &s_bid, &s_epoch);
+  const int64_t base = ((int64_t)bid * SCAN_THREADS + threadIdx.x) * ITEMS;
+  uint32_t vals[ITEMS];
+  load_run<ITEMS>(in, base, n, vals);
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) s += vals[k];
+  uint32_t tot;
+  const uint32_t pre = block_excl_scan(s, lds, &tot);
+  if (threadIdx.x < 64) {
+    const uint32_t excl = lb_lookback(state, bid, epoch, tot, err);
+    if (threadIdx.x == 0) s_excl = excl;
+  }
+  __syncthreads();
+  uint32_t run = s_excl + pre;
+  if (base + ITEMS <= n && ((reinterpret_cast<uintptr_t>(out + base) & 15u) == 0)) {
+    uint4* p = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+    for (int q = 0; q < ITEMS / 4; ++q) {
+      uint4 t;
+      t.x = run; run += vals[4 * q];
+      t.y = run; run += vals[4 * q + 1];
+      t.z = run; run += vals[4 * q + 2];
+      t.w = run; run += vals[4 * q + 3];
+      p[q] = t;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      if (base + k < n) out[base + k] = run;
+      run += vals[k];
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (total_out && bid == nblocks - 1) *total_out = (uint64_t)s_excl + tot;
+    lb_end(ctl, epoch, nblocks);
+  }
+}
+
+// Sorted index list of the set bits of a row-padded bitmap: word w covers voxels
+// (w / nwx) * nx + (w % nwx) * 64 + bit, so word order is flat-index order. One launch: each
+// thread popcounts ITEMS consecutive words, the look-back scan gives its rank, and it writes the
+// indices of its set bits in order.
+template <int ITEMS>
+__global__ __launch_bounds__(SCAN_THREADS) void k_bitmap_list(
+    const uint64_t* __restrict__ map, int64_t nwords, int64_t nwx, int64_t nx,
+    uint32_t* __restrict__ out, uint64_t* __restrict__ total_out, uint32_t* __restrict__ ctl,
+    uint64_t* __restrict__ state, unsigned nblocks, uint32_t* __restrict__ err) {
+  __shared__ uint32_t lds[SCAN_THREADS / 64];
+  __shared__ uint32_t s_bid, s_epoch, s_excl;
+  uint32_t bid, epoch;
+  lb_begin(ctl, bid, epoch, &s_bid, &s_epoch);
+  const int64_t base = ((int64_t)bid * SCAN_THREADS + threadIdx.x) * ITEMS;
+  uint64_t w[ITEMS];
+  load_run<ITEMS>(map, base, nwords, w);
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) s += (uint32_t)__popcll(w[k]);
+  uint32_t tot;
+  const uint32_t pre = block_excl_scan(s, lds, &tot);
+  if (threadIdx.x < 64) {
+    const uint32_t excl = lb_lookback(state, bid, epoch, tot, err);
+    if (threadIdx.x == 0) s_excl = excl;
+  }
+  __syncthreads();
+  uint32_t r = s_excl + pre;
+  for (int k = 0; k < ITEMS; ++k) {
+    uint64_t m = w[k];
+    if (!m) continue;
+    const int64_t wi = base + k;
+    const int64_t row = wi / nwx;
+    const uint32_t rb = (uint32_t)(row * nx + (wi - row * nwx) * 64);
+    while (m) {
+      out[r++] = rb + (uint32_t)(__ffsll((unsigned long long)m) - 1);
+      m &= m - 1;
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (total_out && bid == nblocks - 1) *total_out = (uint64_t)s_excl + tot;
+    lb_end(ctl, epoch, nblocks);
+  }
+}
+
+// Rank directory of a uint32 bitmap: chunk_rank[c] = number of set bits in words [0, 32c).
+// The rank of bit f is then chunk_rank[f >> 10] + the popcounts of the words of its chunk
+// before word f >> 5 + the bits below f in that word.
+__global__ __launch_bounds__(SCAN_THREADS) void k_map_rank(
+    const uint32_t* __restrict__ map, int64_t words, uint32_t* __restrict__ chunk_rank,
+    uint64_t* __restrict__ total_out, uint32_t* __restrict__ ctl, uint64_t* __restrict__ state,
+    unsigned nblocks, uint32_t* __restrict__ err) {
+  constexpr int ITEMS = MAP_CHUNK_WORDS;
+  __shared__ uint32_t lds[SCAN_THREADS / 64];
+  __shared__ uint32_t s_bid, s_epoch, s_excl;
+  uint32_t bid, epoch;
+  lb_begin(ctl, bid, epoch, &s_bid, &s_epoch);
+  const int64_t chunk = (int64_t)bid * SCAN_THREADS + threadIdx.x;
+  const int64_t base = chunk * ITEMS;
+  uint32_t w[ITEMS];
+  load_run<ITEMS>(map, base, words, w);
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) s += (uint32_t)__popc(w[k]);
+  uint32_t tot;
+  const uint32_t pre = block_excl_scan(s, lds, &tot);
+  if (threadIdx.x < 64) {
+    const uint32_t excl = lb_lookback(state, bid, epoch, tot, err);
+    if (threadIdx.x == 0) s_excl = excl;
+  }
+  __syncthreads();
+  if (base < words) chunk_rank[chunk] = s_excl + pre;
+  if (threadIdx.x == 0) {
+    if (total_out && bid == nblocks - 1) *total_out = (uint64_t)s_excl + tot;
+    lb_end(ctl, epoch, nblocks);
+  }
+}
+
+// Single block: scan of up to SCAN_TILE values, optional total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_single(const uint32_t* in, int64_t n,
+                                                              uint32_t* out,
+                                                              uint64_t* __restrict__ total_out) {
+  __shared__ uint32_t lds[SCAN_THREADS / 64];
+  const int64_t base = (int64_t)threadIdx.x * SCAN_ITEMS;
   uint32_t vals[SCAN_ITEMS];
   uint32_t s = 0;
 #pragma unroll
@@ -98,28 +326,97 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* in, 
     s += vals[k];
   }
   uint32_t tot;
-  uint32_t run = block_excl_scan(s, lds, &tot) + (offs ? offs[blockIdx.x] : 0u);
+  uint32_t run = block_excl_scan(s, lds, &tot);
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) out[base + k] = run;
     run += vals[k];
   }
+  if (total_out && threadIdx.x == 0) *total_out = tot;
 }
 
-void scan_exclusive_u32(Ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
-  if (n <= 0) return;
-  const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  if (nb == 1) {
-    PNM_LAUNCH(c, "k_scan_down", k_scan_down, dim3(1), dim3(SCAN_THREADS), 0, in, n,
-                       (const uint32_t*)nullptr, out);
+static void lb_reserve(Ctx* c, int64_t nb) {
+  if (!c->lb_ctl) {
+    PNM_HIP(hipMalloc(reinterpret_cast<void**>(&c->lb_ctl), 256));
+    const uint32_t init[4] = {0u, 0u, 1u, 0u};  // ticket, done, epoch
+    PNM_HIP(hipMemcpyAsync(c->lb_ctl, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    PNM_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (nb > 0x7fffffff) fail(PNM_EINVAL, "scan: too many blocks");
+  if (nb <= c->lb_cap) return;
+  const int64_t cap = std::max<int64_t>(nb, std::max<int64_t>(4096, 2 * c->lb_cap));
+  if (c->lb_state) {
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    PNM_HIP(hipFree(c->lb_state));
+    c->lb_state = nullptr;
+    c->lb_cap = 0;
+  }
+  PNM_HIP(hipMalloc(reinterpret_cast<void**>(&c->lb_state), sizeof(uint64_t) * cap));
+  // epoch 0 is never used, so zeroed words read as "not ready"
+  PNM_HIP(hipMemsetAsync(c->lb_state, 0, sizeof(uint64_t) * cap, c->stream));
+  PNM_HIP(hipStreamSynchronize(c->stream));
+  c->lb_cap = cap;
+}
+
+// Long inputs take 32 values per thread, so that fewer blocks are chained by the look-back.
+constexpr int64_t LB_LONG = (int64_t)SCAN_THREADS * 8 * 256;
+
+void bitmap_list(Ctx* c, const uint64_t* map, int64_t nwords, int64_t nwx, int64_t nx,
+                 uint32_t* out, uint64_t* total_dev) {
+  if (nwords <= 0) {
+    fill_u64(c, total_dev, 0, 1);
     return;
   }
-  uint32_t* sums = c->arena.get_n<uint32_t>((size_t)nb);
-  PNM_LAUNCH(c, "k_scan_reduce", k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, in, n,
-                     sums);
-  scan_exclusive_u32(c, sums, sums, nb);
-  PNM_LAUNCH(c, "k_scan_down", k_scan_down, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, in, n,
-                     (const uint32_t*)sums, out);
+  if (nwords > LB_LONG) {
+    const int64_t nb = cdiv(nwords, SCAN_THREADS * 32);
+    lb_reserve(c, nb);
+    PNM_LAUNCH(c, "k_bitmap_list", k_bitmap_list<32>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0,
+               map, nwords, nwx, nx, out, total_dev, c->lb_ctl, c->lb_state, (unsigned)nb,
+               c->err_dev);
+  } else {
+    const int64_t nb = cdiv(nwords, SCAN_THREADS * 8);
+    lb_reserve(c, nb);
+    PNM_LAUNCH(c, "k_bitmap_list", k_bitmap_list<8>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0,
+               map, nwords, nwx, nx, out, total_dev, c->lb_ctl, c->lb_state, (unsigned)nb,
+               c->err_dev);
+  }
+}
+
+void map_rank(Ctx* c, const uint32_t* map, int64_t words, uint32_t* chunk_rank,
+              uint64_t* total_dev) {
+  if (words <= 0) {
+    if (total_dev) fill_u64(c, total_dev, 0, 1);
+    return;
+  }
+  const int64_t nb = cdiv(words, (int64_t)SCAN_THREADS * MAP_CHUNK_WORDS);
+  lb_reserve(c, nb);
+  PNM_LAUNCH(c, "k_map_rank", k_map_rank, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, map, words,
+             chunk_rank, total_dev, c->lb_ctl, c->lb_state, (unsigned)nb, c->err_dev);
+}
+
+void scan_exclusive_u32(Ctx* c, const uint32_t* in, uint32_t* out, int64_t n,
+                        uint64_t* total_dev) {
+  if (n <= 0) {
+    if (total_dev) fill_u64(c, total_dev, 0, 1);
+    return;
+  }
+  if (n <= SCAN_TILE) {
+    PNM_LAUNCH(c, "k_scan_single", k_scan_single, dim3(1), dim3(SCAN_THREADS), 0, in, n, out,
+               total_dev);
+    return;
+  }
+  if (n > LB_LONG) {
+    const int64_t nb = cdiv(n, SCAN_THREADS * 32);
+    lb_reserve(c, nb);
+    PNM_LAUNCH(c, "k_scan_lookback", k_scan_lookback<32>, dim3((unsigned)nb),
+               dim3(SCAN_THREADS), 0, in, n, out, total_dev, c->lb_ctl, c->lb_state,
+               (unsigned)nb, c->err_dev);
+  } else {
+    const int64_t nb = cdiv(n, SCAN_THREADS * 8);
+    lb_reserve(c, nb);
+    PNM_LAUNCH(c, "k_scan_lookback", k_scan_lookback<8>, dim3((unsigned)nb), dim3(SCAN_THREADS),
+               0, in, n, out, total_dev, c->lb_ctl, c->lb_state, (unsigned)nb, c->err_dev);
+  }
 }
 
 // ---------------- stable LSD radix sort ----------------
@@ -261,12 +558,6 @@ __global__ void k_gather_staged64(const uint64_t* __restrict__ stage,
   for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) out[base + k] = stage[b * cap + k];
 }
 
-__global__ void k_staged_total(const uint32_t* __restrict__ counts,
-                               const uint32_t* __restrict__ offs, int64_t nblocks,
-                               uint64_t* __restrict__ total) {
-  *total = (uint64_t)offs[nblocks - 1] + counts[nblocks - 1];
-}
-
 void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64_t nblocks,
                     int64_t cap, uint32_t* out, uint64_t* total_dev) {
   if (nblocks <= 0) {
@@ -274,16 +565,15 @@ void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64
     return;
   }
   uint32_t* offs = c->arena.get_n<uint32_t>((size_t)nblocks);
-  scan_exclusive_u32(c, counts, offs, nblocks);
+  scan_exclusive_u32(c, counts, offs, nblocks, total_dev);
   PNM_LAUNCH(c, "k_gather_staged", k_gather_staged, dim3((unsigned)nblocks), dim3(64), 0, stage,
              counts, offs, cap, out);
-  PNM_LAUNCH(c, "k_staged_total", k_staged_total, dim3(1), dim3(1), 0, counts, offs, nblocks,
-             total_dev);
 }
 
 // Several staged lists with the same segment layout in one pass: their counts are contiguous
 // (list k at counts + k * nseg), one exclusive scan covers all of them, one gather launch moves
-// every segment to its list (offsets relative to the list's first segment).
+// every segment to its list (offsets relative to the list's first segment); the block of a
+// list's last segment writes the list's total.
 struct MultiLists {
   const uint32_t* stage[4];
   uint32_t* out[4];
@@ -299,13 +589,7 @@ __global__ void k_gather_multi(MultiLists L, const uint32_t* __restrict__ counts
   const uint32_t* __restrict__ st = L.stage[k] + sgm * cap;
   uint32_t* __restrict__ o = L.out[k];
   for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) o[base + j] = st[j];
-}
-__global__ void k_multi_totals(MultiLists L, const uint32_t* __restrict__ counts,
-                               const uint32_t* __restrict__ offs, int64_t nseg, int K) {
-  const int k = threadIdx.x;
-  if (k >= K) return;
-  const int64_t last = (int64_t)(k + 1) * nseg - 1;
-  *L.total[k] = (uint64_t)(offs[last] + counts[last] - offs[(int64_t)k * nseg]);
+  if (sgm == nseg - 1 && threadIdx.x == 0) *L.total[k] = (uint64_t)(base + cnt);
 }
 
 void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uint32_t* counts,
@@ -325,7 +609,6 @@ void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uin
   scan_exclusive_u32(c, counts, offs, K * nseg);
   PNM_LAUNCH(c, "k_gather_staged", k_gather_multi, dim3((unsigned)(K * nseg)), dim3(64), 0, L,
              counts, offs, nseg, cap);
-  PNM_LAUNCH(c, "k_staged_total", k_multi_totals, dim3(1), dim3(64), 0, L, counts, offs, nseg, K);
 }
 
 void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
@@ -335,11 +618,9 @@ void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int
     return;
   }
   uint32_t* offs = c->arena.get_n<uint32_t>((size_t)nblocks);
-  scan_exclusive_u32(c, counts, offs, nblocks);
+  scan_exclusive_u32(c, counts, offs, nblocks, total_dev);
   PNM_LAUNCH(c, "k_gather_staged", k_gather_staged64, dim3((unsigned)nblocks), dim3(64), 0,
              stage, counts, offs, cap, out);
-  PNM_LAUNCH(c, "k_staged_total", k_staged_total, dim3(1), dim3(1), 0, counts, offs, nblocks,
-             total_dev);
 }
 
 // ---------------- unique of a sorted list ----------------

@@ -20,14 +20,16 @@ static int nbits_u64(uint64_t v) {
   return b;
 }
 
-// Sort the peak list by voxel index and refine the first min(total, PNM_MAX_PORES) peaks.
+// Sort the peak list by voxel index (unless it already is) and refine the first
+// min(total, PNM_MAX_PORES) peaks.
 static int64_t finish_pores(Ctx* c, const float* sdf, Vol v, const float org[3],
-                            const float spc[3], uint32_t* peaks, int64_t npeaks,
+                            const float spc[3], uint32_t* peaks, int64_t npeaks, bool sorted,
                             pnm_pore_t* out_d, int64_t cap, int64_t* total) {
   *total = npeaks;
   const int64_t take = std::min<int64_t>(npeaks, PNM_MAX_PORES);
   if (take > cap) return -take;  // capacity error, caller reports
-  if (npeaks > 1) radix_sort_u32(c, peaks, nullptr, npeaks, 0, nbits_u64((uint64_t)(v.n - 1)));
+  if (!sorted && npeaks > 1)
+    radix_sort_u32(c, peaks, nullptr, npeaks, 0, nbits_u64((uint64_t)(v.n - 1)));
   launch_centroids(c, sdf, v, org, spc, peaks, take, out_d);
   return take;
 }
@@ -46,12 +48,12 @@ int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const 
   compact_staged(c, stage, cnt, nb, scap, peaks, ctr);
   uint64_t h[1];
   readback(c, ctr, 1, h);
-  return finish_pores(c, sdf, v, org, spc, peaks, (int64_t)h[0], out_d, cap, total);
+  return finish_pores(c, sdf, v, org, spc, peaks, (int64_t)h[0], false, out_d, cap, total);
 }
 
 struct SegInternal {
   SegResult r;
-  uint32_t* peaks;  // unsorted peak list (extract_pores predicate)
+  uint32_t* peaks;  // peak list (extract_pores predicate) in ascending voxel order
   int64_t npeaks;
   // inputs of relabel_seg, for a caller that fuses it (seg_topology)
   const int32_t* roots;
@@ -79,15 +81,12 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   // staging: the tile pass fills nseg segments of scap slots; flatten and the flood use nb1
   // blocks of 256
   const int64_t stage_words = std::max(nseg * scap, nb1 * 256);
-  uint32_t* stage_a = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_c = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
-  // counts: [peaks | open] from the tile pass, [solid keys | band] from the flood tile, each
-  // pair contiguous for one multi-list compaction
-  uint32_t* cnt_pair1 = c->arena.get_n<uint32_t>((size_t)(2 * nseg));
+  // counts: open walks from the tile pass; [solid keys | band] from the flood tile, contiguous
+  // for one multi-list compaction
+  uint32_t* cnt_c = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* cnt_pair2 = c->arena.get_n<uint32_t>((size_t)(2 * nseg));
-  uint32_t* cnt_a = cnt_pair1;
-  uint32_t* cnt_c = cnt_pair1 + nseg;
   uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
   uint32_t* stage_r = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
   uint32_t* cnt_r = c->arena.get_n<uint32_t>((size_t)nseg);
@@ -109,20 +108,17 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   o.walk = walk;
   o.min_pore = min_pore;
   o.min_solid = min_solid;
-  o.peak_stage = stage_a;
-  o.peak_cnt = cnt_a;
+  const int64_t nwx = cdiv(v.nx, 64), nwords = nwx * v.ny * v.nz;
+  uint64_t* peak_map = c->arena.get_n<uint64_t>((size_t)nwords);
+  o.peak_map = peak_map;
   o.lroot_stage = stage_r;
   o.lroot_cnt = cnt_r;
   o.open_stage = stage_c;
   o.open_cnt = cnt_c;
   o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
   launch_tile(c, sdf, v, thr, o);
-  {
-    const uint32_t* st[2] = {stage_a, stage_c};
-    uint32_t* outs[2] = {peaks, open};
-    uint64_t* tots[2] = {ctr + 0, ctr + 1};
-    compact_staged_multi(c, 2, st, cnt_pair1, nseg, scap, outs, tots);
-  }
+  bitmap_list(c, peak_map, nwords, nwx, v.nx, peaks, ctr + 0);  // peaks in voxel order
+  compact_staged(c, stage_c, cnt_c, nseg, scap, open, ctr + 1);
   launch_ccl_seams(c, v, lab_ccl);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
   if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
@@ -219,8 +215,8 @@ int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], cons
                        int32_t* seg, int32_t* pairs_d, int64_t cap_pairs, int64_t* npairs,
                        bool* pairs_overflow) {
   SegInternal s = segment_core(c, sdf, v, spc, nullptr, nullptr);
-  const int64_t np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, pores_d, cap_pores,
-                                  total_pores);
+  const int64_t np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, true, pores_d,
+                                  cap_pores, total_pores);
   int64_t needed = 0;
   const int64_t nt = seg_topology(c, sdf, s.roots, s.labels, s.pore_id, s.solid_id, seg, v,
                                   -s.r.n_solids, s.r.n_pores, pairs_d, cap_pairs, &needed);

@@ -55,6 +55,14 @@ struct Ctx {
   Arena arena;
   uint64_t* pinned = nullptr;  // hipHostMalloc'ed readback words
   static constexpr int kPinnedWords = 64;
+  // Error word in mapped pinned memory: kernels set it on an internal failure (a timed-out
+  // wait); readback() reports it after every synchronisation.
+  uint32_t* err_host = nullptr;
+  uint32_t* err_dev = nullptr;
+  // Look-back scan state: per-block status words (grow-only) and [ticket, done, epoch].
+  uint64_t* lb_state = nullptr;
+  int64_t lb_cap = 0;
+  uint32_t* lb_ctl = nullptr;
   // Kernel timing: when `prof` is set every launch is bracketed by two HIP events on the
   // launch stream; pnm_ctx_profile_read aggregates them per kernel name.
   bool prof = false;
@@ -112,7 +120,19 @@ void fill_u64(Ctx* c, uint64_t* p, uint64_t v, int64_t count);
 // Copies `words` device uint64 counters to pinned memory and synchronises the stream.
 void readback(Ctx* c, const void* dev, int words, uint64_t* host_out);
 // Exclusive prefix sum of uint32 (n up to 2^31); returns nothing, out may alias in.
-void scan_exclusive_u32(Ctx* c, const uint32_t* in, uint32_t* out, int64_t n);
+// Sorted flat indices of the set bits of a row-padded bitmap (word w: voxels (w / nwx) * nx +
+// (w % nwx) * 64 + bit) and their count (uint64 at total_dev), in one launch.
+void bitmap_list(Ctx* c, const uint64_t* map, int64_t nwords, int64_t nwx, int64_t nx,
+                 uint32_t* out, uint64_t* total_dev);
+// Rank directory of a uint32 bitmap: chunk_rank[c] = set bits in words [0, 32 c) (one entry per
+// MAP_CHUNK_WORDS = 32 words), total at total_dev; one launch.
+constexpr int MAP_CHUNK_WORDS = 32;
+void map_rank(Ctx* c, const uint32_t* map, int64_t words, uint32_t* chunk_rank,
+              uint64_t* total_dev);
+// Exclusive scan of n uint32 values (in == out allowed); the total, when total_dev is given, is
+// written there as uint64.
+void scan_exclusive_u32(Ctx* c, const uint32_t* in, uint32_t* out, int64_t n,
+                        uint64_t* total_dev = nullptr);
 // Stable LSD radix sort by keys[begin_bit, end_bit); vals optional (nullptr). Result lands in
 // keys/vals (the alt buffers are scratch).
 void radix_sort_u64(Ctx* c, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
@@ -159,8 +179,7 @@ struct TileOut {
   uint64_t* walk;         // walk records: sdf>0 voxels on tile boundaries or left open [n]
   int32_t* min_pore;      // first-appearance minimum over the tile, at in-tile roots   [n]
   int32_t* min_solid;     // seeded with its own index at every tile-local marker root  [n]
-  uint32_t* peak_stage;   // staged peaks (extract_pores predicate): tile_segment_cap() per segment
-  uint32_t* peak_cnt;
+  uint64_t* peak_map;     // extract_pores peaks: bit x%64 of word (z*ny + y)*cdiv(nx,64) + x/64
   uint32_t* lroot_stage;  // staged tile-local marker roots
   uint32_t* lroot_cnt;
   uint32_t* open_stage;   // staged sdf>0 voxels whose walk left the tile
