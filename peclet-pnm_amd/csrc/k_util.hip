@@ -536,66 +536,86 @@ void radix_sort_u32(Ctx* c, uint32_t* keys, uint32_t* vals, int64_t n, int begin
 }
 
 // ---------------- block-staged compaction ----------------
-// Producer kernels write the k-th item of block b to stage[b * cap + k] and the item count to
-// counts[b]; this gathers them in block order (deterministic, no contended atomics).
-__global__ void k_gather_staged(const uint32_t* __restrict__ stage,
-                                const uint32_t* __restrict__ counts,
-                                const uint32_t* __restrict__ offs, int64_t cap,
-                                uint32_t* __restrict__ out) {
-  const int64_t b = blockIdx.x;
-  const uint32_t cnt = counts[b];
-  const uint32_t base = offs[b];
-  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) out[base + k] = stage[b * cap + k];
-}
+// Producer kernels write the k-th item of segment b to stage[b * cap + k] and the item count to
+// counts[b]; this gathers them in segment order (deterministic, no contended atomics). Several
+// lists with the same segment layout go in one pass: their counts are contiguous (list k at
+// counts + k * nseg), one exclusive scan covers all of them, one gather launch moves every
+// segment to its list (offsets relative to the list's first segment).
+//
+// One wave per GATHER_SEGS consecutive segments of one list: the lanes read the segments' counts
+// and offsets together, then the wave walks the segments' 64-item chunks in batches of
+// GATHER_BATCH, issuing every load of a batch (clamped, in-bounds addresses) before its stores,
+// so a batch costs one memory round trip. The wave holding a list's last segment writes the
+// list's total.
+constexpr int GATHER_SEGS = 16;
+constexpr int GATHER_BATCH = 8;
 
-__global__ void k_gather_staged64(const uint64_t* __restrict__ stage,
-                                  const uint32_t* __restrict__ counts,
-                                  const uint32_t* __restrict__ offs, int64_t cap,
-                                  uint64_t* __restrict__ out) {
-  const int64_t b = blockIdx.x;
-  const uint32_t cnt = counts[b];
-  const uint32_t base = offs[b];
-  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) out[base + k] = stage[b * cap + k];
-}
-
-void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64_t nblocks,
-                    int64_t cap, uint32_t* out, uint64_t* total_dev) {
-  if (nblocks <= 0) {
-    fill_u64(c, total_dev, 0, 1);
-    return;
-  }
-  uint32_t* offs = c->arena.get_n<uint32_t>((size_t)nblocks);
-  scan_exclusive_u32(c, counts, offs, nblocks, total_dev);
-  PNM_LAUNCH(c, "k_gather_staged", k_gather_staged, dim3((unsigned)nblocks), dim3(64), 0, stage,
-             counts, offs, cap, out);
-}
-
-// Several staged lists with the same segment layout in one pass: their counts are contiguous
-// (list k at counts + k * nseg), one exclusive scan covers all of them, one gather launch moves
-// every segment to its list (offsets relative to the list's first segment); the block of a
-// list's last segment writes the list's total.
-struct MultiLists {
-  const uint32_t* stage[4];
-  uint32_t* out[4];
+template <class T>
+struct GatherLists {
+  const T* stage[4];
+  T* out[4];
   uint64_t* total[4];
 };
-__global__ void k_gather_multi(MultiLists L, const uint32_t* __restrict__ counts,
-                               const uint32_t* __restrict__ offs, int64_t nseg, int64_t cap) {
-  const int64_t b = blockIdx.x;
-  const int k = (int)(b / nseg);
-  const int64_t sgm = b - (int64_t)k * nseg;
-  const uint32_t cnt = counts[b];
-  const uint32_t base = offs[b] - offs[(int64_t)k * nseg];
-  const uint32_t* __restrict__ st = L.stage[k] + sgm * cap;
-  uint32_t* __restrict__ o = L.out[k];
-  for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) o[base + j] = st[j];
-  if (sgm == nseg - 1 && threadIdx.x == 0) *L.total[k] = (uint64_t)(base + cnt);
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_lists(GatherLists<T> L,
+                                                      const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ offs,
+                                                      int64_t nseg, int64_t cap,
+                                                      int64_t waves_per_list, int K) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wv = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = (int)(wv / waves_per_list);
+  if (k >= K) return;  // wave-uniform
+  const int64_t s0 = (wv - (int64_t)k * waves_per_list) * GATHER_SEGS;  // first segment in list
+  const int64_t cb = (int64_t)k * nseg;
+  const int nq = (int)(nseg - s0 < GATHER_SEGS ? nseg - s0 : GATHER_SEGS);
+  const T* __restrict__ st = L.stage[k];
+  T* __restrict__ o = L.out[k];
+  uint32_t c = 0, of = 0;
+  if (lane < nq) {
+    c = counts[cb + s0 + lane];
+    of = offs[cb + s0 + lane] - offs[cb];
+  }
+  if (s0 + nq == nseg) {  // this wave holds the list's last segment
+    const uint32_t cl = (uint32_t)__builtin_amdgcn_readlane((int)c, nq - 1);
+    const uint32_t ol = (uint32_t)__builtin_amdgcn_readlane((int)of, nq - 1);
+    if (lane == 0) *L.total[k] = (uint64_t)ol + cl;
+  }
+  int q = 0;
+  uint32_t j = 0;  // next item of segment q (wave-uniform)
+  while (true) {
+    T v[GATHER_BATCH];
+    uint32_t dst[GATHER_BATCH];
+    bool ok[GATHER_BATCH];
+#pragma unroll
+    for (int bt = 0; bt < GATHER_BATCH; ++bt) {
+      while (q < nq && j >= (uint32_t)__builtin_amdgcn_readlane((int)c, q)) {
+        ++q;
+        j = 0;
+      }
+      const int qq = q < nq ? q : nq - 1;
+      const uint32_t cq = q < nq ? (uint32_t)__builtin_amdgcn_readlane((int)c, qq) : 0u;
+      const uint32_t oq = (uint32_t)__builtin_amdgcn_readlane((int)of, qq);
+      const uint32_t idx = j + (uint32_t)lane;
+      ok[bt] = idx < cq;
+      dst[bt] = oq + idx;
+      v[bt] = st[(s0 + qq) * cap + (idx < (uint32_t)cap ? idx : (uint32_t)cap - 1u)];
+      j += 64u;
+    }
+#pragma unroll
+    for (int bt = 0; bt < GATHER_BATCH; ++bt)
+      if (ok[bt]) o[dst[bt]] = v[bt];
+    if (q >= nq) break;
+  }
 }
 
-void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uint32_t* counts,
-                          int64_t nseg, int64_t cap, uint32_t* const* out,
-                          uint64_t* const* total_dev) {
-  MultiLists L{};
+template <class T>
+static void gather_lists(Ctx* c, int K, const T* const* stage, const uint32_t* counts,
+                         int64_t nseg, int64_t cap, T* const* out, uint64_t* const* total_dev,
+                         bool scan_total) {
+  if (cap < 64) fail(PNM_EINVAL, "compaction: segment capacity below 64");
+  GatherLists<T> L{};
   for (int k = 0; k < K; ++k) {
     L.stage[k] = stage[k];
     L.out[k] = out[k];
@@ -606,21 +626,26 @@ void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uin
     return;
   }
   uint32_t* offs = c->arena.get_n<uint32_t>((size_t)(K * nseg));
-  scan_exclusive_u32(c, counts, offs, K * nseg);
-  PNM_LAUNCH(c, "k_gather_staged", k_gather_multi, dim3((unsigned)(K * nseg)), dim3(64), 0, L,
-             counts, offs, nseg, cap);
+  scan_exclusive_u32(c, counts, offs, K * nseg, scan_total ? total_dev[0] : nullptr);
+  const int64_t wpl = (nseg + GATHER_SEGS - 1) / GATHER_SEGS;
+  PNM_LAUNCH(c, "k_gather_staged", k_gather_lists<T>, dim3(cdiv(K * wpl, 4)), dim3(256), 0, L,
+             counts, offs, nseg, cap, wpl, K);
+}
+
+void compact_staged(Ctx* c, const uint32_t* stage, const uint32_t* counts, int64_t nblocks,
+                    int64_t cap, uint32_t* out, uint64_t* total_dev) {
+  gather_lists<uint32_t>(c, 1, &stage, counts, nblocks, cap, &out, &total_dev, false);
+}
+
+void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uint32_t* counts,
+                          int64_t nseg, int64_t cap, uint32_t* const* out,
+                          uint64_t* const* total_dev) {
+  gather_lists<uint32_t>(c, K, stage, counts, nseg, cap, out, total_dev, false);
 }
 
 void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
                       int64_t cap, uint64_t* out, uint64_t* total_dev) {
-  if (nblocks <= 0) {
-    fill_u64(c, total_dev, 0, 1);
-    return;
-  }
-  uint32_t* offs = c->arena.get_n<uint32_t>((size_t)nblocks);
-  scan_exclusive_u32(c, counts, offs, nblocks, total_dev);
-  PNM_LAUNCH(c, "k_gather_staged", k_gather_staged64, dim3((unsigned)nblocks), dim3(64), 0,
-             stage, counts, offs, cap, out);
+  gather_lists<uint64_t>(c, 1, &stage, counts, nblocks, cap, &out, &total_dev, false);
 }
 
 // ---------------- unique of a sorted list ----------------
