@@ -200,6 +200,50 @@ def flow_sample(ctx, sdf, mu=1.0, fx=1.0, rtol=1e-9, max_iter=40000, geometry_on
     return s, net, t_st
 
 
+def direction_spread(ctx, seeds, size=128, rtol=1e-9):
+    """How far g = Q/dp of one throat depends on the flow around it: each packing is solved with
+    the body force along x, y and z (same throats, since the geometry is the same), and the
+    median relative differences of the throats' g between directions are returned. A model that
+    sees only the throat's geometry gives one g per throat, so its median error for the x-driven
+    g cannot go far below that of the mean over the y and z directions ("x_vs_mean_yz")."""
+    torch = _torch()
+    from . import synth
+
+    G = []
+    for seed in seeds:
+        prm = packing_params(seed)
+        sdf, _, _ = synth.polydisperse_sdf((size, size, size), prm["median"], prm["sigma"],
+                                           prm["porosity"], seed=int(seed))
+        s = torch.from_numpy(np.ascontiguousarray(sdf, dtype=np.float32)).cuda()
+        n, dims = sdf.size, sdf.shape[::-1]
+        cols, th0 = [], None
+        for ax in range(3):
+            f = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(7)]
+            force = [0.0, 0.0, 0.0]
+            force[ax] = 1.0
+            ctx.stokes_solve_d(s, dims, (1, 1, 1), tuple(force), *f, mu=1.0, rtol=rtol,
+                               max_iter=40000)
+            net = ctx.extract_network_flow_d(s, dims, (0, 0, 0), (1, 1, 1), *f[:4], ox=f[4],
+                                             oy=f[5], oz=f[6],
+                                             grad_p_xyz=tuple(-v for v in force))
+            th = np.asarray(net["throats"], np.int64).reshape(-1, 2)
+            if th0 is not None and not np.array_equal(th, th0):
+                raise RuntimeError("throat sets differ between flow directions")
+            th0 = th
+            logg, ok = conductance_targets(net)
+            cols.append(np.where(ok, logg.astype(np.float64), np.nan))
+        G.append(np.stack(cols, axis=1))
+    G = np.concatenate(G) if G else np.zeros((0, 3))
+
+    def mre(a, b):
+        m = np.isfinite(a) & np.isfinite(b)
+        return float(np.median(np.abs(np.expm1(a[m] - b[m])))) if m.any() else None
+
+    return {"throats": int(G.shape[0]), "x_vs_y": mre(G[:, 0], G[:, 1]),
+            "x_vs_z": mre(G[:, 0], G[:, 2]), "y_vs_z": mre(G[:, 1], G[:, 2]),
+            "x_vs_mean_yz": mre(G[:, 0], 0.5 * (G[:, 1] + G[:, 2]))}
+
+
 def packing_params(seed):
     """Grain-pack parameters of training/test packing `seed` (varied so the model generalises)."""
     rng = np.random.default_rng(seed)

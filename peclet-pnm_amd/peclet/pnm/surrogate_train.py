@@ -1,7 +1,7 @@
 """Train the throat-conductance surrogate on GPU-generated data and evaluate it.
 
     python -m peclet.pnm.surrogate_train --train 56 --test 8 --size 128 --steps 4000 \\
-        --out <dir> [--weights <file.npz>]
+        --out <dir> [--weights <file.npz>] [--direction-spread N]
 
 Every packing is a polydisperse grain pack (packing_params(seed)); its fields come from the
 project Stokes solver and its throats and g = Q/dp from extract_network_flow. Train and test
@@ -29,6 +29,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--delta", type=float, default=0.25, help="Huber threshold (<= 0: MSE)")
+    ap.add_argument("--direction-spread", type=int, default=0, metavar="N",
+                    help="also solve N test packings with the flow along x, y and z and report "
+                         "how much each throat's g changes with the direction")
     ap.add_argument("--out", required=True)
     ap.add_argument("--weights", default=None)
     args = ap.parse_args(argv)
@@ -88,6 +91,9 @@ def main(argv=None):
         "loss_history": hist, "geometric_c": c_geo,
         "packings_train": info_tr, "packings_test": info_te,
     }
+    if args.direction_spread > 0:
+        report["direction_spread"] = S.direction_spread(
+            ctx, [9500 + k for k in range(args.direction_spread)], args.size)
     wpath = args.weights or os.path.join(args.out, "surrogate_weights.npz")
     np.savez(wpath, **model.state(), geometric_c=np.float64(c_geo))
     with open(os.path.join(args.out, "surrogate_report.json"), "w") as f:

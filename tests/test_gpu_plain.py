@@ -116,6 +116,17 @@ def test_random_shapes(capi_ctx, shape):
     check_against_oracle(capi_ctx, sdf, (0.5, -1.0, 2.0), (1.0, 0.5, 0.75))
 
 
+# Coarse levels make most neighbourhoods hold ties, including ties with wrapped neighbours at
+# every face of the volume (the tie goes to the larger flat index, whose order wraps there).
+TIE_SHAPES = [(3, 3, 3), (2, 1, 65), (2, 130, 3), (16, 8, 64), (17, 9, 66), (32, 16, 128)]
+
+
+@pytest.mark.parametrize("shape", TIE_SHAPES)
+def test_heavy_ties(capi_ctx, shape):
+    sdf = quantized(shape, sum(shape) * 11 + 5, step=1.5)
+    check_against_oracle(capi_ctx, sdf, (0, 0, 0), (1, 1, 1))
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_nan_inf(capi_ctx, seed):
     sdf = quantized((10, 12, 70), seed, nan_frac=0.05, inf_frac=0.01)
