@@ -1,7 +1,7 @@
 // k_topology.hip — boundary-pair topology (extractTopologyView, pore_extraction.hpp:401-451).
 //
 // The reference appends one (min,max) pair per differing +x/+y/+z face (tens of millions at
-// 512^3) and sorts them on the host. Here each block deduplicates the pairs of a 64x4x4 brick in
+// 512^3) and sorts them on the host. Here each block deduplicates the pairs of a 64x4x16 brick in
 // an LDS hash set and appends only its distinct keys; the short candidate list is then radix
 // sorted and made unique on the device. The result is the same sorted unique list.
 //
@@ -12,7 +12,7 @@
 namespace pnm {
 
 constexpr int TOPO_ZB = 16;         // z-slices per block (a 64x4x16 brick)
-constexpr int TOPO_TABLE = 2048;    // LDS hash slots (16 KiB)
+constexpr int TOPO_TABLE = 512;     // LDS hash slots (4 KiB): a brick has ~50 distinct pairs
 constexpr int TOPO_PROBES = 32;     // linear-probe limit before a key bypasses the table
 constexpr uint64_t kEmpty = ~0ull;
 
