@@ -15,6 +15,28 @@ constexpr int TILE_Z = 16;
 
 // Periodic +-1 neighbours; equal to the reference's ((x%N)+N)%N for x in [-1, N]
 // (pore_extraction.hpp:59-64), including N = 1 and N = 2 where neighbours alias.
+// Division of 32-bit values by a divisor fixed for a launch: q0 = mulhi(n, floor(2^32 / d)) is
+// q or q - 1, one correction makes it exact (d >= 2; d == 1 passes magic 0 and is special-cased).
+struct UDiv {
+  uint32_t d, magic;
+};
+inline UDiv make_udiv(uint32_t d) {
+  return UDiv{d, d >= 2 ? (uint32_t)((1ull << 32) / d) : 0u};
+}
+__device__ __forceinline__ uint32_t udiv(uint32_t n, UDiv v, uint32_t& rem) {
+  if (v.d == 1u) {
+    rem = 0u;
+    return n;
+  }
+  uint32_t q = __umulhi(n, v.magic);
+  uint32_t r = n - q * v.d;
+  if (r >= v.d) {
+    ++q;
+    r -= v.d;
+  }
+  rem = r;
+  return q;
+}
 __device__ __forceinline__ int wm1(int x, int N) { return x > 0 ? x - 1 : N - 1; }
 __device__ __forceinline__ int wp1(int x, int N) { return x + 1 < N ? x + 1 : 0; }
 

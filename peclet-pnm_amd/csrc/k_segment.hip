@@ -71,7 +71,7 @@ void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* ke
 // (relabel_min :335-336; every marker of a label comes after its root, which k_tile seeded),
 // and stages the component roots (solid keys) and the voxels still unlabelled (the band of the
 // next global sweep).
-constexpr int FLOOD_SWEEPS = 2;
+constexpr int FLOOD_SWEEPS = 1;
 constexpr int FR_X = 64 + 2 * FLOOD_SWEEPS, FR_Y = TILE_Y + 2 * FLOOD_SWEEPS,
               FR_Z = TILE_Z + 2 * FLOOD_SWEEPS;
 constexpr int FR_N = FR_X * FR_Y * FR_Z;
@@ -278,7 +278,7 @@ void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_cc
 // minimum labelled neighbour of the previous state; voxels still unlabelled are staged per block
 // for the next band list.
 __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__ L, int nx,
-                                                     int ny, int nz,
+                                                     int ny, int nz, UDiv dx, UDiv dy,
                                                      const uint32_t* __restrict__ band,
                                                      const uint64_t* __restrict__ count,
                                                      int32_t* __restrict__ newlab,
@@ -292,25 +292,30 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
   uint32_t i = 0;
   if (j < nband) {
     i = band[j];
-    const int x = (int)(i % (uint32_t)nx);
-    const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
-    const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
-    const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
-    const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
-    const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
-    int best = -1;
+    uint32_t x, y;
+    const uint32_t r = udiv(i, dx, x);  // r = z * ny + y
+    const uint32_t z = udiv(r, dy, y);
+    // neighbour offsets along each axis: -1, 0, +1 steps, wrapped at the faces
+    const int plane = nx * ny;
+    const int ox[3] = {x > 0 ? -1 : nx - 1, 0, (int)x + 1 < nx ? 1 : 1 - nx};
+    const int oy[3] = {y > 0 ? -nx : (ny - 1) * nx, 0, (int)y + 1 < ny ? nx : (1 - ny) * nx};
+    const int oz[3] = {z > 0 ? -plane : (nz - 1) * plane, 0,
+                       (int)z + 1 < nz ? plane : (1 - nz) * plane};
+    int nl[26];
+    int k = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const int row = (zs[a] * ny + ys[b]) * nx;
+      for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
           if (a == 1 && b == 1 && cc == 1) continue;
-          const int nl = L[row + xs[cc]];
-          if (nl != -1 && (best == -1 || nl < best)) best = nl;
+          nl[k++] = L[(int)i + oz[a] + oy[b] + ox[cc]];
         }
-      }
+    int best = -1;
+#pragma unroll
+    for (int q = 0; q < 26; ++q)  // a label is >= 0
+      if (nl[q] >= 0 && (best == -1 || nl[q] < best)) best = nl[q];
     newlab[j] = best;
     keep = (best == -1);
   }
@@ -340,7 +345,8 @@ void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uin
                  uint32_t* stage, uint32_t* stage_cnt, int32_t* min_solid) {
   const int64_t nb = cdiv(grid_items, 256);
   PNM_LAUNCH(c, "k_flood_sweep", k_flood_sweep, dim3((unsigned)nb), dim3(256), 0, labels, v.nx,
-             v.ny, v.nz, band, count, newlab, stage, stage_cnt);
+             v.ny, v.nz, make_udiv((uint32_t)v.nx), make_udiv((uint32_t)v.ny), band, count,
+             newlab, stage, stage_cnt);
   compact_staged(c, stage, stage_cnt, nb, 256, band_next, count_next);
   PNM_LAUNCH(c, "k_flood_apply", k_flood_apply, dim3((unsigned)nb), dim3(256), 0, labels, band,
              newlab, count, min_solid);

@@ -199,7 +199,7 @@ void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,
 void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots);
 void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                         uint32_t* key_cnt);
-// Flatten + the first two flood sweeps per tile: reads the CCL labels (tile-local roots whose
+// Flatten + the first flood sweep(s) per tile: reads the CCL labels (tile-local roots whose
 // own entries already hold their component root), writes the flooded labels to a second array
 // (segments as tile_segments / tile_segment_cap).
 void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
