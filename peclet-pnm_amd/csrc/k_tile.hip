@@ -294,6 +294,16 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
       if (a == l) o.min_solid[i] = i;
     }
   }
+  // x-face labels of each row chunk (first and last column), compact for the seam pass: entry
+  // (chunk, z, y); the 8 waves of the tile fill 64 contiguous bytes per slice
+  if (y < ny && lane < nzt) {
+    const int rb = (lane * TILE_Y + wave) * 64;
+    const int a0 = LAB[rb], aL = LAB[rb + xlast - x0];
+    int2 v;
+    v.x = a0 == -1 ? -1 : gidx(a0);
+    v.y = aL == -1 ? -1 : gidx(aL);
+    o.xface[((int64_t)blockIdx.x * nz + z0 + lane) * ny + y] = v;
+  }
   {  // stage the tile-local roots (k_lroot_flatten resolves them after the seams)
     unsigned nr = 0;
     for (int zl = 0; zl < nzt; ++zl) {
@@ -448,16 +458,22 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
 // chunk start is always cross-tile. Only rows at a tile boundary in y or z have crossing
 // forward rows: y % TILE_Y in {0, TILE_Y-1}, y = ny-1, z % TILE_Z = TILE_Z-1, z = nz-1.
 
-__device__ __forceinline__ void seam_rows(int y, int z, int nx, int ny, int nz, int rows[4],
+// Forward rows of row (y, z): (y+1, z), (y-1, z+1), (y, z+1), (y+1, z+1) as (y', z'), and
+// whether each lies in the same tile as the row.
+__device__ __forceinline__ void seam_rows(int y, int z, int ny, int nz, int fy[4], int fz[4],
                                           bool same[4]) {
   const int yp = wp1(y, ny), ym = wm1(y, ny), zp = wp1(z, nz);
   const bool ty_p = (y + 1 < ny) && ((y + 1) / TILE_Y == y / TILE_Y);
   const bool ty_m = (y >= 1) && ((y - 1) / TILE_Y == y / TILE_Y);
   const bool tz_p = (z + 1 < nz) && ((z + 1) / TILE_Z == z / TILE_Z);
-  rows[0] = (z * ny + yp) * nx;
-  rows[1] = (zp * ny + ym) * nx;
-  rows[2] = (zp * ny + y) * nx;
-  rows[3] = (zp * ny + yp) * nx;
+  fy[0] = yp;
+  fz[0] = z;
+  fy[1] = ym;
+  fz[1] = zp;
+  fy[2] = y;
+  fz[2] = zp;
+  fy[3] = yp;
+  fz[3] = zp;
   same[0] = ty_p;
   same[1] = ty_m && tz_p;
   same[2] = tz_p;
@@ -475,95 +491,144 @@ __device__ __forceinline__ int z_boundary_slice(int j, int nz) {  // j < ceil(nz
   return z < nz ? z : nz - 1;
 }
 
-// Full run-union rule on the crossing forward rows of one row chunk (one wave).
-// mode 0: rows at a y boundary (index blockIdx.y * 4 + wave into them), every z;
-// mode 1: slices at a z boundary (index blockIdx.z), rows not at a y boundary.
-__global__ __launch_bounds__(256) void k_ccl_seams_rows(int* L, int nx, int ny, int nz, int mode,
-                                                        int nsel) {
-  const int lane = threadIdx.x;
-  const int x0 = blockIdx.x * 64;
-  int y, z;
-  if (mode == 0) {
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    if (j >= nsel) return;  // wave-uniform
-    y = y_boundary_row(j, ny);
-    z = blockIdx.z;
-  } else {
-    y = blockIdx.y * 4 + threadIdx.y;
-    if (y >= ny || y_boundary(y, ny)) return;  // wave-uniform
-    z = z_boundary_slice(blockIdx.z, nz);
+// Per-block set of the tile-local root pairs a seam pass has to unite: one seam crosses the
+// same pair of grain cores in many rows, so each distinct pair is united once, after the block
+// has collected them. A pair that finds no slot is united at once.
+constexpr int SEAM_TABLE = 512;
+constexpr int SEAM_PROBES = 16;
+constexpr unsigned long long kNoPair = ~0ull;
+__device__ __forceinline__ void seam_pair(unsigned long long* tab, int* L, int a, int b) {
+  if (a == b) return;
+  const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
+  const unsigned long long key = ((unsigned long long)lo << 32) | hi;
+  uint32_t h = (lo * 0x9E3779B1u ^ hi * 0x85EBCA77u) >> 23;  // 9 bits: SEAM_TABLE slots
+  for (int p = 0; p < SEAM_PROBES; ++p) {
+    const unsigned long long cur = tab[h];
+    if (cur == key) return;
+    if (cur == kNoPair) {
+      const unsigned long long prv = atomicCAS(&tab[h], kNoPair, key);
+      if (prv == kNoPair || prv == key) return;
+    }
+    h = (h + 1) & (SEAM_TABLE - 1);
   }
-  int rows[4];
-  bool same[4];
-  seam_rows(y, z, nx, ny, nz, rows, same);
-  if (same[0] && same[1] && same[2] && same[3]) return;
-  const int x = x0 + lane;
-  const bool in = x < nx;
-  const int xlast = min(x0 + 63, nx - 1);
-  const int i = in ? (z * ny + y) * nx + x : 0;
-  const int my = in ? L[i] : -1;
-  const bool mk = my != -1;
-  const uint64_t m = __ballot(mk);
-  if (m == 0) return;
-  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
-  const bool run_end = mk && (x == xlast || !((m >> (lane + 1)) & 1ull));
-  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (same[r]) continue;  // wave-uniform; edges of same-tile rows: k_ccl_seams_edges
-    const int c = in ? L[rows[r] + x] : -1;
-    const bool cm = c != -1;
-    const int prev_c = __shfl_up(c, 1, 64);
-    if (mk && cm) {
-      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
-      if (!dup) uf_unite(L, my, c);
-    }
-    if (mk && !cm) {
-      if (run_start) {
-        const int a = L[rows[r] + xm];
-        if (a != -1) uf_unite(L, my, a);
-      }
-      if (run_end) {
-        const int b = L[rows[r] + xp];
-        if (b != -1) uf_unite(L, my, b);
-      }
-    }
+  uf_unite(L, (int)lo, (int)hi);
+}
+__device__ __forceinline__ void seam_flush(unsigned long long* tab, int* L) {
+  __syncthreads();
+  for (int s = threadIdx.y * 64 + threadIdx.x; s < SEAM_TABLE; s += 256) {
+    const unsigned long long k = tab[s];
+    if (k != kNoPair) uf_unite(L, (int)(k >> 32), (int)(uint32_t)k);
   }
 }
 
-// One thread per row chunk: the +x relation at the chunk start, and for forward rows in the
-// same tile the chunk's first column with column x-1 and its last column with column x+1.
-__global__ __launch_bounds__(256) void k_ccl_seams_edges(int* L, int nx, int ny, int nz) {
-  const int cx = (nx + 63) >> 6;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)cx * ny * nz) return;
-  const int c = (int)(t % cx);
-  const int64_t r = t / cx;
-  const int y = (int)(r % ny), z = (int)(r / ny);
-  const int x0 = c * 64, xlast = min(x0 + 63, nx - 1);
-  const int row = (z * ny + y) * nx;
-  const int m0 = L[row + x0], mL = L[row + xlast];
-  if (m0 == -1 && mL == -1) return;
-  const int xm = wm1(x0, nx), xp = wp1(xlast, nx);
-  if (m0 != -1) {
-    const int nl = L[row + xm];
-    if (nl != -1) uf_unite(L, m0, nl);
-  }
-  int rows[4];
-  bool same[4];
-  seam_rows(y, z, nx, ny, nz, rows, same);
+// Full run-union rule on the crossing forward rows of row chunks, one row per wave at a time.
+// mode 0: rows at a y boundary (boundary row blockIdx.y), slices blockIdx.z * 16 .. + 15;
+// mode 1: slices at a z boundary (blockIdx.z), rows of y tile blockIdx.y not at a y boundary.
+__global__ __launch_bounds__(256) void k_ccl_seams_rows(int* L, int nx, int ny, int nz, int mode,
+                                                        int nsel) {
+  __shared__ unsigned long long tab[SEAM_TABLE];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  for (int s = wave * 64 + lane; s < SEAM_TABLE; s += 256) tab[s] = kNoPair;
+  __syncthreads();
+  const int x0 = blockIdx.x * 64;
+  const int x = x0 + lane;
+  const bool in = x < nx;
+  const int xlast = min(x0 + 63, nx - 1);
+  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
+  const int nrow = mode == 0 ? 16 : TILE_Y;
+  for (int k = wave; k < nrow; k += 4) {
+    int y, z;
+    if (mode == 0) {
+      if ((int)blockIdx.y >= nsel) break;  // block-uniform
+      y = y_boundary_row(blockIdx.y, ny);
+      z = blockIdx.z * 16 + k;
+      if (z >= nz) break;  // wave-uniform
+    } else {
+      y = blockIdx.y * TILE_Y + k;
+      if (y >= ny) break;  // wave-uniform
+      if (y_boundary(y, ny)) continue;
+      z = z_boundary_slice(blockIdx.z, nz);
+    }
+    int fy[4], fz[4];
+    bool same[4];
+    seam_rows(y, z, ny, nz, fy, fz, same);
+    if (same[0] && same[1] && same[2] && same[3]) continue;
+    const int i = in ? (z * ny + y) * nx + x : 0;
+    const int my = in ? L[i] : -1;
+    const bool mk = my != -1;
+    const uint64_t m = __ballot(mk);
+    if (m == 0) continue;
+    const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+    const bool run_end = mk && (x == xlast || !((m >> (lane + 1)) & 1ull));
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (!same[q]) continue;
-    if (m0 != -1 && L[rows[q] + x0] == -1) {
-      const int a = L[rows[q] + xm];
-      if (a != -1) uf_unite(L, m0, a);
-    }
-    if (mL != -1 && L[rows[q] + xlast] == -1) {
-      const int b = L[rows[q] + xp];
-      if (b != -1) uf_unite(L, mL, b);
+    for (int r = 0; r < 4; ++r) {
+      if (same[r]) continue;  // wave-uniform; edges of same-tile rows: k_ccl_seams_edges
+      const int rb = (fz[r] * ny + fy[r]) * nx;
+      const int c = in ? L[rb + x] : -1;
+      const bool cm = c != -1;
+      const int prev_c = __shfl_up(c, 1, 64);
+      if (mk && cm) {
+        const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+        if (!dup) seam_pair(tab, L, my, c);
+      }
+      if (mk && !cm) {
+        if (run_start) {
+          const int a = L[rb + xm];
+          if (a != -1) seam_pair(tab, L, my, a);
+        }
+        if (run_end) {
+          const int b = L[rb + xp];
+          if (b != -1) seam_pair(tab, L, my, b);
+        }
+      }
     }
   }
+  seam_flush(tab, L);
+}
+
+// One thread per row chunk, from the x-face labels k_tile wrote (entry (chunk, z, y): labels of
+// the chunk's first and last column): the +x relation at the chunk start, and for forward rows
+// in the same tile the chunk's first column with column x-1 and its last column with column
+// x+1. 256 consecutive rows of one chunk per block share a pair set.
+__global__ __launch_bounds__(256) void k_ccl_seams_edges(int* L, const int2* __restrict__ xf,
+                                                         int ny, int nz, int cx) {
+  __shared__ unsigned long long tab[SEAM_TABLE];
+  const int tid = threadIdx.x;
+  for (int s = tid; s < SEAM_TABLE; s += 256) tab[s] = kNoPair;
+  __syncthreads();
+  const int64_t per_chunk = (int64_t)ny * nz;
+  const int64_t t = (int64_t)blockIdx.x * 256 + tid;
+  if (t < (int64_t)cx * per_chunk) {
+    const int bx = (int)(t / per_chunk);
+    const int r = (int)(t - (int64_t)bx * per_chunk);
+    const int z = r / ny, y = r - z * ny;
+    const int bxm = bx > 0 ? bx - 1 : cx - 1, bxp = bx + 1 < cx ? bx + 1 : 0;
+    auto face = [&](int b, int zz, int yy) { return xf[((int64_t)b * nz + zz) * ny + yy]; };
+    const int2 me = face(bx, z, y);
+    if (me.x != -1 || me.y != -1) {
+      if (me.x != -1) {
+        const int left = face(bxm, z, y).y;
+        if (left != -1) seam_pair(tab, L, me.x, left);
+      }
+      int fy[4], fz[4];
+      bool same[4];
+      seam_rows(y, z, ny, nz, fy, fz, same);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!same[q]) continue;
+        const int2 f = face(bx, fz[q], fy[q]);
+        if (me.x != -1 && f.x == -1) {
+          const int a = face(bxm, fz[q], fy[q]).y;
+          if (a != -1) seam_pair(tab, L, me.x, a);
+        }
+        if (me.y != -1 && f.y == -1) {
+          const int b = face(bxp, fz[q], fy[q]).x;
+          if (b != -1) seam_pair(tab, L, me.y, b);
+        }
+      }
+    }
+  }
+  seam_flush(tab, L);
 }
 
 // ---- tile-local roots -> component roots ----
@@ -694,15 +759,17 @@ void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
              o);
 }
 
-void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels) {
+void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
   const unsigned cx = cdiv(v.nx, 64);
   const int ysel = 2 * (int)cdiv(v.ny, TILE_Y), zsel = (int)cdiv(v.nz, TILE_Z);
-  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows, dim3(cx, cdiv(ysel, 4), (unsigned)v.nz),
+  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows, dim3(cx, (unsigned)ysel, cdiv(v.nz, 16)),
              dim3(64, 4), 0, labels, v.nx, v.ny, v.nz, 0, ysel);
-  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows, dim3(cx, cdiv(v.ny, 4), (unsigned)zsel),
-             dim3(64, 4), 0, labels, v.nx, v.ny, v.nz, 1, zsel);
+  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows,
+             dim3(cx, cdiv(v.ny, TILE_Y), (unsigned)zsel), dim3(64, 4), 0, labels, v.nx, v.ny,
+             v.nz, 1, zsel);
   PNM_LAUNCH(c, "k_ccl_seams_edges", k_ccl_seams_edges,
-             dim3(cdiv((int64_t)cx * v.ny * v.nz, 256)), dim3(256), 0, labels, v.nx, v.ny, v.nz);
+             dim3(cdiv((int64_t)cx * v.ny * v.nz, 256)), dim3(256), 0, labels,
+             reinterpret_cast<const int2*>(xface), v.ny, v.nz, (int)cx);
 }
 
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,

@@ -111,6 +111,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   const int64_t nwx = cdiv(v.nx, 64), nwords = nwx * v.ny * v.nz;
   uint64_t* peak_map = c->arena.get_n<uint64_t>((size_t)nwords);
   o.peak_map = peak_map;
+  int32_t* xface = c->arena.get_n<int32_t>((size_t)(2 * nwx * v.ny * v.nz));
+  o.xface = reinterpret_cast<int2*>(xface);
   o.lroot_stage = stage_r;
   o.lroot_cnt = cnt_r;
   o.open_stage = stage_c;
@@ -119,7 +121,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   launch_tile(c, sdf, v, thr, o);
   bitmap_list(c, peak_map, nwords, nwx, v.nx, peaks, ctr + 0);  // peaks in voxel order
   compact_staged(c, stage_c, cnt_c, nseg, scap, open, ctr + 1);
-  launch_ccl_seams(c, v, lab_ccl);
+  launch_ccl_seams(c, v, lab_ccl, xface);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
   if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
     launch_ccl_flatten(c, v, lab_ccl, min_solid, stage_b, cnt_b);

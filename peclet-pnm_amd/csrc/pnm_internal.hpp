@@ -180,6 +180,7 @@ struct TileOut {
   int32_t* min_pore;      // first-appearance minimum over the tile, at in-tile roots   [n]
   int32_t* min_solid;     // seeded with its own index at every tile-local marker root  [n]
   uint64_t* peak_map;     // extract_pores peaks: bit x%64 of word (z*ny + y)*cdiv(nx,64) + x/64
+  int2* xface;            // labels of each row chunk's first/last column, entry (chunk, z, y)
   uint32_t* lroot_stage;  // staged tile-local marker roots
   uint32_t* lroot_cnt;
   uint32_t* open_stage;   // staged sdf>0 voxels whose walk left the tile
@@ -189,7 +190,9 @@ struct TileOut {
 int64_t tile_segments(Vol v);
 int64_t tile_segment_cap();
 void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o);
-void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels);
+// Cross-tile and wrapping marker relations; xface: the tile pass's x-face labels (int2 per
+// (chunk, z, y)).
+void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface);
 // After the seams: every tile-local root (staged by the tile pass) -> its component root.
 void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
                           const uint32_t* cnt);

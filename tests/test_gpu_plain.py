@@ -248,7 +248,15 @@ def test_512_properties(capi_ctx, kind):
     seg3 = seg.view(shape)
     top = pairs[: 2 * npairs].view(-1, 2)
     ref = _torch_topology(seg3)
-    assert torch.equal(top, ref)
+    if not torch.equal(top, ref):
+        a = set(map(tuple, top.cpu().tolist()))
+        b = set(map(tuple, ref.cpu().tolist()))
+        sorted_ok = bool((top[1:, 0] > top[:-1, 0]).logical_or(
+            (top[1:, 0] == top[:-1, 0]) & (top[1:, 1] > top[:-1, 1])).all())
+        raise AssertionError(f"topology differs: {top.shape[0]} pairs vs {ref.shape[0]}; "
+                             f"sorted unique {sorted_ok}; only in result {sorted(a - b)[:8]} "
+                             f"({len(a - b)}); only in reference {sorted(b - a)[:8]} "
+                             f"({len(b - a)}); seg range [{int(seg.min())}, {int(seg.max())}]")
     npl, nsl = int(seg.max()), int(-seg.min())
     assert npl == tot  # no NaN and no capped walks: one pore per peak
     idx = torch.arange(n, device="cuda", dtype=torch.int64)
