@@ -322,8 +322,11 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     return ax >= 0 && ax < 64 && x0 + ax < nx && ay >= 0 && ay < TILE_Y && y0 + ay < ny &&
            az >= 0 && az < nzt;
   };
-  for (int zl = 0; zl < nzt; ++zl) {
-    if (!((pore_mask >> zl) & 1u)) continue;
+  // Each thread walks only the set bits of its masks: its sdf>0 slices, then the slices whose
+  // walk is still active (the wave's trip count is its busiest lane's, not 16).
+  uint32_t act = 0;
+  for (uint32_t m = pore_mask; m; m &= m - 1) {
+    const int zl = __ffs(m) - 1;
     const int l = (zl * TILE_Y + wave) * 64 + lane;
     const int d = DIR[l] & kDirMask;
     uint32_t w;
@@ -333,41 +336,44 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
       int dx, dy, dz;
       dir_delta(d, dx, dy, dz);
       const int ax = lane + dx, ay = wave + dy, az = zl + dz;
-      w = in_tile(ax, ay, az) ? cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT)
-                              : cs_pack(l, 0, CS_EXIT);
+      if (in_tile(ax, ay, az)) {
+        w = cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT);
+        act |= 1u << zl;
+      } else {
+        w = cs_pack(l, 0, CS_EXIT);
+      }
     }
     LAB[l] = (int)w;
   }
   __syncthreads();
   while (true) {
-    bool active = false;
-    for (int zl = 0; zl < nzt; ++zl) {
-      if (!((pore_mask >> zl) & 1u)) continue;
+    for (uint32_t m = act; m; m &= m - 1) {
+      const int zl = __ffs(m) - 1;
       const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const uint32_t w = (uint32_t)LAB[l];
-      if (cs_st(w) != CS_ACT) continue;
+      const uint32_t w = (uint32_t)LAB[l];  // active: only this thread writes it
       const uint32_t wv = (uint32_t)LAB[cs_u(w)];
       const int s2 = cs_s(w) + cs_s(wv);
       const uint32_t sv = cs_st(wv);
       if (sv == CS_LONG || s2 > kMaxSteps) {
         LAB[l] = (int)cs_pack(cs_u(w), cs_s(w), CS_LONG);
+        act &= ~(1u << zl);
       } else {
         LAB[l] = (int)cs_pack(cs_u(wv), s2, sv);
-        active |= sv == CS_ACT;
+        if (sv != CS_ACT) act &= ~(1u << zl);
       }
     }
-    if (!__syncthreads_or(active)) break;
+    if (!__syncthreads_or(act != 0)) break;
   }
 
   // ---- walk outputs; LAB at each fixed point collects the smallest local index whose walk
   //      ends there (every other LAB word read below belongs to the reading thread) ----
   bool capped_any = false;
-  for (int zl = 0; zl < nzt; ++zl) {
+  uint32_t open_mask = 0;
+  for (uint32_t m = pore_mask; m; m &= m - 1) {
+    const int zl = __ffs(m) - 1;
     const int l = (zl * TILE_Y + wave) * 64 + lane;
     const int i = ((z0 + zl) * ny + y) * nx + x;
-    const bool pore = (pore_mask >> zl) & 1u;
-    bool open = false;
-    if (pore) {
+    {
       int target, steps, fin_u = -1;
       bool fin = false, capped = false;
       const bool root_here = (DIR[l] & kDirMask) == 13;
@@ -430,19 +436,25 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
         if (!capped) atomicMin(&LAB[fin_u], l);
         capped_any |= capped;
       }
-      open = !fin;
+      if (!fin) open_mask |= 1u << zl;
       const bool boundary = x == x0 || x == xlast || y == y0 || y == ylast || zl == 0 ||
                             zl == nzt - 1;
-      if (open || boundary) o.walk[i] = walk_record(target, steps, fin, capped);
+      if (!fin || boundary) o.walk[i] = walk_record(target, steps, fin, capped);
     }
+  }
+  // stage the open walks in (slice, lane) order
+  for (int zl = 0; zl < nzt; ++zl) {
+    const bool open = (open_mask >> zl) & 1u;
     const uint64_t om = __ballot(open);
-    if (open) o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] = (uint32_t)i;
+    if (open)
+      o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
+          (uint32_t)(((z0 + zl) * ny + y) * nx + x);
     nop += (unsigned)__popcll(om);
   }
   if (__any(capped_any) && lane == 0) atomicOr(o.cap_flag, 1ull);
   __syncthreads();
-  for (int zl = 0; zl < nzt; ++zl) {
-    if (!((pore_mask >> zl) & 1u)) continue;
+  for (uint32_t m = pore_mask; m; m &= m - 1) {
+    const int zl = __ffs(m) - 1;
     const int l = (zl * TILE_Y + wave) * 64 + lane;
     if ((DIR[l] & kDirMask) == 13) o.min_pore[gidx(l)] = gidx(LAB[l]);
   }
