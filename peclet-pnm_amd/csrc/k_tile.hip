@@ -277,10 +277,20 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     }
   }
   __syncthreads();
+  // flatten: a read-only chase and one store of this thread's own entry. (A find with path
+  // halving here would write ancestors into other voxels' entries, and such a write landing after
+  // that voxel's owner stored its root leaves an intermediate node as its label.)
   for (int zl = 0; zl < nzt; ++zl) {
     const int l = (zl * TILE_Y + wave) * 64 + lane;
-    const int a = LAB[l];
-    if (a != -1) LAB[l] = lds_find(LAB, a);
+    int r = LAB[l];
+    if (r != -1) {
+      int p = LAB[r];
+      while (p != r) {
+        r = p;
+        p = LAB[r];
+      }
+      LAB[l] = r;
+    }
   }
   __syncthreads();
   if (col_in) {

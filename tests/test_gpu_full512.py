@@ -53,3 +53,27 @@ def test_full512_bit_exact(capi_ctx, kind):
     assert digest(segh) == exp["seg"]
     assert npairs == exp["nconn"]
     assert digest(pairs[: 2 * npairs].view(-1, 2).cpu().numpy()) == exp["conns"]
+
+
+def test_full512_repeated_alternating(capi_ctx):
+    """Both workloads alternately on one context, six times each: every label volume and
+    connection list must still equal the oracle's (a timing-dependent race in a flatten once
+    left an intermediate union-find node as a label in about one run in ten to thirty)."""
+    with open(os.path.join(HERE, "golden", "full512.json")) as f:
+        exp = json.load(f)
+    vols = {k: torch.from_numpy(np.ascontiguousarray(make_sdf(k), dtype=np.float32)).cuda()
+            for k in ("polydisperse", "boolean")}
+    n = 512 ** 3
+    seg = torch.empty(n, dtype=torch.int32, device="cuda")
+    pores = torch.empty(4 * 1_000_000, dtype=torch.float32, device="cuda")
+    pairs = torch.empty(2 * 20_000_000, dtype=torch.int32, device="cuda")
+    bad = []
+    for it in range(6):
+        for kind, s in vols.items():
+            _, _, npairs = capi_ctx.extract_pore_network_d(s, SHAPE, (0, 0, 0), (1, 1, 1), pores,
+                                                           seg, pairs)
+            ok = (digest(seg.cpu().numpy()) == exp[kind]["seg"] and npairs == exp[kind]["nconn"]
+                  and digest(pairs[: 2 * npairs].view(-1, 2).cpu().numpy()) == exp[kind]["conns"])
+            if not ok:
+                bad.append((it, kind, npairs))
+    assert not bad, bad
