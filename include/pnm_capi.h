@@ -84,6 +84,10 @@ int64_t pnm_ctx_arena_bytes(pnm_ctx_t ctx);
  * read: names (newline-separated into `names`), total milliseconds and launch counts.
  * *count = number of distinct kernels (PNM_ECAPACITY when it exceeds cap). */
 pnm_status pnm_ctx_profile(pnm_ctx_t ctx, int enable);
+/* Restrict kernel timing to the launches whose kernel names are listed in `names`
+ * (newline-separated); NULL or "" times every launch again. Launches that are not timed carry
+ * no events, so the stream runs as it does with timing off. */
+pnm_status pnm_ctx_profile_only(pnm_ctx_t ctx, const char* names);
 pnm_status pnm_ctx_profile_read(pnm_ctx_t ctx, char* names, int64_t names_len, double* total_ms,
                                 int64_t* launches, int64_t cap, int64_t* count);
 

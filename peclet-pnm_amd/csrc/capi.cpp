@@ -71,8 +71,22 @@ hipEvent_t Ctx::take_event() {
   return e;
 }
 
+bool Ctx::timed(const char* nm) const {
+  if (!prof) return false;
+  if (prof_only.empty()) return true;
+  const size_t len = std::strlen(nm);
+  size_t pos = 0;
+  while (pos <= prof_only.size()) {
+    size_t end = prof_only.find('\n', pos);
+    if (end == std::string::npos) end = prof_only.size();
+    if (end - pos == len && prof_only.compare(pos, len, nm) == 0) return true;
+    pos = end + 1;
+  }
+  return false;
+}
+
 KTimer::KTimer(Ctx* ctx, const char* nm) : c(ctx), name(nm) {
-  if (!c->prof) return;
+  if (!c->timed(nm)) return;
   a = c->take_event();
   PNM_HIP(hipEventRecord(a, c->stream));
 }
@@ -270,6 +284,13 @@ pnm_status pnm_ctx_profile(pnm_ctx_t ctx, int enable) {
   return guarded([&] {
     Ctx* c = check_ctx(ctx);
     c->prof = enable != 0;
+  });
+}
+
+pnm_status pnm_ctx_profile_only(pnm_ctx_t ctx, const char* names) {
+  return guarded([&] {
+    Ctx* c = check_ctx(ctx);
+    c->prof_only = names ? names : "";
   });
 }
 

@@ -57,64 +57,65 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   if (tid == 0) s_base = 0;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, z0 = blockIdx.z * TOPO_ZB;
   if (FUSED) {
-    // Rows of the brick + halo over the 4 waves; lane covers column lane (and lane 0 the halo
-    // column 64). Positions past the volume hold the wrapped voxel: the +x/+y/+z neighbour of
-    // the volume's last voxel is read there. Brick voxels inside the volume also write seg.
-    const int gx0 = wrap_any(x0 + lane, nx), gx1 = wrap_any(x0 + 64, nx);
-    // pass 1: the value's sign picks the label source (root of an sdf>0 voxel, flagged in the
-    // top bit, or the solid label); pass 2: the id. Loads are issued in branch-free batches of
-    // RB rows into registers (rows past the region re-read a valid row).
+    // Rows of the brick + halo over the 4 waves, lane = column; the halo column 64 of the 85 rows
+    // is one more chain on threads 0..84, issued with the first batch. Positions past the volume
+    // hold the wrapped voxel: the +x/+y/+z neighbour of the volume's last voxel is read there.
+    // Brick voxels inside the volume also write seg.
+    // Each value's sign picks the label source (root of an sdf>0 voxel, flagged in the top bit,
+    // or the solid label), the label gives the id. Loads are issued in branch-free batches of RB
+    // rows into registers (rows past the region re-read a valid row).
     constexpr int NROW = TB_Y * TB_Z, RB = 11, MAXR = (NROW + 3) / 4;
-    const bool two = lane == 0;
+    const int gx = wrap_any(x0 + lane, nx), gxh = wrap_any(x0 + 64, nx);
     const bool near = ny >= TB_Y && nz >= TB_Z;  // row coordinates lie in [0, 2N)
+    auto row_base = [&](int row) {
+      const int ry = row % TB_Y, rz = row / TB_Y;
+      const int wy = near ? wrap_near(y0 + ry, ny) : wrap_any(y0 + ry, ny);
+      const int wz = near ? wrap_near(z0 + rz, nz) : wrap_any(z0 + rz, nz);
+      return ((int64_t)wz * ny + wy) * nx;
+    };
+    auto label_of = [&](float v, int64_t g) {
+      const int32_t* src = v > 0.0f ? ids.roots : ids.labels;
+      const int r = src[g];
+      return v > 0.0f ? (int)((uint32_t)r | 0x80000000u) : r;
+    };
+    auto id_of = [&](int k) {
+      const int a = k < -1 ? ids.pore_id[(uint32_t)k & 0x7fffffffu]
+                           : ids.solid_id[k >= 0 ? k : 0];
+      return k == -1 ? 0 : a;
+    };
+    const int hrow = tid < NROW ? tid : 0;
+    const int swave = __builtin_amdgcn_readfirstlane(wave);  // row math in scalar registers
+#pragma unroll 1
     for (int j0 = 0; j0 < MAXR; j0 += RB) {
-      // the sign of the value picks the one array to read: the root of an sdf>0 voxel (flagged
-      // in the top bit) or the solid label
-      int64_t g0[RB], g1[RB];
-      float v0[RB], v1[RB];
+      const bool halo = j0 == 0;
+      int64_t g[RB];
+      float v[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
-        const int row0 = wave + (j0 + j) * 4;
-        const int row = row0 < NROW ? row0 : wave;
-        const int ry = row % TB_Y, rz = row / TB_Y;
-        const int wy = near ? wrap_near(y0 + ry, ny) : wrap_any(y0 + ry, ny);
-        const int wz = near ? wrap_near(z0 + rz, nz) : wrap_any(z0 + rz, nz);
-        const int64_t base = ((int64_t)wz * ny + wy) * nx;
-        g0[j] = base + gx0;
-        g1[j] = base + gx1;
-        v0[j] = ids.sdf[g0[j]];
-        v1[j] = ids.sdf[g1[j]];
+        const int row0 = swave + (j0 + j) * 4;
+        g[j] = row_base(row0 < NROW ? row0 : swave) + gx;
+        v[j] = ids.sdf[g[j]];
       }
-      int k0[RB], k1[RB];
+      const int64_t gh = row_base(hrow) + gxh;
+      const float vh = halo ? ids.sdf[gh] : 0.0f;
+      int k[RB];
 #pragma unroll
-      for (int j = 0; j < RB; ++j) {
-        const int32_t* src0 = v0[j] > 0.0f ? ids.roots : ids.labels;
-        const int32_t* src1 = v1[j] > 0.0f ? ids.roots : ids.labels;
-        const int r0 = src0[g0[j]], r1 = src1[g1[j]];
-        k0[j] = v0[j] > 0.0f ? (int)((uint32_t)r0 | 0x80000000u) : r0;
-        k1[j] = v1[j] > 0.0f ? (int)((uint32_t)r1 | 0x80000000u) : r1;
-      }
-      int s0[RB], s1[RB];
+      for (int j = 0; j < RB; ++j) k[j] = label_of(v[j], g[j]);
+      const int kh = halo ? label_of(vh, gh) : -1;
 #pragma unroll
-      for (int j = 0; j < RB; ++j) {
-        const int a0 = k0[j] < -1 ? ids.pore_id[(uint32_t)k0[j] & 0x7fffffffu]
-                                  : ids.solid_id[k0[j] >= 0 ? k0[j] : 0];
-        const int a1 = k1[j] < -1 ? ids.pore_id[(uint32_t)k1[j] & 0x7fffffffu]
-                                  : ids.solid_id[k1[j] >= 0 ? k1[j] : 0];
-        s0[j] = k0[j] == -1 ? 0 : a0;
-        s1[j] = k1[j] == -1 ? 0 : a1;
-      }
+      for (int j = 0; j < RB; ++j) k[j] = id_of(k[j]);
+      const int sh = halo ? id_of(kh) : 0;
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         const int row = wave + (j0 + j) * 4;
         if (row < NROW) {
           const int ry = row % TB_Y, rz = row / TB_Y;
-          S[row * TB_X + lane] = s0[j];
-          if (two) S[row * TB_X + 64] = s1[j];
+          S[row * TB_X + lane] = k[j];
           if (ry < 4 && rz < TOPO_ZB && x0 + lane < nx && y0 + ry < ny && z0 + rz < nz)
-            ids.seg_out[((int64_t)(z0 + rz) * ny + (y0 + ry)) * nx + gx0] = s0[j];
+            ids.seg_out[((int64_t)(z0 + rz) * ny + (y0 + ry)) * nx + gx] = k[j];
         }
       }
+      if (halo && tid < NROW) S[tid * TB_X + 64] = sh;
     }
   }
   __syncthreads();

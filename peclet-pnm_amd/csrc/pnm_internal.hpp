@@ -64,8 +64,11 @@ struct Ctx {
   int64_t lb_cap = 0;
   uint32_t* lb_ctl = nullptr;
   // Kernel timing: when `prof` is set every launch is bracketed by two HIP events on the
-  // launch stream; pnm_ctx_profile_read aggregates them per kernel name.
+  // launch stream (only launches named in `prof_only`, newline-separated, when it is not
+  // empty); pnm_ctx_profile_read aggregates them per kernel name.
   bool prof = false;
+  std::string prof_only;
+  bool timed(const char* name) const;
   struct Rec {
     const char* name;
     hipEvent_t a, b;

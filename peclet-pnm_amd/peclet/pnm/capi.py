@@ -22,7 +22,8 @@ PNM_MAX_PORES = 1000000
 EXPORTS = (
     "pnm_version", "pnm_execution_space", "pnm_last_error", "pnm_device_count",
     "pnm_ctx_create", "pnm_ctx_destroy", "pnm_ctx_set_stream", "pnm_ctx_stream", "pnm_ctx_trim",
-    "pnm_ctx_arena_bytes", "pnm_ctx_profile", "pnm_ctx_profile_read", "pnm_free", "pnm_extract_pores", "pnm_segment_volume",
+    "pnm_ctx_arena_bytes", "pnm_ctx_profile", "pnm_ctx_profile_only", "pnm_ctx_profile_read",
+    "pnm_free", "pnm_extract_pores", "pnm_segment_volume",
     "pnm_extract_topology", "pnm_extract_pore_network", "pnm_extract_network_flow",
     "pnm_flow_result_free", "pnm_extract_pores_d", "pnm_segment_volume_d",
     "pnm_extract_topology_d", "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
@@ -87,6 +88,8 @@ def lib():
         L.pnm_ctx_arena_bytes.restype = ctypes.c_int64
         L.pnm_ctx_profile.argtypes = [vp, ctypes.c_int]
         L.pnm_ctx_profile.restype = st
+        L.pnm_ctx_profile_only.argtypes = [vp, ctypes.c_char_p]
+        L.pnm_ctx_profile_only.restype = st
         L.pnm_ctx_profile_read.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64,
                                            ctypes.POINTER(ctypes.c_double), i64p, ctypes.c_int64,
                                            i64p]
@@ -188,6 +191,11 @@ class Context:
     def profile(self, enable=True):
         """Bracket every kernel launch with HIP events on the launch stream."""
         _check(lib().pnm_ctx_profile(self.h, int(bool(enable))))
+
+    def profile_only(self, names=None):
+        """Time only the launches of these kernel names (None: every launch)."""
+        arg = "\n".join(names).encode() if names else None
+        _check(lib().pnm_ctx_profile_only(self.h, arg))
 
     def profile_read(self):
         """{kernel name: (total ms, launches)} since the last read (synchronises)."""
