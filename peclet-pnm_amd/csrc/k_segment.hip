@@ -76,7 +76,7 @@ constexpr int FR_X = 64 + 2 * FLOOD_SWEEPS, FR_Y = TILE_Y + 2 * FLOOD_SWEEPS,
               FR_Z = TILE_Z + 2 * FLOOD_SWEEPS;
 constexpr int FR_N = FR_X * FR_Y * FR_Z;
 constexpr int FR_SEG = 64 * TILE_Z;  // staging slots per wave
-constexpr int FLOOD_LIST = 4096;     // LDS list of band candidates per tile
+constexpr int FLOOD_LIST = 3072;     // LDS list of band candidates per tile (3 blocks/CU)
 // region word: >= 0 label, -1 band candidate, -2 other unlabelled, <= -3 labelled this sweep
 
 __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restrict__ sdf,
@@ -100,65 +100,53 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   constexpr int H = FLOOD_SWEEPS;
   // load: labels (a marker's component is L[L[i]]: its tile-local root points at the component
-  // root) and band candidates. Rows of the region are spread over the waves; a lane holds
-  // columns lane and (lanes < 2H) 64 + lane, whose wrapped x is fixed.
+  // root) and band candidates. Rows of the region are spread over the waves (row arithmetic in
+  // scalar registers), a lane holding column lane; the 2H halo columns of every row are one
+  // more load chain on the first threads, issued with the first batch. Per batch: the raw
+  // labels and values, then the component roots, then one LDS store. Loads are issued in
+  // branch-free batches of RB rows into registers (rows past the region re-read a valid row).
   const int rows = FR_Y * (nzt + 2 * H);
   const int gx0 = wrap_any(x0 - H + lane, nx);
-  const bool two = lane < 2 * H;
-  const int gx1 = two ? wrap_any(x0 - H + 64 + lane, nx) : gx0;
-  // pass 1: raw labels (tile-local root or -1) and band candidates; pass 2: local root ->
-  // component root. Loads are issued in branch-free batches of RB rows into registers (rows
-  // past the region re-read a valid row), so RB rows of loads are in flight per thread.
-  constexpr int RB = 15;
+  constexpr int RB = 12;
   constexpr int MAXR = (FR_Y * FR_Z + TILE_Y - 1) / TILE_Y;  // rows per wave, full tile
   const bool near = ny > TILE_Y && nz > 2;  // halo coordinates lie in [-N, 2N)
+  auto row_base = [&](int row) {
+    const int ry = row % FR_Y, rz = row / FR_Y;
+    const int gy = near ? wrap_near(y0 - H + ry, ny) : wrap_any(y0 - H + ry, ny);
+    const int gz = near ? wrap_near(z0 - H + rz, nz) : wrap_any(z0 - H + rz, nz);
+    return ((int64_t)gz * ny + gy) * nx;
+  };
+  auto region_word = [](int l, int root, float v) {
+    return l >= 0 ? root : (!(v >= 0.0f) ? -1 : -2);
+  };
+  const int swave = __builtin_amdgcn_readfirstlane(wave);
+  const int nhalo = rows * 2 * H;  // <= 2H * FR_Y * FR_Z < NT
+  const int hrow = tid < nhalo ? tid / (2 * H) : 0, hcol = tid % (2 * H);
+  const int64_t gh = row_base(hrow) + wrap_any(x0 - H + 64 + hcol, nx);
+#pragma unroll 1
   for (int j0 = 0; j0 < MAXR; j0 += RB) {
-    int la[RB], lb[RB];
-    float va[RB], vb[RB];
+    const bool halo = j0 == 0;
+    int la[RB];
+    float va[RB];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const int row0 = wave + (j0 + j) * TILE_Y;
-      const int row = row0 < rows ? row0 : wave;
-      const int ry = row % FR_Y, rz = row / FR_Y;
-      const int gy = near ? wrap_near(y0 - H + ry, ny) : wrap_any(y0 - H + ry, ny);
-      const int gz = near ? wrap_near(z0 - H + rz, nz) : wrap_any(z0 - H + rz, nz);
-      const int64_t gb = ((int64_t)gz * ny + gy) * nx;
-      la[j] = L[gb + gx0];
-      va[j] = sdf[gb + gx0];
-      lb[j] = L[gb + gx1];
-      vb[j] = sdf[gb + gx1];
+      const int row0 = swave + (j0 + j) * TILE_Y;
+      const int64_t g = row_base(row0 < rows ? row0 : swave) + gx0;
+      la[j] = L[g];
+      va[j] = sdf[g];
     }
+    const int lh = halo ? L[gh] : -1;
+    const float vh = halo ? sdf[gh] : 0.0f;
+    int ra[RB];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) ra[j] = L[la[j] >= 0 ? la[j] : 0];
+    const int rh = halo ? L[lh >= 0 ? lh : 0] : -1;
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const int row = wave + (j0 + j) * TILE_Y;
-      if (row < rows) {
-        R[row * FR_X + lane] = la[j] >= 0 ? la[j] : (!(va[j] >= 0.0f) ? -1 : -2);
-        if (two) R[row * FR_X + 64 + lane] = lb[j] >= 0 ? lb[j] : (!(vb[j] >= 0.0f) ? -1 : -2);
-      }
+      const int row = swave + (j0 + j) * TILE_Y;
+      if (row < rows) R[row * FR_X + lane] = region_word(la[j], ra[j], va[j]);
     }
-  }
-  for (int j0 = 0; j0 < MAXR; j0 += RB) {
-    int ka[RB], kb[RB], ra[RB], rb[RB];
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const int row0 = wave + (j0 + j) * TILE_Y;
-      const int row = row0 < rows ? row0 : wave;
-      ka[j] = R[row * FR_X + lane];
-      kb[j] = two ? R[row * FR_X + 64 + lane] : -2;
-    }
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      ra[j] = L[ka[j] >= 0 ? ka[j] : 0];
-      rb[j] = L[kb[j] >= 0 ? kb[j] : 0];
-    }
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const int row = wave + (j0 + j) * TILE_Y;
-      if (row < rows) {
-        if (ka[j] >= 0) R[row * FR_X + lane] = ra[j];
-        if (two && kb[j] >= 0) R[row * FR_X + 64 + lane] = rb[j];
-      }
-    }
+    if (halo && tid < nhalo) R[hrow * FR_X + 64 + hcol] = region_word(lh, rh, vh);
   }
   __syncthreads();
   // which of this thread's tile voxels start unlabelled (flooded voxels lower the minimum)
