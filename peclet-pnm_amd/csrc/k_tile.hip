@@ -2,8 +2,9 @@
 //
 // A 512-thread block owns a 64(x) x 8(y) x 16(z) tile. It marches the tile's slices in z,
 // computing for every voxel the 26-neighbour lexicographic (value, index) maximum
-// (pore_extraction.hpp:93-104, :280-304) by the separable key maximum (lex_key in
-// k_common.hpp), and keeps two tile-sized arrays in LDS:
+// (pore_extraction.hpp:93-104, :280-304) as the maximum of ordered 64-bit keys (lex_key in
+// k_common.hpp) over the 3x3 column neighbourhood of three slices, and keeps two tile-sized
+// arrays in LDS:
 //   LAB: marker labels (init_markers :177), first as the start of the voxel's x-run;
 //   DIR: for sdf>0 voxels, the direction (0..26, 13 = itself) of the ascent successor.
 // After the march it finishes, without leaving LDS,
@@ -18,8 +19,8 @@
 //     voxels a walk can enter the tile through — so a resolver hops tile to tile;
 //   - the first-appearance minimum of each in-tile root over the tile's final walks (relabel_min
 //     :333-334), stored at the root: this seeds min_pore for every sdf>0 peak.
-// The extract_pores peak predicate (:87-105) is staged per wave; marker roots seed the solid
-// first-appearance minimum.
+// The extract_pores peak predicate (:87-105) goes into a bitmap of peak voxels; marker roots seed
+// the solid first-appearance minimum.
 #pragma clang fp contract(off)
 
 #include "k_common.hpp"
@@ -32,7 +33,7 @@ constexpr int T_SEG = 64 * TILE_Z;  // staging slots per wave
 // DIR byte: bits 0-4 the ascent direction of an sdf>0 voxel (0..26, 13 = itself; kNoDir for
 // other voxels), bit 5 the extract_pores peak predicate.
 constexpr uint8_t kNoDir = 31;
-constexpr uint8_t kDirMask = 31, kPeakBit = 32;
+constexpr uint8_t kDirMask = 31;
 
 __device__ __forceinline__ int lds_find(int* L, int x) {
   int p = L[x];
@@ -105,11 +106,10 @@ __device__ __forceinline__ void dir_delta(int d, int& dx, int& dy, int& dz) {
   dx = d % 3 - 1;
 }
 
-__global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ sdf, int nx,
-                                                      int ny, int nz, float thr, TileOut o) {
-  __shared__ uint64_t K[T_LY][T_LX];
-  __shared__ uint64_t XM[T_LY][64];
-  __shared__ uint8_t XD[T_LY][64];
+// Three 512-thread blocks per CU need 6 waves per SIMD, i.e. at most 80 VGPRs.
+__global__ __launch_bounds__(64 * TILE_Y) __attribute__((amdgpu_waves_per_eu(6)))
+void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, TileOut o) {
+  __shared__ uint64_t K[2][T_LY][T_LX];  // keys of the haloed slice, two slices in rotation
   __shared__ int LAB[T_VOX];
   __shared__ uint8_t DIR[T_VOX];
   __shared__ uint64_t MASK[TILE_Z * TILE_Y];  // marker mask of each tile row
@@ -154,30 +154,33 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     b = plane[off1];
   };
   // 3x3 (x, y) key maximum of this thread's column at slice zg (values a, b already loaded),
-  // with the winner's (dy, dx) as dy*3+dx, and the centre key.
+  // with the winner's (dy, dx) as dy*3+dx, and the centre key. Slices alternate between the two
+  // key buffers, so one barrier per slice suffices: a thread overwrites a buffer two slices
+  // later, after the barrier of the slice in between, which every thread reaches only once it
+  // has read the buffer. The scan runs in (dy, dx) row-major order with a strict comparison,
+  // so among equal keys (a voxel met twice through a periodic wrap) the first position wins.
   auto slice_max = [&](int zg, float a, float b, uint64_t& centre, int& code) -> uint64_t {
     const uint32_t zbase = (uint32_t)wrap_near(zg, nz) * (uint32_t)plane_n;
-    (&K[0][0])[tid] = lex_key(a, zbase + (uint32_t)off0);
-    if (has1) (&K[0][0])[tid + 64 * TILE_Y] = lex_key(b, zbase + (uint32_t)off1);
+    const int pb = zg & 1;
+    uint64_t* Kp = &K[pb][0][0];
+    Kp[tid] = lex_key(a, zbase + (uint32_t)off0);
+    if (has1) Kp[tid + 64 * TILE_Y] = lex_key(b, zbase + (uint32_t)off1);
     __syncthreads();
-    for (int e = tid; e < T_LY * 64; e += 64 * TILE_Y) {
-      const int ly = e >> 6, lx = e & 63;
-      uint64_t m = K[ly][lx];
-      uint8_t d = 0;
-      const uint64_t k1 = K[ly][lx + 1], k2 = K[ly][lx + 2];
-      if (k1 > m) { m = k1; d = 1; }
-      if (k2 > m) { m = k2; d = 2; }
-      XM[ly][lx] = m;
-      XD[ly][lx] = d;
-    }
-    centre = K[wave + 1][lane + 1];
-    __syncthreads();
-    uint64_t m = XM[wave][lane];
-    int dy = 0;
-    const uint64_t m1 = XM[wave + 1][lane], m2 = XM[wave + 2][lane];
-    if (m1 > m) { m = m1; dy = 1; }
-    if (m2 > m) { m = m2; dy = 2; }
-    code = dy * 3 + XD[wave + dy][lane];
+    uint64_t m = K[pb][wave][lane];
+    int c = 0;
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        if (dy == 0 && dx == 0) continue;
+        const uint64_t k = K[pb][wave + dy][lane + dx];
+        if (dy == 1 && dx == 1) centre = k;
+        if (k > m) {
+          m = k;
+          c = dy * 3 + dx;
+        }
+      }
+    code = c;
     return m;
   };
 
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   load_slice(z0 + 2, A0, A1);
   uint64_t m_cur = slice_max(z0, B0, B1, c_cur, d_cur);
   load_slice(z0 + 3, B0, B1);
-  uint32_t pore_mask = 0;
+  uint32_t pore_mask = 0, peak_mask = 0;  // this thread's sdf>0 slices and peak slices
   // one slice of the march: needs slice k = zl + 2 (values va, vb)
   auto march = [&](int zl, float va, float vb) {
     const int z = z0 + zl;
@@ -227,7 +230,8 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     const int st = smask ? 63 - __clzll((long long)smask) : lane;
     LAB[l] = mk ? l - (lane - st) : -1;
     if (lane == 0) MASK[zl * TILE_Y + wave] = mm;
-    DIR[l] = dir | (is_peak ? kPeakBit : 0);
+    DIR[l] = dir;
+    if (is_peak) peak_mask |= 1u << zl;
     (void)i;
     m_prev = m_cur;
     m_cur = m_next;
@@ -249,14 +253,11 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   __syncthreads();
   // peak masks: one word per (row, 64-column chunk), rows in flat order, so the set bits of the
   // whole map in word order are the peaks in ascending voxel order
-  if (y < ny) {
-    const int64_t nwx = gridDim.x;
-    for (int zl = 0; zl < nzt; ++zl) {
-      const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const bool is_peak = col_in && (DIR[l] & kPeakBit);
-      const uint64_t pm = __ballot(is_peak);
-      if (lane == 0) o.peak_map[((int64_t)(z0 + zl) * ny + y) * nwx + blockIdx.x] = pm;
-    }
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    const uint64_t pm = __ballot((peak_mask >> zl) & 1u);
+    if (lane == 0 && zl < nzt && y < ny)
+      o.peak_map[((int64_t)(z0 + zl) * ny + y) * gridDim.x + blockIdx.x] = pm;
   }
 
   // ---- CCL inside the tile: forward rows (y+1,z), (y-1,z+1), (y,z+1), (y+1,z+1) ----
@@ -277,83 +278,93 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
     }
   }
   __syncthreads();
-  // flatten: a read-only chase and one store of this thread's own entry. (A find with path
-  // halving here would write ancestors into other voxels' entries, and such a write landing after
-  // that voxel's owner stored its root leaves an intermediate node as its label.)
-  for (int zl = 0; zl < nzt; ++zl) {
-    const int l = (zl * TILE_Y + wave) * 64 + lane;
-    int r = LAB[l];
-    if (r != -1) {
-      int p = LAB[r];
-      while (p != r) {
-        r = p;
-        p = LAB[r];
+  // flatten: each thread chases the roots of its 16 voxels side by side (LDS reads of all 16
+  // chains in flight together), reading only: the forest stays as the unions left it, so no
+  // store can race with a chase (a find with path halving would write ancestors into other
+  // voxels' entries, and such a write landing after that voxel's owner stored its root would
+  // leave an intermediate node as its label). The results go straight out: the label, the
+  // solid first-appearance seed of a component root (a component's root is its smallest
+  // marker, one of the tile-local roots, seeded before any flood can lower it), the staged
+  // tile-local roots (k_lroot_flatten resolves them after the seams) and the x-face labels of
+  // the row chunk (first and last column; entry (chunk, z, y), compact for the seam pass).
+  {
+    int r[TILE_Z], p[TILE_Z];
+#pragma unroll
+    for (int zl = 0; zl < TILE_Z; ++zl) r[zl] = zl < nzt ? LAB[(zl * TILE_Y + wave) * 64 + lane] : -1;
+#pragma unroll
+    for (int zl = 0; zl < TILE_Z; ++zl) p[zl] = LAB[r[zl] >= 0 ? r[zl] : 0];
+    bool more = true;
+    while (more) {
+      more = false;
+#pragma unroll
+      for (int zl = 0; zl < TILE_Z; ++zl) {
+        const bool go = r[zl] >= 0 && p[zl] != r[zl];
+        r[zl] = go ? p[zl] : r[zl];
+        more |= go;
       }
-      LAB[l] = r;
+      if (!more) break;
+#pragma unroll
+      for (int zl = 0; zl < TILE_Z; ++zl) p[zl] = LAB[r[zl] >= 0 ? r[zl] : 0];
     }
-  }
-  __syncthreads();
-  if (col_in) {
-    for (int zl = 0; zl < nzt; ++zl) {
-      const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const int a = LAB[l];
-      const int i = ((z0 + zl) * ny + y) * nx + x;
-      o.labels[i] = a == -1 ? -1 : gidx(a);
-      // a component's root is its smallest marker, one of the tile-local roots: seed the
-      // solid first-appearance minimum there before any flood can lower it
-      if (a == l) o.min_solid[i] = i;
-    }
-  }
-  // x-face labels of each row chunk (first and last column), compact for the seam pass: entry
-  // (chunk, z, y); the 8 waves of the tile fill 64 contiguous bytes per slice
-  if (y < ny && lane < nzt) {
-    const int rb = (lane * TILE_Y + wave) * 64;
-    const int a0 = LAB[rb], aL = LAB[rb + xlast - x0];
-    int2 v;
-    v.x = a0 == -1 ? -1 : gidx(a0);
-    v.y = aL == -1 ? -1 : gidx(aL);
-    o.xface[((int64_t)blockIdx.x * nz + z0 + lane) * ny + y] = v;
-  }
-  {  // stage the tile-local roots (k_lroot_flatten resolves them after the seams)
     unsigned nr = 0;
-    for (int zl = 0; zl < nzt; ++zl) {
+#pragma unroll
+    for (int zl = 0; zl < TILE_Z; ++zl) {
+      if (zl >= nzt) break;
       const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const bool r = col_in && LAB[l] == l;
-      const uint64_t rm = __ballot(r);
-      if (r) o.lroot_stage[seg_id * T_SEG + nr + __popcll(rm & below)] = (uint32_t)gidx(l);
+      const int i = ((z0 + zl) * ny + y) * nx + x;
+      if (col_in) {
+        o.labels[i] = r[zl] == -1 ? -1 : gidx(r[zl]);
+        if (r[zl] == l) o.min_solid[i] = i;
+      }
+      const bool lroot = col_in && r[zl] == l;
+      const uint64_t rm = __ballot(lroot);
+      if (lroot) o.lroot_stage[seg_id * T_SEG + nr + __popcll(rm & below)] = (uint32_t)gidx(l);
       nr += (unsigned)__popcll(rm);
+      const int a0 = __shfl(r[zl], 0), aL = __shfl(r[zl], xlast - x0);
+      if (lane == 0 && y < ny) {
+        int2 v;
+        v.x = a0 == -1 ? -1 : gidx(a0);
+        v.y = aL == -1 ? -1 : gidx(aL);
+        o.xface[((int64_t)blockIdx.x * nz + z0 + zl) * ny + y] = v;
+      }
     }
     if (lane == 0) o.lroot_cnt[seg_id] = nr;
   }
-  __syncthreads();
 
   // ---- ascent walks inside the tile: pointer jumping on LAB (now the chase state) ----
+  // The chase state goes into the LAB words of sdf>0 voxels, which no marker chase above reads
+  // (marker labels point at markers only), so no barrier is needed before it.
   auto in_tile = [&](int ax, int ay, int az) {
     return ax >= 0 && ax < 64 && x0 + ax < nx && ay >= 0 && ay < TILE_Y && y0 + ay < ny &&
            az >= 0 && az < nzt;
   };
-  // Each thread walks only the set bits of its masks: its sdf>0 slices, then the slices whose
-  // walk is still active (the wave's trip count is its busiest lane's, not 16).
+  // Each thread walks only its sdf>0 slices and then only the slices whose walk is still active.
   uint32_t act = 0;
-  for (uint32_t m = pore_mask; m; m &= m - 1) {
-    const int zl = __ffs(m) - 1;
-    const int l = (zl * TILE_Y + wave) * 64 + lane;
-    const int d = DIR[l] & kDirMask;
-    uint32_t w;
-    if (d == 13) {
-      w = cs_pack(l, 0, CS_FIX);
-    } else {
-      int dx, dy, dz;
-      dir_delta(d, dx, dy, dz);
-      const int ax = lane + dx, ay = wave + dy, az = zl + dz;
-      if (in_tile(ax, ay, az)) {
-        w = cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT);
-        act |= 1u << zl;
+  {
+    uint8_t dv[TILE_Z];
+#pragma unroll
+    for (int zl = 0; zl < TILE_Z; ++zl) dv[zl] = DIR[(zl * TILE_Y + wave) * 64 + lane];
+#pragma unroll
+    for (int zl = 0; zl < TILE_Z; ++zl) {
+      if (!((pore_mask >> zl) & 1u)) continue;
+      const int l = (zl * TILE_Y + wave) * 64 + lane;
+      const int d = dv[zl] & kDirMask;
+      uint32_t w;
+      if (d == 13) {
+        w = cs_pack(l, 0, CS_FIX);
       } else {
-        w = cs_pack(l, 0, CS_EXIT);
+        int dx, dy, dz;
+        dir_delta(d, dx, dy, dz);
+        const int ax = lane + dx, ay = wave + dy, az = zl + dz;
+        if (in_tile(ax, ay, az)) {
+          w = cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT);
+          act |= 1u << zl;
+        } else {
+          w = cs_pack(l, 0, CS_EXIT);
+        }
       }
+      LAB[l] = (int)w;
     }
-    LAB[l] = (int)w;
   }
   __syncthreads();
   while (true) {
@@ -376,97 +387,128 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_tile(const float* __restrict__ 
   }
 
   // ---- walk outputs; LAB at each fixed point collects the smallest local index whose walk
-  //      ends there (every other LAB word read below belongs to the reading thread) ----
+  //      ends there (every other LAB word read below is read by the one lane handling it) ----
   bool capped_any = false;
-  uint32_t open_mask = 0;
-  for (uint32_t m = pore_mask; m; m &= m - 1) {
-    const int zl = __ffs(m) - 1;
-    const int l = (zl * TILE_Y + wave) * 64 + lane;
-    const int i = ((z0 + zl) * ny + y) * nx + x;
-    {
-      int target, steps, fin_u = -1;
-      bool fin = false, capped = false;
-      const bool root_here = (DIR[l] & kDirMask) == 13;
-      const uint32_t w = root_here ? cs_pack(l, 0, CS_FIX) : (uint32_t)LAB[l];
-      const int u = cs_u(w);
-      steps = cs_s(w);
-      if (cs_st(w) == CS_FIX) {
+  // Output of the walk of sdf>0 voxel l (slice zl, column xl of this wave's row); returns
+  // whether the walk is still open.
+  auto walk_out = [&](int l, int zl, int xl) -> bool {
+    const int xg = x0 + xl;
+    const int i = ((z0 + zl) * ny + y) * nx + xg;
+    int target, steps, fin_u = -1;
+    bool fin = false, capped = false;
+    const bool root_here = (DIR[l] & kDirMask) == 13;
+    const uint32_t w = root_here ? cs_pack(l, 0, CS_FIX) : (uint32_t)LAB[l];
+    const int u = cs_u(w);
+    steps = cs_s(w);
+    if (cs_st(w) == CS_FIX) {
+      fin = true;
+      fin_u = u;
+      target = gidx(u);
+    } else if (cs_st(w) == CS_EXIT) {
+      if (steps == kMaxSteps) {  // the 512th step lands on u, whose successor is outside
         fin = true;
-        fin_u = u;
+        capped = true;
         target = gidx(u);
-      } else if (cs_st(w) == CS_EXIT) {
-        if (steps == kMaxSteps) {  // the 512th step lands on u, whose successor is outside
+      } else {
+        int dx, dy, dz;
+        dir_delta(DIR[u] & kDirMask, dx, dy, dz);
+        const int gx = wrap_near(x0 + (u & 63) + dx, nx);
+        const int gy = wrap_near(y0 + ((u >> 6) & 7) + dy, ny);
+        const int gz = wrap_near(z0 + (u >> 9) + dz, nz);
+        target = (gz * ny + gy) * nx + gx;
+        steps += 1;
+      }
+    } else {  // CS_LONG: walk the 512 steps one by one
+      int cx = xl, cy = wave, cz = zl, cl = l;
+      steps = 0;
+      while (true) {
+        const int d = DIR[cl] & kDirMask;
+        if (d == 13) {
           fin = true;
-          capped = true;
-          target = gidx(u);
-        } else {
-          int dx, dy, dz;
-          dir_delta(DIR[u] & kDirMask, dx, dy, dz);
-          const int gx = wrap_near(x0 + (u & 63) + dx, nx);
-          const int gy = wrap_near(y0 + ((u >> 6) & 7) + dy, ny);
-          const int gz = wrap_near(z0 + (u >> 9) + dz, nz);
-          target = (gz * ny + gy) * nx + gx;
-          steps += 1;
+          fin_u = cl;
+          target = gidx(cl);
+          break;
         }
-      } else {  // CS_LONG: walk the 512 steps one by one
-        int cx = lane, cy = wave, cz = zl, cl = l;
-        steps = 0;
-        while (true) {
-          const int d = DIR[cl] & kDirMask;
-          if (d == 13) {
-            fin = true;
-            fin_u = cl;
-            target = gidx(cl);
-            break;
-          }
-          int dx, dy, dz;
-          dir_delta(d, dx, dy, dz);
-          const int ax = cx + dx, ay = cy + dy, az = cz + dz;
-          ++steps;
-          if (!in_tile(ax, ay, az)) {
-            target = (wrap_near(z0 + az, nz) * ny + wrap_near(y0 + ay, ny)) * nx +
-                     wrap_near(x0 + ax, nx);
-            break;
-          }
-          cx = ax;
-          cy = ay;
-          cz = az;
-          cl = (az * TILE_Y + ay) * 64 + ax;
-          if (steps == kMaxSteps) {
-            fin = true;
-            capped = (DIR[cl] & kDirMask) != 13;
-            if (!capped) fin_u = cl;
-            target = gidx(cl);
-            break;
-          }
+        int dx, dy, dz;
+        dir_delta(d, dx, dy, dz);
+        const int ax = cx + dx, ay = cy + dy, az = cz + dz;
+        ++steps;
+        if (!in_tile(ax, ay, az)) {
+          target = (wrap_near(z0 + az, nz) * ny + wrap_near(y0 + ay, ny)) * nx +
+                   wrap_near(x0 + ax, nx);
+          break;
+        }
+        cx = ax;
+        cy = ay;
+        cz = az;
+        cl = (az * TILE_Y + ay) * 64 + ax;
+        if (steps == kMaxSteps) {
+          fin = true;
+          capped = (DIR[cl] & kDirMask) != 13;
+          if (!capped) fin_u = cl;
+          target = gidx(cl);
+          break;
         }
       }
-      if (fin) {
-        o.roots[i] = target;
-        if (!capped) atomicMin(&LAB[fin_u], l);
-        capped_any |= capped;
-      }
-      if (!fin) open_mask |= 1u << zl;
-      const bool boundary = x == x0 || x == xlast || y == y0 || y == ylast || zl == 0 ||
-                            zl == nzt - 1;
-      if (!fin || boundary) o.walk[i] = walk_record(target, steps, fin, capped);
     }
+    if (fin) {
+      o.roots[i] = target;
+      if (!capped) atomicMin(&LAB[fin_u], l);
+      capped_any |= capped;
+    }
+    const bool boundary = xg == x0 || xg == xlast || y == y0 || y == ylast || zl == 0 ||
+                          zl == nzt - 1;
+    if (!fin || boundary) o.walk[i] = walk_record(target, steps, fin, capped);
+    return !fin;
+  };
+  // The wave's sdf>0 voxels as a compact list in (slice, lane) order, in the key buffers (free
+  // after the march): the wave then takes ceil(count / 64) turns instead of its busiest lane's
+  // count of slices. A wave with more voxels than its share of the buffers uses the per-lane
+  // loop. Open walks are staged in the same (slice, lane) order either way.
+  constexpr int PLIST_W = (int)(sizeof(K) / sizeof(uint16_t)) / TILE_Y;
+  uint16_t* plist = reinterpret_cast<uint16_t*>(&K[0][0][0]) + wave * PLIST_W;
+  unsigned np = 0;
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    const bool pb = (pore_mask >> zl) & 1u;
+    const uint64_t pm = __ballot(pb);
+    const unsigned pos = np + (unsigned)__popcll(pm & below);
+    if (pb && pos < (unsigned)PLIST_W) plist[pos] = (uint16_t)((zl * TILE_Y + wave) * 64 + lane);
+    np += (unsigned)__popcll(pm);
   }
-  // stage the open walks in (slice, lane) order
-  for (int zl = 0; zl < nzt; ++zl) {
-    const bool open = (open_mask >> zl) & 1u;
-    const uint64_t om = __ballot(open);
-    if (open)
-      o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
-          (uint32_t)(((z0 + zl) * ny + y) * nx + x);
-    nop += (unsigned)__popcll(om);
+  if (np <= (unsigned)PLIST_W) {
+    for (unsigned j0 = 0; j0 < np; j0 += 64) {
+      const unsigned j = j0 + lane;
+      const int li = j < np ? (int)plist[j] : 0;
+      const bool open = j < np && walk_out(li, li >> 9, li & 63);
+      const uint64_t om = __ballot(open);
+      if (open)
+        o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
+            (uint32_t)(((z0 + (li >> 9)) * ny + y) * nx + x0 + (li & 63));
+      nop += (unsigned)__popcll(om);
+    }
+  } else {
+    uint32_t open_mask = 0;
+    for (uint32_t m = pore_mask; m; m &= m - 1) {
+      const int zl = __ffs(m) - 1;
+      if (walk_out((zl * TILE_Y + wave) * 64 + lane, zl, lane)) open_mask |= 1u << zl;
+    }
+    for (int zl = 0; zl < nzt; ++zl) {
+      const bool open = (open_mask >> zl) & 1u;
+      const uint64_t om = __ballot(open);
+      if (open)
+        o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
+            (uint32_t)(((z0 + zl) * ny + y) * nx + x);
+      nop += (unsigned)__popcll(om);
+    }
   }
   if (__any(capped_any) && lane == 0) atomicOr(o.cap_flag, 1ull);
   __syncthreads();
-  for (uint32_t m = pore_mask; m; m &= m - 1) {
+  // roots of this thread's column (sdf>0 peaks): min_pore = the smallest walker's voxel
+  for (uint32_t m = pore_mask & peak_mask; m; m &= m - 1) {
     const int zl = __ffs(m) - 1;
     const int l = (zl * TILE_Y + wave) * 64 + lane;
-    if ((DIR[l] & kDirMask) == 13) o.min_pore[gidx(l)] = gidx(LAB[l]);
+    o.min_pore[gidx(l)] = gidx(LAB[l]);
   }
   if (lane == 0) o.open_cnt[seg_id] = nop;
 }
