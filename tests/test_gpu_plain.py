@@ -106,8 +106,10 @@ def quantized(shape, seed, step=0.5, nan_frac=0.0, inf_frac=0.0):
     return v.astype(np.float32)
 
 
+# (20, 17, 65) and (34, 25, 129): several tiles in y and z with a last row chunk one column wide
+# (its first lane is also its last one in the seam passes).
 SHAPES = [(1, 1, 1), (1, 1, 2), (2, 2, 2), (1, 3, 70), (3, 5, 65), (5, 7, 63), (4, 9, 128),
-          (17, 13, 11), (33, 2, 130), (8, 64, 8)]
+          (17, 13, 11), (33, 2, 130), (8, 64, 8), (20, 17, 65), (34, 25, 129)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
