@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
                                                    uint64_t* __restrict__ overflow,
                                                    unsigned long long* ocount) {
   __shared__ unsigned long long table[TOPO_TABLE];
-  __shared__ int S[FUSED ? TB_X * TB_Y * TB_Z : 1];
+  __shared__ int S[FUSED ? TB_X * (TB_Y * TB_Z + 1) : 1];  // + a spare row (see below)
+  __shared__ uint32_t RBASE[FUSED ? TB_Y * TB_Z : 1];  // flat offset of each region row
   __shared__ unsigned wc[4];
   __shared__ unsigned s_base;
   const int tid = threadIdx.y * 64 + threadIdx.x;
@@ -56,6 +57,13 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   for (int s = tid; s < TOPO_TABLE; s += 256) table[s] = kEmpty;
   if (tid == 0) s_base = 0;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, z0 = blockIdx.z * TOPO_ZB;
+  constexpr int NROW = TB_Y * TB_Z;
+  if (FUSED && tid < NROW) {  // wrapped flat offset of each row of the region, once per block
+    const int ry = tid % TB_Y, rz = tid / TB_Y;
+    RBASE[tid] = ((uint32_t)wrap_any(z0 + rz, nz) * (uint32_t)ny + (uint32_t)wrap_any(y0 + ry, ny)) *
+                 (uint32_t)nx;
+  }
+  if (FUSED) __syncthreads();
   if (FUSED) {
     // Rows of the brick + halo over the 4 waves, lane = column; the halo column 64 of the 85 rows
     // is one more chain on threads 0..84, issued with the first batch. Positions past the volume
@@ -63,16 +71,11 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
     // Brick voxels inside the volume also write seg.
     // Each value's sign picks the label source (root of an sdf>0 voxel, flagged in the top bit,
     // or the solid label), the label gives the id. Loads are issued in branch-free batches of RB
-    // rows into registers (rows past the region re-read a valid row).
-    constexpr int NROW = TB_Y * TB_Z, RB = 11, MAXR = (NROW + 3) / 4;
+    // rows into registers (rows past the region re-read a valid row). Row offsets come from the
+    // table (flat offsets fit 32 bits: volumes hold fewer than 2^31 voxels).
+    constexpr int RB = 11, MAXR = (NROW + 3) / 4;
     const int gx = wrap_any(x0 + lane, nx), gxh = wrap_any(x0 + 64, nx);
-    const bool near = ny >= TB_Y && nz >= TB_Z;  // row coordinates lie in [0, 2N)
-    auto row_base = [&](int row) {
-      const int ry = row % TB_Y, rz = row / TB_Y;
-      const int wy = near ? wrap_near(y0 + ry, ny) : wrap_any(y0 + ry, ny);
-      const int wz = near ? wrap_near(z0 + rz, nz) : wrap_any(z0 + rz, nz);
-      return ((int64_t)wz * ny + wy) * nx;
-    };
+    auto row_base = [&](int row) -> int64_t { return (int64_t)RBASE[row]; };
     auto label_of = [&](float v, int64_t g) {
       const int32_t* src = v > 0.0f ? ids.roots : ids.labels;
       const int r = src[g];
@@ -97,23 +100,25 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
         v[j] = ids.sdf[g[j]];
       }
       const int64_t gh = row_base(hrow) + gxh;
-      const float vh = halo ? ids.sdf[gh] : 0.0f;
+      const float vh = ids.sdf[gh];  // both batches load it (branch-free; cached the second time)
       int k[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) k[j] = label_of(v[j], g[j]);
-      const int kh = halo ? label_of(vh, gh) : -1;
+      const int kh = label_of(vh, gh);
 #pragma unroll
       for (int j = 0; j < RB; ++j) k[j] = id_of(k[j]);
-      const int sh = halo ? id_of(kh) : 0;
+      const int sh = id_of(kh);
+      // The LDS store is unconditional (rows past the region go to the spare row): a store under
+      // a branch lets the compiler sink that row's loads into the branch, one wait per load.
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
-        const int row = wave + (j0 + j) * 4;
-        if (row < NROW) {
-          const int ry = row % TB_Y, rz = row / TB_Y;
-          S[row * TB_X + lane] = k[j];
-          if (ry < 4 && rz < TOPO_ZB && x0 + lane < nx && y0 + ry < ny && z0 + rz < nz)
-            ids.seg_out[((int64_t)(z0 + rz) * ny + (y0 + ry)) * nx + gx] = k[j];
-        }
+        const int row = swave + (j0 + j) * 4;
+        S[(row < NROW ? row : NROW) * TB_X + lane] = k[j];
+        const int ry = row % TB_Y, rz = row / TB_Y;
+        // a brick row inside the volume is its own unwrapped row: g is the voxel's flat index
+        if (row < NROW && ry < 4 && rz < TOPO_ZB && x0 + lane < nx && y0 + ry < ny &&
+            z0 + rz < nz)
+          ids.seg_out[g[j]] = k[j];
       }
       if (halo && tid < NROW) S[tid * TB_X + 64] = sh;
     }
