@@ -100,22 +100,27 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   constexpr int H = FLOOD_SWEEPS;
   // load: labels (a marker's component is L[L[i]]: its tile-local root points at the component
-  // root) and band candidates. Rows of the region are spread over the waves (row arithmetic in
-  // scalar registers), a lane holding column lane; the 2H halo columns of every row are one
-  // more load chain on the first threads, issued with the first batch. Per batch: the raw
-  // labels and values, then the component roots, then one LDS store. Loads are issued in
-  // branch-free batches of RB rows into registers (rows past the region re-read a valid row).
+  // root) and band candidates. Rows of the region are spread over the waves, a lane holding
+  // column lane; the 2H halo columns of every row are one more load chain on the first threads,
+  // issued with the first batch. Per batch: the raw labels and values, then the component
+  // roots, then one LDS store. Loads are issued in branch-free batches of RB rows into
+  // registers (rows past the region re-read a valid row). The wrapped flat offset of each
+  // region row comes from a table the block fills first, kept in the candidate list's LDS
+  // (not used before the sweeps); flat offsets fit 32 bits (volumes hold fewer than 2^31
+  // voxels).
   const int rows = FR_Y * (nzt + 2 * H);
   const int gx0 = wrap_any(x0 - H + lane, nx);
   constexpr int RB = 12;
   constexpr int MAXR = (FR_Y * FR_Z + TILE_Y - 1) / TILE_Y;  // rows per wave, full tile
-  const bool near = ny > TILE_Y && nz > 2;  // halo coordinates lie in [-N, 2N)
-  auto row_base = [&](int row) {
-    const int ry = row % FR_Y, rz = row / FR_Y;
-    const int gy = near ? wrap_near(y0 - H + ry, ny) : wrap_any(y0 - H + ry, ny);
-    const int gz = near ? wrap_near(z0 - H + rz, nz) : wrap_any(z0 - H + rz, nz);
-    return ((int64_t)gz * ny + gy) * nx;
-  };
+  uint32_t* rbase = reinterpret_cast<uint32_t*>(cand_list);
+  static_assert(sizeof(cand_list) >= sizeof(uint32_t) * FR_Y * FR_Z, "row table fits");
+  if (tid < rows) {
+    const int ry = tid % FR_Y, rz = tid / FR_Y;
+    rbase[tid] = ((uint32_t)wrap_any(z0 - H + rz, nz) * (uint32_t)ny +
+                  (uint32_t)wrap_any(y0 - H + ry, ny)) * (uint32_t)nx;
+  }
+  __syncthreads();
+  auto row_base = [&](int row) -> int64_t { return (int64_t)rbase[row]; };
   auto region_word = [](int l, int root, float v) {
     return l >= 0 ? root : (!(v >= 0.0f) ? -1 : -2);
   };
