@@ -129,14 +129,31 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   const bool in = x < nx && y < ny;
   const int xp = in ? wp1(x, nx) : 0, yp = in ? wp1(y, ny) : 0;
   // a face pair repeats along x (the previous lane) and along z (this lane's previous slice):
-  // both repeats skip the hash set
+  // both repeats skip the hash set. The keys are computed branch-free; a direction in which no
+  // lane of the wave has a new key is skipped as a whole; the first probe (the key found, or
+  // inserted into an empty slot) is straight-line code, and only lanes whose first slot held
+  // another key continue probing.
   uint64_t last[3] = {kEmpty, kEmpty, kEmpty};
+  auto insert_slow = [&](uint64_t key, uint32_t h) {
+#pragma unroll 1
+    for (int p = 1; p < TOPO_PROBES; ++p) {
+      h = (h + 1) & (TOPO_TABLE - 1);
+      const unsigned long long cur = table[h];
+      if (cur == key) return;
+      if (cur == kEmpty) {
+        const unsigned long long prv = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
+        if (prv == kEmpty || prv == key) return;
+      }
+    }
+    // crowded table: the global unique pass removes repeats
+    const unsigned long long sl = atomicAdd(ocount, 1ull);
+    overflow[sl] = key;
+  };
   for (int zz = 0; zz < TOPO_ZB; ++zz) {
     const int z = z0 + zz;
     const bool act = in && z < nz;
-    uint64_t key[3] = {kEmpty, kEmpty, kEmpty};
+    int my = 0, nb[3] = {0, 0, 0};
     if (act) {
-      int my, nb[3];
       if (FUSED) {
         const int e = (zz * TB_Y + wave) * TB_X + lane;
         my = S[e];
@@ -150,42 +167,29 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
         nb[1] = seg[(z * ny + yp) * nx + x];
         nb[2] = seg[(zp * ny + y) * nx + x];
       }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int nl = nb[k];
-        if (my == nl) continue;
-        const int l1 = my < nl ? my : nl, l2 = my < nl ? nl : my;
-        key[k] = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
-                 (uint64_t)((uint32_t)l2 - (uint32_t)lo);
-      }
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const uint64_t prev = __shfl_up(key[k], 1, 64);
-      const bool rep = key[k] == last[k];
-      last[k] = key[k];
-      if (key[k] == kEmpty || rep || (lane > 0 && prev == key[k])) continue;
-      uint32_t h = hash64(key[k]) & (TOPO_TABLE - 1);
-      bool placed = false;
-      for (int p = 0; p < TOPO_PROBES; ++p) {
+      const int nl = nb[k];
+      const int l1 = my < nl ? my : nl, l2 = my < nl ? nl : my;
+      const uint64_t kv = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
+                          (uint64_t)((uint32_t)l2 - (uint32_t)lo);
+      const uint64_t key = act && my != nl ? kv : kEmpty;
+      const uint64_t prev = __shfl_up(key, 1, 64);
+      const bool need = key != kEmpty && key != last[k] && !(lane > 0 && prev == key);
+      last[k] = key;
+      if (!__any(need)) continue;  // wave-uniform
+      const uint32_t h = hash64(key) & (TOPO_TABLE - 1);
+      bool placed = true;
+      if (need) {
         const unsigned long long cur = table[h];
-        if (cur == key[k]) {
-          placed = true;
-          break;
-        }
+        placed = cur == key;
         if (cur == kEmpty) {
-          const unsigned long long prv = atomicCAS(&table[h], kEmpty, (unsigned long long)key[k]);
-          if (prv == kEmpty || prv == key[k]) {
-            placed = true;
-            break;
-          }
+          const unsigned long long prv = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
+          placed = prv == kEmpty || prv == key;
         }
-        h = (h + 1) & (TOPO_TABLE - 1);
       }
-      if (!placed) {  // crowded table: the global unique pass removes repeats
-        const unsigned long long s = atomicAdd(ocount, 1ull);
-        overflow[s] = key[k];
-      }
+      if (__any(!placed) && !placed) insert_slow(key, h);
     }
   }
   __syncthreads();
