@@ -617,7 +617,12 @@ __global__ __launch_bounds__(256) void k_ccl_seams_rows(int* L, int nx, int ny, 
     bool same[4];
     seam_rows(y, z, ny, nz, fy, fz, same);
     if (same[0] && same[1] && same[2] && same[3]) continue;
+    // the row and its crossing forward rows are loaded together (rows not needed re-read the
+    // row itself), then used
     const int i = in ? (z * ny + y) * nx + x : 0;
+    int fwd[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) fwd[r] = L[(same[r] || !in) ? i : (fz[r] * ny + fy[r]) * nx + x];
     const int my = in ? L[i] : -1;
     const bool mk = my != -1;
     const uint64_t m = __ballot(mk);
@@ -628,7 +633,7 @@ __global__ __launch_bounds__(256) void k_ccl_seams_rows(int* L, int nx, int ny, 
     for (int r = 0; r < 4; ++r) {
       if (same[r]) continue;  // wave-uniform; edges of same-tile rows: k_ccl_seams_edges
       const int rb = (fz[r] * ny + fy[r]) * nx;
-      const int c = in ? L[rb + x] : -1;
+      const int c = in ? fwd[r] : -1;
       const bool cm = c != -1;
       const int prev_c = __shfl_up(c, 1, 64);
       if (mk && cm) {
