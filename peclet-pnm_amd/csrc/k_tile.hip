@@ -734,113 +734,118 @@ void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
 // Hops along the walk records: from the voxel a walk entered a tile through, that voxel's own
 // record gives where its walk ends inside that tile (or leaves it again) and after how many
 // steps, which is where the original walk goes too. When the remaining step budget ends inside a
-// tile, the rest is walked step by step on the SDF. Each thread advances RESOLVE_WALKS walks in
-// lockstep (walk j + k * threads), so that many independent record loads are in flight per
-// thread; the rare step-by-step remainders run after all hops.
+// tile, the rest is walked step by step on the SDF. The open walks are read where k_tile staged
+// them: one wave per staging segment (a tile row's open walks in (slice, lane) order). Each
+// thread advances RESOLVE_WALKS walks in lockstep, so that many independent record loads are in
+// flight per thread; the rare step-by-step remainders run after all hops.
 constexpr int RESOLVE_WALKS = 4;
 __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__ sdf, int nx,
                                                        int ny, int nz, int64_t n,
-                                                       const uint32_t* __restrict__ open,
-                                                       const uint64_t* __restrict__ count,
+                                                       const uint32_t* __restrict__ stage,
+                                                       const uint32_t* __restrict__ cnt,
+                                                       int64_t nseg, int seg_cap,
                                                        const uint64_t* __restrict__ walk,
                                                        int32_t* __restrict__ roots,
                                                        int32_t* __restrict__ min_pore,
                                                        unsigned long long* cap_flag) {
   constexpr int K = RESOLVE_WALKS;
-  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-  const int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t cnt = (int64_t)*count;
   const int lane = threadIdx.x & 63;
-  int vox[K], w[K], t[K], root[K];
-  uint32_t live = 0, slow = 0;
+  const int64_t seg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (seg >= nseg) return;  // wave-uniform; the kernel has no block barrier
+  const int count = (int)cnt[seg];
+  const uint32_t* __restrict__ list = stage + seg * seg_cap;
   unsigned long long flags = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int64_t jj = j0 + k * nthreads;
-    vox[k] = jj < cnt ? (int)open[jj] : 0;
-    root[k] = -1;
-    if (jj < cnt) live |= 1u << k;
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint64_t own = walk[vox[k]];  // vox = 0 for a missing walk: a valid, ignored load
-    w[k] = walk_target(own);
-    t[k] = walk_steps(own);
-  }
-  while (live) {
-    uint64_t rec[K];
+  for (int base = 0; base < count; base += 64 * K) {
+    int vox[K], w[K], t[K], root[K];
+    uint32_t live = 0, slow = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      // every hop target is an sdf>0 voxel on a tile boundary and carries a record; a target
-      // or record that breaks this is reported (bit 1) instead of followed
-      const bool go = ((live >> k) & 1u) && (uint64_t)(uint32_t)w[k] < (uint64_t)n;
-      rec[k] = walk[go ? w[k] : 0];
+      const int jj = base + k * 64 + lane;
+      vox[k] = jj < count ? (int)list[jj] : 0;
+      root[k] = -1;
+      if (jj < count) live |= 1u << k;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (!((live >> k) & 1u)) continue;
-      const uint32_t bit = 1u << k;
-      if ((uint64_t)(uint32_t)w[k] >= (uint64_t)n) {
-        flags |= 2ull;
-        live &= ~bit;
-        continue;
+      const uint64_t own = walk[vox[k]];  // vox = 0 for a missing walk: a valid, ignored load
+      w[k] = walk_target(own);
+      t[k] = walk_steps(own);
+    }
+    while (live) {
+      uint64_t rec[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        // every hop target is an sdf>0 voxel on a tile boundary and carries a record; a
+        // target or record that breaks this is reported (bit 1) instead of followed
+        const bool go = ((live >> k) & 1u) && (uint64_t)(uint32_t)w[k] < (uint64_t)n;
+        rec[k] = walk[go ? w[k] : 0];
       }
-      const int sw = walk_steps(rec[k]);
-      const bool fin = walk_final(rec[k]);
-      if (!fin && (sw == 0 || sw > kMaxSteps)) {
-        flags |= 2ull;
-        live &= ~bit;
-      } else if (t[k] == kMaxSteps) {  // the budget ran out exactly on entering this tile
-        root[k] = w[k];
-        if (!(fin && sw == 0)) flags |= 1ull;
-        live &= ~bit;
-      } else if (t[k] + sw <= kMaxSteps) {
-        t[k] += sw;
-        if (fin) {
-          root[k] = walk_target(rec[k]);
-          if (walk_capped(rec[k])) flags |= 1ull;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (!((live >> k) & 1u)) continue;
+        const uint32_t bit = 1u << k;
+        if ((uint64_t)(uint32_t)w[k] >= (uint64_t)n) {
+          flags |= 2ull;
           live &= ~bit;
-        } else {
-          w[k] = walk_target(rec[k]);
+          continue;
         }
-      } else {  // fewer steps left than this tile's walk takes: the remainder, below
-        slow |= bit;
-        live &= ~bit;
+        const int sw = walk_steps(rec[k]);
+        const bool fin = walk_final(rec[k]);
+        if (!fin && (sw == 0 || sw > kMaxSteps)) {
+          flags |= 2ull;
+          live &= ~bit;
+        } else if (t[k] == kMaxSteps) {  // the budget ran out exactly on entering this tile
+          root[k] = w[k];
+          if (!(fin && sw == 0)) flags |= 1ull;
+          live &= ~bit;
+        } else if (t[k] + sw <= kMaxSteps) {
+          t[k] += sw;
+          if (fin) {
+            root[k] = walk_target(rec[k]);
+            if (walk_capped(rec[k])) flags |= 1ull;
+            live &= ~bit;
+          } else {
+            w[k] = walk_target(rec[k]);
+          }
+        } else {  // fewer steps left than this tile's walk takes: the remainder, below
+          slow |= bit;
+          live &= ~bit;
+        }
       }
     }
-  }
-  for (uint32_t m = slow; m; m &= m - 1) {
-    const int k = __ffs(m) - 1;
-    int ww = 0, tt = 0;
+    for (uint32_t m = slow; m; m &= m - 1) {
+      const int k = __ffs(m) - 1;
+      int ww = 0, tt = 0;
 #pragma unroll
-    for (int q = 0; q < K; ++q)
-      if (q == k) {
-        ww = w[q];
-        tt = t[q];
+      for (int q = 0; q < K; ++q)
+        if (q == k) {
+          ww = w[q];
+          tt = t[q];
+        }
+      for (; tt < kMaxSteps; ++tt) {
+        const int wx = ww % nx, wy = (ww / nx) % ny, wz = ww / (nx * ny);
+        const int b = ascent_best(sdf, wx, wy, wz, nx, ny, nz, ww, sdf[ww]);
+        if (b == ww) break;
+        ww = b;
       }
-    for (; tt < kMaxSteps; ++tt) {
-      const int wx = ww % nx, wy = (ww / nx) % ny, wz = ww / (nx * ny);
-      const int b = ascent_best(sdf, wx, wy, wz, nx, ny, nz, ww, sdf[ww]);
-      if (b == ww) break;
-      ww = b;
-    }
-    {
-      const int wx = ww % nx, wy = (ww / nx) % ny, wz = ww / (nx * ny);
-      if (ascent_best(sdf, wx, wy, wz, nx, ny, nz, ww, sdf[ww]) != ww) flags |= 1ull;
+      {
+        const int wx = ww % nx, wy = (ww / nx) % ny, wz = ww / (nx * ny);
+        if (ascent_best(sdf, wx, wy, wz, nx, ny, nz, ww, sdf[ww]) != ww) flags |= 1ull;
+      }
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+        if (q == k) root[q] = ww;
     }
 #pragma unroll
-    for (int q = 0; q < K; ++q)
-      if (q == k) root[q] = ww;
+    for (int k = 0; k < K; ++k) {
+      if (root[k] >= 0) roots[vox[k]] = root[k];
+      // of two neighbouring lanes with one root, the one with the larger voxel index skips
+      const int prev = __shfl_up(root[k], 1, 64), prev_i = __shfl_up(vox[k], 1, 64);
+      if (root[k] >= 0 && vox[k] < root[k] && (lane == 0 || prev != root[k] || prev_i > vox[k]))
+        atomicMin(min_pore + root[k], vox[k]);
+    }
   }
   if (flags) atomicOr(cap_flag, flags);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (root[k] >= 0) roots[vox[k]] = root[k];
-    // of two neighbouring lanes with one root, the one with the larger voxel index skips
-    const int prev = __shfl_up(root[k], 1, 64), prev_i = __shfl_up(vox[k], 1, 64);
-    if (root[k] >= 0 && vox[k] < root[k] && (lane == 0 || prev != root[k] || prev_i > vox[k]))
-      atomicMin(min_pore + root[k], vox[k]);
-  }
 }
 
 // rootsOut convention for voxels that are not sdf>0: their own index (:383-384).
@@ -882,13 +887,12 @@ void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
              reinterpret_cast<const int2*>(xface), v.ny, v.nz, (int)cx);
 }
 
-void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,
-                          const uint64_t* count, int64_t grid_items, const uint64_t* walk,
+void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* stage,
+                          const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
                           int32_t* roots, int32_t* min_pore, uint64_t* cap_flag) {
-  if (grid_items <= 0) return;
-  PNM_LAUNCH(c, "k_roots_resolve", k_roots_resolve,
-             dim3(cdiv(cdiv(grid_items, RESOLVE_WALKS), 256)), dim3(256), 0,
-             sdf, v.nx, v.ny, v.nz, v.n, open, count, walk, roots, min_pore,
+  if (nseg <= 0) return;
+  PNM_LAUNCH(c, "k_roots_resolve", k_roots_resolve, dim3(cdiv(nseg, 4)), dim3(256), 0, sdf, v.nx,
+             v.ny, v.nz, v.n, stage, cnt, nseg, seg_cap, walk, roots, min_pore,
              reinterpret_cast<unsigned long long*>(cap_flag));
 }
 

@@ -93,11 +93,10 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band = c->arena.get_n<uint32_t>((size_t)n);
   uint32_t* band2 = c->arena.get_n<uint32_t>((size_t)n);
-  uint32_t* open = c->arena.get_n<uint32_t>((size_t)n);
   int32_t* newlab = c->arena.get_n<int32_t>((size_t)n);
   uint32_t* skeys = c->arena.get_n<uint32_t>((size_t)n);
   uint64_t* ctr = c->arena.get_n<uint64_t>(64);
-  // ctr: [0] peaks [1] open walks [2] solid keys [3] cap flag [4] pores [5] solids [6] full keys
+  // ctr: [0] peaks [1] (unused) [2] solid keys [3] cap flag [4] pores [5] solids [6] full keys
   //      [8..] band counts
   fill_u64(c, ctr, 0, 64);
 
@@ -120,7 +119,11 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
   launch_tile(c, sdf, v, thr, o);
   bitmap_list(c, peak_map, nwords, nwx, v.nx, peaks, ctr + 0);  // peaks in voxel order
-  compact_staged(c, stage_c, cnt_c, nseg, scap, open, ctr + 1);
+  // the walks that left their tile, read where the tile pass staged them (before the flood
+  // tile reuses that staging)
+  if (!labels_only)
+    launch_roots_resolve(c, sdf, v, stage_c, cnt_c, nseg, (int)scap, walk, roots, min_pore,
+                         ctr + 3);
   launch_ccl_seams(c, v, lab_ccl, xface);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
   if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
@@ -139,10 +142,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   }
   uint64_t h[9];
   readback(c, ctr, 9, h);
-  const int64_t npeaks = (int64_t)h[0], nopen = (int64_t)h[1], nskeys = (int64_t)h[2],
-                nband = (int64_t)h[8];
-  if (!labels_only)
-    launch_roots_resolve(c, sdf, v, open, ctr + 1, nopen, walk, roots, min_pore, ctr + 3);
+  const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2], nband = (int64_t)h[8];
 
   // Remaining Jacobi sweeps until a sweep labels nothing (pore_extraction.hpp:238-269). Band
   // lengths live on the device (ctr[8 + s]); the host checks them every two sweeps.

@@ -199,8 +199,10 @@ void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface);
 // After the seams: every tile-local root (staged by the tile pass) -> its component root.
 void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
                           const uint32_t* cnt);
-void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* open,
-                          const uint64_t* count, int64_t grid_items, const uint64_t* walk,
+// Walks that left their tile, read from the tile pass's staging (nseg segments of seg_cap slots,
+// counts in cnt): one wave per segment.
+void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* stage,
+                          const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
                           int32_t* roots, int32_t* min_pore, uint64_t* cap_flag);
 void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots);
 void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
