@@ -425,64 +425,71 @@ constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_WAVES = RS_THREADS / 64;
 
-template <class K>
+// DB-bit digits (8 or 9): NB = 2^DB buckets, each thread owning NB / RS_THREADS of them.
+template <class K, int DB>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* __restrict__ keys, int64_t n,
                                                         int shift, uint32_t* __restrict__ gh,
                                                         unsigned nblocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr int NB = 1 << DB;
+  __shared__ uint32_t h[NB];
+  for (int d = threadIdx.x; d < NB; d += RS_THREADS) h[d] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t i = base + r * RS_THREADS + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (NB - 1u)], 1u);
   }
   __syncthreads();
-  gh[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+  for (int d = threadIdx.x; d < NB; d += RS_THREADS)
+    gh[(int64_t)d * nblocks + blockIdx.x] = h[d];
 }
 
 // Each block re-reads its tile in 16 rounds of 256 keys (in input order). Within a round, a
-// wave finds the lanes sharing its digit with 9 ballots; per-wave digit counts are prefixed
-// across the 4 waves in wave order, so the scatter is stable.
-template <class K, bool VALS>
+// wave finds the lanes sharing its digit with DB + 1 ballots (the extra bit marks keys past the
+// end); per-wave digit counts are prefixed across the 4 waves in wave order, so the scatter is
+// stable.
+template <class K, bool VALS, int DB>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const K* __restrict__ keys, const uint32_t* __restrict__ vals, K* __restrict__ okeys,
     uint32_t* __restrict__ ovals, int64_t n, int shift, const uint32_t* __restrict__ ghs,
     unsigned nblocks) {
-  __shared__ uint32_t gbase[256];
-  __shared__ uint32_t cnt[RS_WAVES][256];
-  __shared__ uint32_t wbase[RS_WAVES][256];
+  constexpr int NB = 1 << DB;
+  __shared__ uint32_t gbase[NB];
+  __shared__ uint32_t cnt[RS_WAVES][NB];
+  __shared__ uint32_t wbase[RS_WAVES][NB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  gbase[t] = ghs[(int64_t)t * nblocks + blockIdx.x];
+  for (int d = t; d < NB; d += RS_THREADS) gbase[d] = ghs[(int64_t)d * nblocks + blockIdx.x];
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t i = base + r * RS_THREADS + t;
     const bool valid = i < n;
     const K k = valid ? keys[i] : (K)0;
-    const uint32_t d = valid ? ((uint32_t)(k >> shift) & 0xFFu) : 256u;
+    const uint32_t d = valid ? ((uint32_t)(k >> shift) & (NB - 1u)) : (uint32_t)NB;
     uint64_t peers = ~0ull;
 #pragma unroll
-    for (int b = 0; b < 9; ++b) {
+    for (int b = 0; b <= DB; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & below);
+    for (int dd = t; dd < NB; dd += RS_THREADS) {
 #pragma unroll
-    for (int q = 0; q < RS_WAVES; ++q) cnt[q][t] = 0;
+      for (int q = 0; q < RS_WAVES; ++q) cnt[q][dd] = 0;
+    }
     __syncthreads();
     if (valid && rank == 0) cnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
-    {
-      uint32_t s = gbase[t];
+    for (int dd = t; dd < NB; dd += RS_THREADS) {
+      uint32_t s = gbase[dd];
 #pragma unroll
       for (int q = 0; q < RS_WAVES; ++q) {
-        wbase[q][t] = s;
-        s += cnt[q][t];
+        wbase[q][dd] = s;
+        s += cnt[q][dd];
       }
-      gbase[t] = s;
+      gbase[dd] = s;
     }
     __syncthreads();
     if (valid) {
@@ -493,28 +500,42 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   }
 }
 
+template <class K, int DB>
+static void radix_pass(Ctx* c, const K* ks, const uint32_t* vs, K* kd, uint32_t* vd, int64_t n,
+                       int shift, uint32_t* gh, unsigned nb) {
+  PNM_LAUNCH(c, "k_rs_hist", (k_rs_hist<K, DB>), dim3(nb), dim3(RS_THREADS), 0, ks, n, shift, gh,
+             nb);
+  scan_exclusive_u32(c, gh, gh, ((int64_t)1 << DB) * nb);
+  if (vs)
+    PNM_LAUNCH(c, "k_rs_scatter", (k_rs_scatter<K, true, DB>), dim3(nb), dim3(RS_THREADS), 0, ks,
+               vs, kd, vd, n, shift, gh, nb);
+  else
+    PNM_LAUNCH(c, "k_rs_scatter", (k_rs_scatter<K, false, DB>), dim3(nb), dim3(RS_THREADS), 0,
+               ks, vs, kd, vd, n, shift, gh, nb);
+}
+
+// LSD radix sort over bits [begin_bit, end_bit): 9-bit digits when that takes fewer passes than
+// 8-bit ones (a 34-bit key: 4 passes instead of 5).
 template <class K>
 static void radix_sort_impl(Ctx* c, K* keys, uint32_t* vals, int64_t n, int begin_bit,
                             int end_bit) {
   if (n <= 1 || end_bit <= begin_bit) return;
+  const int nbits = end_bit - begin_bit;
+  const int db = (nbits + 8) / 9 < (nbits + 7) / 8 ? 9 : 8;
   const unsigned nb = (unsigned)((n + RS_TILE - 1) / RS_TILE);
   K* kalt = c->arena.get_n<K>((size_t)n);
   uint32_t* valt = vals ? c->arena.get_n<uint32_t>((size_t)n) : nullptr;
-  uint32_t* gh = c->arena.get_n<uint32_t>((size_t)256 * nb);
+  uint32_t* gh = c->arena.get_n<uint32_t>(((size_t)1 << db) * nb);
   K* ks = keys;
   K* kd = kalt;
   uint32_t* vs = vals;
   uint32_t* vd = valt;
   int passes = 0;
-  for (int shift = begin_bit; shift < end_bit; shift += 8) {
-    PNM_LAUNCH(c, "k_rs_hist", k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, ks, n, shift, gh,
-                       nb);
-    scan_exclusive_u32(c, gh, gh, (int64_t)256 * nb);
-    if (vals)
-      PNM_LAUNCH(c, "k_rs_scatter", (k_rs_scatter<K, true>), dim3(nb), dim3(RS_THREADS), 0, ks,
-                         vs, kd, vd, n, shift, gh, nb);
+  for (int shift = begin_bit; shift < end_bit; shift += db) {
+    if (db == 9)
+      radix_pass<K, 9>(c, ks, vs, kd, vd, n, shift, gh, nb);
     else
-      PNM_LAUNCH(c, "k_rs_scatter", (k_rs_scatter<K, false>), dim3(nb), dim3(RS_THREADS), 0, ks, vs, kd, vd, n, shift, gh, nb);
+      radix_pass<K, 8>(c, ks, vs, kd, vd, n, shift, gh, nb);
     std::swap(ks, kd);
     std::swap(vs, vd);
     ++passes;
