@@ -366,24 +366,59 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       LAB[l] = (int)w;
     }
   }
+  // One jump of active walk l (only its owner writes LAB[l]); returns whether it stays active.
+  auto jump = [&](int l) -> bool {
+    const uint32_t w = (uint32_t)LAB[l];
+    const uint32_t wv = (uint32_t)LAB[cs_u(w)];
+    const int s2 = cs_s(w) + cs_s(wv);
+    const uint32_t sv = cs_st(wv);
+    if (sv == CS_LONG || s2 > kMaxSteps) {
+      LAB[l] = (int)cs_pack(cs_u(w), cs_s(w), CS_LONG);
+      return false;
+    }
+    LAB[l] = (int)cs_pack(cs_u(wv), s2, sv);
+    return sv == CS_ACT;
+  };
+  // The wave's active walks as a list in the key buffers (free after the march), compacted in
+  // place after every round: a round then costs ceil(active / 64) turns instead of the busiest
+  // lane's count of active slices. A wave with more active walks than its share of the buffers
+  // keeps the per-lane loop over its bit masks.
+  constexpr int ALIST_W = (int)(sizeof(K) / sizeof(uint16_t)) / TILE_Y;
+  uint16_t* alist = reinterpret_cast<uint16_t*>(&K[0][0][0]) + wave * ALIST_W;
+  unsigned na = 0;
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    const bool ab = (act >> zl) & 1u;
+    const uint64_t am = __ballot(ab);
+    const unsigned pos = na + (unsigned)__popcll(am & below);
+    if (ab && pos < (unsigned)ALIST_W) alist[pos] = (uint16_t)((zl * TILE_Y + wave) * 64 + lane);
+    na += (unsigned)__popcll(am);
+  }
+  const bool listed = na <= (unsigned)ALIST_W;  // wave-uniform
   __syncthreads();
   while (true) {
-    for (uint32_t m = act; m; m &= m - 1) {
-      const int zl = __ffs(m) - 1;
-      const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const uint32_t w = (uint32_t)LAB[l];  // active: only this thread writes it
-      const uint32_t wv = (uint32_t)LAB[cs_u(w)];
-      const int s2 = cs_s(w) + cs_s(wv);
-      const uint32_t sv = cs_st(wv);
-      if (sv == CS_LONG || s2 > kMaxSteps) {
-        LAB[l] = (int)cs_pack(cs_u(w), cs_s(w), CS_LONG);
-        act &= ~(1u << zl);
-      } else {
-        LAB[l] = (int)cs_pack(cs_u(wv), s2, sv);
-        if (sv != CS_ACT) act &= ~(1u << zl);
+    bool any;
+    if (listed) {
+      unsigned nk = 0;
+      for (unsigned j0 = 0; j0 < na; j0 += 64) {
+        // every lane reads its entry before any survivor is written back below j0 + 64
+        const unsigned j = j0 + lane;
+        const int l = j < na ? (int)alist[j] : 0;
+        const bool still = j < na && jump(l);
+        const uint64_t sm = __ballot(still);
+        if (still) alist[nk + (unsigned)__popcll(sm & below)] = (uint16_t)l;
+        nk += (unsigned)__popcll(sm);
       }
+      na = nk;
+      any = na != 0;
+    } else {
+      for (uint32_t m = act; m; m &= m - 1) {
+        const int zl = __ffs(m) - 1;
+        if (!jump((zl * TILE_Y + wave) * 64 + lane)) act &= ~(1u << zl);
+      }
+      any = act != 0;
     }
-    if (!__syncthreads_or(act != 0)) break;
+    if (!__syncthreads_or(any)) break;
   }
 
   // ---- walk outputs; LAB at each fixed point collects the smallest local index whose walk
