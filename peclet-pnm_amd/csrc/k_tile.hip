@@ -107,6 +107,10 @@ __device__ __forceinline__ void dir_delta(int d, int& dx, int& dy, int& dz) {
 }
 
 // Three 512-thread blocks per CU need 6 waves per SIMD, i.e. at most 80 VGPRs.
+// M24: nz * ny and ny * nx are below 2^24, so every product in a flat-index computation has
+// 24-bit factors and uses the full-rate 24-bit multiply (a 32-bit multiply issues at a quarter
+// of the VALU rate).
+template <bool M24>
 __global__ __launch_bounds__(64 * TILE_Y) __attribute__((amdgpu_waves_per_eu(6)))
 void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, TileOut o) {
   __shared__ uint64_t K[2][T_LY][T_LX];  // keys of the haloed slice, two slices in rotation
@@ -125,9 +129,16 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int plane_n = ny * nx;
   unsigned nop = 0;
-  auto gidx = [&](int l) {
-    return ((z0 + (l >> 9)) * ny + y0 + ((l >> 6) & 7)) * nx + x0 + (l & 63);
+  auto mul = [](int a, int b) -> int {
+    return M24 ? (int)__umul24((unsigned)a, (unsigned)b) : a * b;
   };
+  const int tbase = (z0 * ny + y0) * nx + x0;  // flat index of the tile's first voxel
+  // flat index of this wave's row in slice 0 (wave-uniform: a scalar register)
+  const int srow = __builtin_amdgcn_readfirstlane(tbase + wave * nx);
+  auto gidx = [&](int l) {
+    return tbase + mul(l >> 9, plane_n) + mul((l >> 6) & 7, nx) + (l & 63);
+  };
+  auto flat = [&](int gz, int gy, int gx) { return mul(mul(gz, ny) + gy, nx) + gx; };
 
   // This thread's two halo-slice elements (e = tid, tid + 512 of the 10 x 66 slice): in-plane
   // offsets of their wrapped (x, y), fixed for the whole march.
@@ -204,7 +215,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     const int z = z0 + zl;
     const uint64_t m_next = slice_max(z + 1, va, vb, c_next, d_next);
     const int l = (zl * TILE_Y + wave) * 64 + lane;
-    const int i = (z * ny + y) * nx + x;
+    const int i = srow + lane + mul(zl, plane_n);
     const float cv = key_value(c_cur);
     bool is_peak = false;
     uint8_t dir = kNoDir;
@@ -311,7 +322,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     for (int zl = 0; zl < TILE_Z; ++zl) {
       if (zl >= nzt) break;
       const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const int i = ((z0 + zl) * ny + y) * nx + x;
+      const int i = srow + lane + mul(zl, plane_n);
       if (col_in) {
         o.labels[i] = r[zl] == -1 ? -1 : gidx(r[zl]);
         if (r[zl] == l) o.min_solid[i] = i;
@@ -428,7 +439,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   // whether the walk is still open.
   auto walk_out = [&](int l, int zl, int xl) -> bool {
     const int xg = x0 + xl;
-    const int i = ((z0 + zl) * ny + y) * nx + xg;
+    const int i = srow + xl + mul(zl, plane_n);
     int target, steps, fin_u = -1;
     bool fin = false, capped = false;
     const bool root_here = (DIR[l] & kDirMask) == 13;
@@ -450,7 +461,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
         const int gx = wrap_near(x0 + (u & 63) + dx, nx);
         const int gy = wrap_near(y0 + ((u >> 6) & 7) + dy, ny);
         const int gz = wrap_near(z0 + (u >> 9) + dz, nz);
-        target = (gz * ny + gy) * nx + gx;
+        target = flat(gz, gy, gx);
         steps += 1;
       }
     } else {  // CS_LONG: walk the 512 steps one by one
@@ -469,8 +480,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
         const int ax = cx + dx, ay = cy + dy, az = cz + dz;
         ++steps;
         if (!in_tile(ax, ay, az)) {
-          target = (wrap_near(z0 + az, nz) * ny + wrap_near(y0 + ay, ny)) * nx +
-                   wrap_near(x0 + ax, nx);
+          target = flat(wrap_near(z0 + az, nz), wrap_near(y0 + ay, ny), wrap_near(x0 + ax, nx));
           break;
         }
         cx = ax;
@@ -519,7 +529,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const uint64_t om = __ballot(open);
       if (open)
         o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
-            (uint32_t)(((z0 + (li >> 9)) * ny + y) * nx + x0 + (li & 63));
+            (uint32_t)(srow + mul(li >> 9, plane_n) + (li & 63));
       nop += (unsigned)__popcll(om);
     }
   } else {
@@ -533,7 +543,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const uint64_t om = __ballot(open);
       if (open)
         o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
-            (uint32_t)(((z0 + zl) * ny + y) * nx + x);
+            (uint32_t)(srow + lane + mul(zl, plane_n));
       nop += (unsigned)__popcll(om);
     }
   }
@@ -905,8 +915,12 @@ int64_t tile_segments(Vol v) {
 int64_t tile_segment_cap() { return T_SEG; }
 
 void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
-  PNM_LAUNCH(c, "k_tile", k_tile, tile_grid(v), dim3(64, TILE_Y), 0, sdf, v.nx, v.ny, v.nz, thr,
-             o);
+  if ((int64_t)v.nz * v.ny < (1 << 24) && (int64_t)v.ny * v.nx < (1 << 24))
+    PNM_LAUNCH(c, "k_tile", k_tile<true>, tile_grid(v), dim3(64, TILE_Y), 0, sdf, v.nx, v.ny, v.nz,
+               thr, o);
+  else
+    PNM_LAUNCH(c, "k_tile", k_tile<false>, tile_grid(v), dim3(64, TILE_Y), 0, sdf, v.nx, v.ny,
+               v.nz, thr, o);
 }
 
 void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
