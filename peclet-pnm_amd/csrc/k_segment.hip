@@ -154,42 +154,36 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
     if (halo && tid < nhalo) R[hrow * FR_X + 64 + hcol] = region_word(lh, rh, vh);
   }
   __syncthreads();
-  // which of this thread's tile voxels start unlabelled (flooded voxels lower the minimum)
-  uint32_t cand0 = 0;
-  for (int zl = 0; zl < nzt; ++zl)
-    if (R[((zl + H) * FR_Y + wave + H) * FR_X + lane + H] == -1) cand0 |= 1u << zl;
-  // Band candidates of the sweep-1 region, compacted into an LDS list (one segment per wave,
-  // filled by ballot without atomics) so that the sweeps touch only them (a row-by-row scan
-  // costs 26 LDS reads per row whenever one lane is a candidate). If a wave's rows hold more
+  // Band candidates: the tile's voxels that start unlabelled (with one sweep the exact region is
+  // the tile itself). cand0 marks this thread's (flooded voxels lower the minimum); they are also
+  // compacted into an LDS list, one segment per wave holding the wave's own row slice by slice,
+  // filled by ballot without atomics, so that the sweep touches only them. If a wave holds more
   // candidates than its segment, the tile scans the region instead.
+  static_assert(FLOOD_SWEEPS == 1, "the candidate region is the tile");
   constexpr int WL = FLOOD_LIST / TILE_Y;
+  uint32_t cand0 = 0;
   unsigned nw = 0;
-  for (int row = wave; row < rows; row += TILE_Y) {
-    const int ry = row % FR_Y, rz = row / FR_Y;
-    const bool row_ok = ry >= 1 && ry < FR_Y - 1 && rz >= 1 && rz < nzt + 2 * H - 1;
-    for (int rx0 = 0; rx0 < FR_X; rx0 += 64) {
-      const int rx = rx0 + lane;
-      const int e = row * FR_X + rx;
-      const bool cand = row_ok && rx >= 1 && rx < FR_X - 1 && R[e] == -1;
-      const uint64_t m = __ballot(cand);
-      const unsigned pos = nw + (unsigned)__popcll(m & below);
-      if (cand && pos < (unsigned)WL) cand_list[wave * WL + pos] = (uint16_t)e;
-      nw += (unsigned)__popcll(m);
-    }
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    const int e = ((zl + H) * FR_Y + wave + H) * FR_X + lane + H;
+    const bool cand = zl < nzt && R[e] == -1;
+    if (cand) cand0 |= 1u << zl;
+    const uint64_t m = __ballot(cand);
+    const unsigned pos = nw + (unsigned)__popcll(m & below);
+    if (cand && pos < (unsigned)WL) cand_list[wave * WL + pos] = (uint16_t)e;
+    nw += (unsigned)__popcll(m);
   }
   if (lane == 0) s_wcnt[wave] = nw;
   __syncthreads();
   bool listed = true;
 #pragma unroll
   for (int w = 0; w < TILE_Y; ++w) listed &= s_wcnt[w] <= (unsigned)WL;
-  auto sweep_one = [&](int e, int k) {
-    const int rz = e / (FR_X * FR_Y), rem = e - rz * (FR_X * FR_Y);
-    const int ry = rem / FR_X, rx = rem - ry * FR_X;
-    if (rx < k || rx >= FR_X - k || ry < k || ry >= FR_Y - k || rz < k ||
-        rz >= nzt + 2 * H - k)
-      return;
+  // One sweep at region entry e: the minimum labelled neighbour. As unsigned numbers every
+  // unlabelled state (negative) lies above every label (a flat index below 2^31), so the
+  // minimum of the 26 words is the smallest label if there is one.
+  auto sweep_at = [&](int e) {
     if (R[e] != -1) return;
-    int best = -1;
+    uint32_t best = ~0u;
 #pragma unroll
     for (int dz = -1; dz <= 1; ++dz)
 #pragma unroll
@@ -197,15 +191,22 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
 #pragma unroll
         for (int dx = -1; dx <= 1; ++dx) {
           if (dx == 0 && dy == 0 && dz == 0) continue;
-          const int nl = R[e + (dz * FR_Y + dy) * FR_X + dx];
-          if (nl >= 0 && (best < 0 || nl < best)) best = nl;
+          best = min(best, (uint32_t)R[e + (dz * FR_Y + dy) * FR_X + dx]);
         }
-    if (best >= 0) R[e] = -3 - best;  // labelled this sweep: reads as unlabelled until the end
+    if (best < 0x80000000u) R[e] = -3 - (int)best;  // labelled this sweep: reads as unlabelled
+  };
+  auto sweep_one = [&](int e, int k) {  // region scan: entries outside the exact region skip
+    const int rz = e / (FR_X * FR_Y), rem = e - rz * (FR_X * FR_Y);
+    const int ry = rem / FR_X, rx = rem - ry * FR_X;
+    if (rx < k || rx >= FR_X - k || ry < k || ry >= FR_Y - k || rz < k ||
+        rz >= nzt + 2 * H - k)
+      return;
+    sweep_at(e);
   };
   // Jacobi sweeps on the shrinking region
   for (int k = 1; k <= H; ++k) {
     if (listed) {
-      for (unsigned j = lane; j < nw; j += 64) sweep_one(cand_list[wave * WL + j], k);
+      for (unsigned j = lane; j < nw; j += 64) sweep_at(cand_list[wave * WL + j]);
     } else {
       for (int row = wave; row < rows; row += TILE_Y)
         for (int rx = lane; rx < FR_X; rx += 64) sweep_one(row * FR_X + rx, k);
