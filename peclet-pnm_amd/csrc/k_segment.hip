@@ -306,12 +306,12 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
           if (a == 1 && b == 1 && cc == 1) continue;
           nl[k++] = L[(int)i + oz[a] + oy[b] + ox[cc]];
         }
-    int best = -1;
+    // a label is a flat index (below 2^31); -1 (unlabelled) is above every label as unsigned
+    uint32_t best = ~0u;
 #pragma unroll
-    for (int q = 0; q < 26; ++q)  // a label is >= 0
-      if (nl[q] >= 0 && (best == -1 || nl[q] < best)) best = nl[q];
-    newlab[j] = best;
-    keep = (best == -1);
+    for (int q = 0; q < 26; ++q) best = min(best, (uint32_t)nl[q]);
+    newlab[j] = (int)best;
+    keep = (best == ~0u);
   }
   unsigned tot;
   const unsigned pos = block_stage_pos<4>(keep, wc, wave, lane, tot);
