@@ -349,32 +349,63 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     return ax >= 0 && ax < 64 && x0 + ax < nx && ay >= 0 && ay < TILE_Y && y0 + ay < ny &&
            az >= 0 && az < nzt;
   };
-  // Each thread walks only its sdf>0 slices and then only the slices whose walk is still active.
-  uint32_t act = 0;
-  {
-    uint8_t dv[TILE_Z];
-#pragma unroll
-    for (int zl = 0; zl < TILE_Z; ++zl) dv[zl] = DIR[(zl * TILE_Y + wave) * 64 + lane];
-#pragma unroll
-    for (int zl = 0; zl < TILE_Z; ++zl) {
-      if (!((pore_mask >> zl) & 1u)) continue;
-      const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const int d = dv[zl] & kDirMask;
-      uint32_t w;
-      if (d == 13) {
-        w = cs_pack(l, 0, CS_FIX);
+  // Chase state of sdf>0 voxel l (slice zl, column xl of this wave's row): its successor inside
+  // the tile (ACT), a fixed point (FIX), or a successor outside the tile (EXIT).
+  auto walk_init = [&](int l, int zl, int xl) -> bool {
+    const int d = DIR[l] & kDirMask;
+    uint32_t w;
+    bool active = false;
+    if (d == 13) {
+      w = cs_pack(l, 0, CS_FIX);
+    } else {
+      int dx, dy, dz;
+      dir_delta(d, dx, dy, dz);
+      const int ax = xl + dx, ay = wave + dy, az = zl + dz;
+      if (in_tile(ax, ay, az)) {
+        w = cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT);
+        active = true;
       } else {
-        int dx, dy, dz;
-        dir_delta(d, dx, dy, dz);
-        const int ax = lane + dx, ay = wave + dy, az = zl + dz;
-        if (in_tile(ax, ay, az)) {
-          w = cs_pack((az * TILE_Y + ay) * 64 + ax, 1, CS_ACT);
-          act |= 1u << zl;
-        } else {
-          w = cs_pack(l, 0, CS_EXIT);
-        }
+        w = cs_pack(l, 0, CS_EXIT);
       }
-      LAB[l] = (int)w;
+    }
+    LAB[l] = (int)w;
+    return active;
+  };
+  // The wave's sdf>0 voxels as a list in the key buffers (free after the march), in (slice, lane)
+  // order: the walk initialisation takes ceil(count / 64) turns instead of 16 slices, and keeps
+  // in place only the active walks, which the pointer jumping then compacts after every round
+  // (a round costs ceil(active / 64) turns instead of the busiest lane's count of active
+  // slices). A wave with more sdf>0 voxels than its share of the buffers keeps per-lane loops
+  // over its bit masks.
+  constexpr int ALIST_W = (int)(sizeof(K) / sizeof(uint16_t)) / TILE_Y;
+  uint16_t* alist = reinterpret_cast<uint16_t*>(&K[0][0][0]) + wave * ALIST_W;
+  unsigned na = 0;
+#pragma unroll
+  for (int zl = 0; zl < TILE_Z; ++zl) {
+    const bool pb = (pore_mask >> zl) & 1u;
+    const uint64_t pm = __ballot(pb);
+    const unsigned pos = na + (unsigned)__popcll(pm & below);
+    if (pb && pos < (unsigned)ALIST_W) alist[pos] = (uint16_t)((zl * TILE_Y + wave) * 64 + lane);
+    na += (unsigned)__popcll(pm);
+  }
+  const bool listed = na <= (unsigned)ALIST_W;  // wave-uniform
+  uint32_t act = 0;
+  if (listed) {
+    unsigned nk = 0;
+    for (unsigned j0 = 0; j0 < na; j0 += 64) {
+      // every lane reads its entry before any active walk is written back below j0 + 64
+      const unsigned j = j0 + lane;
+      const int l = j < na ? (int)alist[j] : 0;
+      const bool active = j < na && walk_init(l, l >> 9, l & 63);
+      const uint64_t am = __ballot(active);
+      if (active) alist[nk + (unsigned)__popcll(am & below)] = (uint16_t)l;
+      nk += (unsigned)__popcll(am);
+    }
+    na = nk;
+  } else {
+    for (uint32_t m = pore_mask; m; m &= m - 1) {
+      const int zl = __ffs(m) - 1;
+      if (walk_init((zl * TILE_Y + wave) * 64 + lane, zl, lane)) act |= 1u << zl;
     }
   }
   // One jump of active walk l (only its owner writes LAB[l]); returns whether it stays active.
@@ -390,22 +421,6 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     LAB[l] = (int)cs_pack(cs_u(wv), s2, sv);
     return sv == CS_ACT;
   };
-  // The wave's active walks as a list in the key buffers (free after the march), compacted in
-  // place after every round: a round then costs ceil(active / 64) turns instead of the busiest
-  // lane's count of active slices. A wave with more active walks than its share of the buffers
-  // keeps the per-lane loop over its bit masks.
-  constexpr int ALIST_W = (int)(sizeof(K) / sizeof(uint16_t)) / TILE_Y;
-  uint16_t* alist = reinterpret_cast<uint16_t*>(&K[0][0][0]) + wave * ALIST_W;
-  unsigned na = 0;
-#pragma unroll
-  for (int zl = 0; zl < TILE_Z; ++zl) {
-    const bool ab = (act >> zl) & 1u;
-    const uint64_t am = __ballot(ab);
-    const unsigned pos = na + (unsigned)__popcll(am & below);
-    if (ab && pos < (unsigned)ALIST_W) alist[pos] = (uint16_t)((zl * TILE_Y + wave) * 64 + lane);
-    na += (unsigned)__popcll(am);
-  }
-  const bool listed = na <= (unsigned)ALIST_W;  // wave-uniform
   __syncthreads();
   while (true) {
     bool any;
