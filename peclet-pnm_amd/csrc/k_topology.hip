@@ -49,7 +49,8 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
                                                    unsigned long long* ocount) {
   __shared__ unsigned long long table[TOPO_TABLE];
   __shared__ int S[FUSED ? TB_X * (TB_Y * TB_Z + 1) : 1];  // + a spare row (see below)
-  __shared__ uint32_t RBASE[FUSED ? TB_Y * TB_Z : 1];  // flat offset of each region row
+  // flat offset of each region row; bit 31: a row of the brick inside the volume (writes seg)
+  __shared__ uint32_t RBASE[FUSED ? TB_Y * TB_Z : 1];
   __shared__ unsigned wc[4];
   __shared__ unsigned s_base;
   const int tid = threadIdx.y * 64 + threadIdx.x;
@@ -60,8 +61,10 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   constexpr int NROW = TB_Y * TB_Z;
   if (FUSED && tid < NROW) {  // wrapped flat offset of each row of the region, once per block
     const int ry = tid % TB_Y, rz = tid / TB_Y;
+    const bool brick = ry < TB_Y - 1 && rz < TOPO_ZB && y0 + ry < ny && z0 + rz < nz;
     RBASE[tid] = ((uint32_t)wrap_any(z0 + rz, nz) * (uint32_t)ny + (uint32_t)wrap_any(y0 + ry, ny)) *
-                 (uint32_t)nx;
+                     (uint32_t)nx |
+                 (brick ? 0x80000000u : 0u);
   }
   if (FUSED) __syncthreads();
   if (FUSED) {
@@ -75,7 +78,7 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
     // table (flat offsets fit 32 bits: volumes hold fewer than 2^31 voxels).
     constexpr int RB = 11, MAXR = (NROW + 3) / 4;
     const int gx = wrap_any(x0 + lane, nx), gxh = wrap_any(x0 + 64, nx);
-    auto row_base = [&](int row) -> int64_t { return (int64_t)RBASE[row]; };
+    auto row_base = [&](int row) -> int64_t { return (int64_t)(RBASE[row] & 0x7fffffffu); };
     auto label_of = [&](float v, int64_t g) {
       const int32_t* src = v > 0.0f ? ids.roots : ids.labels;
       const int r = src[g];
@@ -91,19 +94,19 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
 #pragma unroll 1
     for (int j0 = 0; j0 < MAXR; j0 += RB) {
       const bool halo = j0 == 0;
-      int64_t g[RB];
+      uint32_t rb[RB];  // the rows' table words (offset, brick flag)
       float v[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         const int row0 = swave + (j0 + j) * 4;
-        g[j] = row_base(row0 < NROW ? row0 : swave) + gx;
-        v[j] = ids.sdf[g[j]];
+        rb[j] = RBASE[row0 < NROW ? row0 : swave];
+        v[j] = ids.sdf[(rb[j] & 0x7fffffffu) + (uint32_t)gx];
       }
       const int64_t gh = row_base(hrow) + gxh;
       const float vh = ids.sdf[gh];  // both batches load it (branch-free; cached the second time)
       int k[RB];
 #pragma unroll
-      for (int j = 0; j < RB; ++j) k[j] = label_of(v[j], g[j]);
+      for (int j = 0; j < RB; ++j) k[j] = label_of(v[j], (rb[j] & 0x7fffffffu) + (uint32_t)gx);
       const int kh = label_of(vh, gh);
 #pragma unroll
       for (int j = 0; j < RB; ++j) k[j] = id_of(k[j]);
@@ -114,11 +117,10 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
       for (int j = 0; j < RB; ++j) {
         const int row = swave + (j0 + j) * 4;
         S[(row < NROW ? row : NROW) * TB_X + lane] = k[j];
-        const int ry = row % TB_Y, rz = row / TB_Y;
-        // a brick row inside the volume is its own unwrapped row: g is the voxel's flat index
-        if (row < NROW && ry < 4 && rz < TOPO_ZB && x0 + lane < nx && y0 + ry < ny &&
-            z0 + rz < nz)
-          ids.seg_out[g[j]] = k[j];
+        // a brick row inside the volume is its own unwrapped row: its offset plus gx is the
+        // voxel's flat index
+        if (row < NROW && (rb[j] >> 31) && x0 + lane < nx)
+          ids.seg_out[(rb[j] & 0x7fffffffu) + (uint32_t)gx] = k[j];
       }
       if (halo && tid < NROW) S[tid * TB_X + 64] = sh;
     }
