@@ -129,6 +129,34 @@ def test_heavy_ties(capi_ctx, shape):
     check_against_oracle(capi_ctx, sdf, (0, 0, 0), (1, 1, 1))
 
 
+def test_pore_heavy_rows(capi_ctx):
+    """Mostly sdf>0: tile rows with more sdf>0 voxels than the per-wave walk lists hold take
+    the per-lane walk loops."""
+    rng = np.random.default_rng(77)
+    sdf = (np.round(rng.uniform(-1.0, 5.0, size=(24, 20, 130)) / 0.5) * 0.5).astype(np.float32)
+    assert (sdf > 0).mean() > 0.75
+    check_against_oracle(capi_ctx, sdf, (0, 0, 0), (1, 1, 1))
+
+
+def test_band_heavy_rows(capi_ctx):
+    """Mostly unlabelled solid (-1.5 < sdf < 0) with sparse markers: tile rows with more band
+    candidates than the flood tile's per-wave list holds take its region scan, and the flood
+    needs many global sweeps."""
+    rng = np.random.default_rng(78)
+    sdf = np.round(rng.uniform(-1.25, -0.25, size=(24, 20, 130)) / 0.25) * 0.25
+    sdf[rng.random(sdf.shape) < 0.03] = -3.0
+    sdf[rng.random(sdf.shape) < 0.03] = 2.0
+    check_against_oracle(capi_ctx, sdf.astype(np.float32), (0, 0, 0), (1, 1, 1))
+
+
+def test_wide_plane_general_multiply(capi_ctx):
+    """ny * nx >= 2^24: the tile pass computes flat indices with 32-bit multiplies instead of
+    24-bit ones."""
+    shape = (1, 4100, 4100)
+    assert shape[1] * shape[2] >= 1 << 24
+    check_against_oracle(capi_ctx, quantized(shape, 79), (0, 0, 0), (1, 1, 1))
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_nan_inf(capi_ctx, seed):
     sdf = quantized((10, 12, 70), seed, nan_frac=0.05, inf_frac=0.01)
