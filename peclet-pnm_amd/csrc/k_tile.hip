@@ -113,7 +113,11 @@ __device__ __forceinline__ void dir_delta(int d, int& dx, int& dy, int& dz) {
 template <bool M24>
 __global__ __launch_bounds__(64 * TILE_Y) __attribute__((amdgpu_waves_per_eu(6)))
 void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, TileOut o) {
-  __shared__ uint64_t K[2][T_LY][T_LX];  // keys of the haloed slice, two slices in rotation
+  // keys of the haloed slice (T_LY x T_LX), then the x-triple maxima of its rows (T_LY x 64)
+  __shared__ uint64_t KB[T_LY * T_LX + T_LY * 64];
+  __shared__ uint8_t XD[T_LY][64];  // the x offset (0..2) of each row maximum
+  uint64_t* const K = KB;
+  uint64_t* const XM = KB + T_LY * T_LX;
   __shared__ int LAB[T_VOX];
   __shared__ uint8_t DIR[T_VOX];
   __shared__ uint64_t MASK[TILE_Z * TILE_Y];  // marker mask of each tile row
@@ -165,33 +169,46 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     b = plane[off1];
   };
   // 3x3 (x, y) key maximum of this thread's column at slice zg (values a, b already loaded),
-  // with the winner's (dy, dx) as dy*3+dx, and the centre key. Slices alternate between the two
-  // key buffers, so one barrier per slice suffices: a thread overwrites a buffer two slices
-  // later, after the barrier of the slice in between, which every thread reaches only once it
-  // has read the buffer. The scan runs in (dy, dx) row-major order with a strict comparison,
-  // so among equal keys (a voxel met twice through a periodic wrap) the first position wins.
+  // with the winner's (dy, dx) as dy*3+dx, and the centre key. Separable: the x-triple maximum
+  // of each of the slice's 10 rows is computed once (1.25 per thread) and shared through LDS,
+  // then each thread takes the maximum of three of them: 4.5 compare-selects per voxel instead
+  // of 8, at the price of a second barrier. The first position wins among equal keys (a voxel
+  // met twice through a periodic wrap), in (dy, dx) row-major order.
   auto slice_max = [&](int zg, float a, float b, uint64_t& centre, int& code) -> uint64_t {
     const uint32_t zbase = (uint32_t)wrap_near(zg, nz) * (uint32_t)plane_n;
-    const int pb = zg & 1;
-    uint64_t* Kp = &K[pb][0][0];
-    Kp[tid] = lex_key(a, zbase + (uint32_t)off0);
-    if (has1) Kp[tid + 64 * TILE_Y] = lex_key(b, zbase + (uint32_t)off1);
+    K[tid] = lex_key(a, zbase + (uint32_t)off0);
+    if (has1) K[tid + 64 * TILE_Y] = lex_key(b, zbase + (uint32_t)off1);
     __syncthreads();
-    uint64_t m = K[pb][wave][lane];
-    int c = 0;
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
-        if (dy == 0 && dx == 0) continue;
-        const uint64_t k = K[pb][wave + dy][lane + dx];
-        if (dy == 1 && dx == 1) centre = k;
-        if (k > m) {
-          m = k;
-          c = dy * 3 + dx;
-        }
+    for (int e = tid; e < T_LY * 64; e += 64 * TILE_Y) {
+      const int ly = e >> 6, lx = e & 63;
+      uint64_t m = K[ly * T_LX + lx];
+      uint8_t d = 0;
+      const uint64_t k1 = K[ly * T_LX + lx + 1], k2 = K[ly * T_LX + lx + 2];
+      if (k1 > m) {
+        m = k1;
+        d = 1;
       }
-    code = c;
+      if (k2 > m) {
+        m = k2;
+        d = 2;
+      }
+      XM[e] = m;
+      XD[ly][lx] = d;
+    }
+    centre = K[(wave + 1) * T_LX + lane + 1];
+    __syncthreads();
+    uint64_t m = XM[wave * 64 + lane];
+    int dy = 0;
+    const uint64_t m1 = XM[(wave + 1) * 64 + lane], m2 = XM[(wave + 2) * 64 + lane];
+    if (m1 > m) {
+      m = m1;
+      dy = 1;
+    }
+    if (m2 > m) {
+      m = m2;
+      dy = 2;
+    }
+    code = dy * 3 + XD[wave + dy][lane];
     return m;
   };
 
@@ -377,8 +394,8 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   // (a round costs ceil(active / 64) turns instead of the busiest lane's count of active
   // slices). A wave with more sdf>0 voxels than its share of the buffers keeps per-lane loops
   // over its bit masks.
-  constexpr int ALIST_W = (int)(sizeof(K) / sizeof(uint16_t)) / TILE_Y;
-  uint16_t* alist = reinterpret_cast<uint16_t*>(&K[0][0][0]) + wave * ALIST_W;
+  constexpr int ALIST_W = (int)(sizeof(KB) / sizeof(uint16_t)) / TILE_Y;
+  uint16_t* alist = reinterpret_cast<uint16_t*>(KB) + wave * ALIST_W;
   unsigned na = 0;
 #pragma unroll
   for (int zl = 0; zl < TILE_Z; ++zl) {
@@ -525,8 +542,8 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   // after the march): the wave then takes ceil(count / 64) turns instead of its busiest lane's
   // count of slices. A wave with more voxels than its share of the buffers uses the per-lane
   // loop. Open walks are staged in the same (slice, lane) order either way.
-  constexpr int PLIST_W = (int)(sizeof(K) / sizeof(uint16_t)) / TILE_Y;
-  uint16_t* plist = reinterpret_cast<uint16_t*>(&K[0][0][0]) + wave * PLIST_W;
+  constexpr int PLIST_W = (int)(sizeof(KB) / sizeof(uint16_t)) / TILE_Y;
+  uint16_t* plist = reinterpret_cast<uint16_t*>(KB) + wave * PLIST_W;
   unsigned np = 0;
 #pragma unroll
   for (int zl = 0; zl < TILE_Z; ++zl) {
