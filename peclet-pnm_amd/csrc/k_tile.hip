@@ -150,12 +150,15 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   const bool has1 = tid + 64 * TILE_Y < kSliceElems;
   int off0, off1 = 0;
   {
+    // coordinates lie in [-1, 2N) when the volume is at least a haloed slice wide and deep
+    const bool near = nx >= T_LX && ny >= T_LY;
+    auto wrap = [&](int v, int n) { return near ? wrap_near(v, n) : wrap_any(v, n); };
     const int ly = tid / T_LX, lx = tid - ly * T_LX;
-    off0 = wrap_any(y0 - 1 + ly, ny) * nx + wrap_any(x0 - 1 + lx, nx);
+    off0 = mul(wrap(y0 - 1 + ly, ny), nx) + wrap(x0 - 1 + lx, nx);
     if (has1) {
       const int e = tid + 64 * TILE_Y;
       const int ly1 = e / T_LX, lx1 = e - ly1 * T_LX;
-      off1 = wrap_any(y0 - 1 + ly1, ny) * nx + wrap_any(x0 - 1 + lx1, nx);
+      off1 = mul(wrap(y0 - 1 + ly1, ny), nx) + wrap(x0 - 1 + lx1, nx);
     }
   }
   // Loads are unconditional (a thread without a second element re-reads its first; slices past
