@@ -52,7 +52,10 @@ int64_t pores_device(Ctx* c, const float* sdf, Vol v, const float org[3], const 
 }
 
 struct SegInternal {
-  SegResult r;
+  SegResult r;  // counts read back only when segment_core is asked for them
+  // bounds known on the host without a read back: pores <= pore_bound (pore keys), solids ==
+  // solid_count (one id per component root)
+  int64_t pore_bound, solid_count;
   uint32_t* peaks;  // peak list (extract_pores predicate) in ascending voxel order
   int64_t npeaks;
   // inputs of relabel_seg, for a caller that fuses it (seg_topology)
@@ -67,7 +70,8 @@ struct SegInternal {
 // seg == nullptr: everything up to the id assignment; the caller writes seg (fused).
 static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc[3],
                                 int32_t* seg, int32_t* roots_out, int32_t* labels_out = nullptr,
-                                int32_t* ccl_out = nullptr, bool labels_only = false) {
+                                int32_t* ccl_out = nullptr, bool labels_only = false,
+                                bool read_counts = true) {
   const int64_t n = v.n;
   const float mn = std::min(spc[0], std::min(spc[1], spc[2]));
   const float thr = -1.5f * mn;  // pore_extraction.hpp:163-164
@@ -183,9 +187,13 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   assign_ids_bitmap(c, sdf, v, pkeys, npk, min_pore, skeys, nskeys, min_solid, ctr + 4);
   if (seg) launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
   if (roots_out) launch_roots_self(c, sdf, v, roots_out);
-  readback(c, ctr + 4, 2, h);
-  out.r.n_pores = (int32_t)h[0];
-  out.r.n_solids = (int32_t)h[1];
+  if (read_counts) {
+    readback(c, ctr + 4, 2, h);
+    out.r.n_pores = (int32_t)h[0];
+    out.r.n_solids = (int32_t)h[1];
+  }
+  out.pore_bound = npk;
+  out.solid_count = nskeys;
   out.peaks = peaks;
   out.npeaks = npeaks;
   out.roots = roots;
@@ -216,12 +224,13 @@ int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], cons
                        pnm_pore_t* pores_d, int64_t cap_pores, int64_t* total_pores,
                        int32_t* seg, int32_t* pairs_d, int64_t cap_pairs, int64_t* npairs,
                        bool* pairs_overflow) {
-  SegInternal s = segment_core(c, sdf, v, spc, nullptr, nullptr);
+  // the topology keys need only label bounds, so the id counts are not read back
+  SegInternal s = segment_core(c, sdf, v, spc, nullptr, nullptr, nullptr, nullptr, false, false);
   const int64_t np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, true, pores_d,
                                   cap_pores, total_pores);
   int64_t needed = 0;
   const int64_t nt = seg_topology(c, sdf, s.roots, s.labels, s.pore_id, s.solid_id, seg, v,
-                                  -s.r.n_solids, s.r.n_pores, pairs_d, cap_pairs, &needed);
+                                  -s.solid_count, s.pore_bound, pairs_d, cap_pairs, &needed);
   *npairs = needed;
   *pairs_overflow = nt < 0;
   return np;
