@@ -86,7 +86,8 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
                                                             uint32_t* __restrict__ key_stage,
                                                             uint32_t* __restrict__ key_cnt,
                                                             uint32_t* __restrict__ band_stage,
-                                                            uint32_t* __restrict__ band_cnt) {
+                                                            uint32_t* __restrict__ band_cnt,
+                                                            const int32_t* __restrict__ xface) {
   __shared__ int R[FR_N];
   __shared__ uint16_t cand_list[FLOOD_LIST];
   __shared__ unsigned s_wcnt[TILE_Y];
@@ -101,23 +102,30 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   constexpr int H = FLOOD_SWEEPS;
   // load: labels (a marker's component is L[L[i]]: its tile-local root points at the component
   // root) and band candidates. Rows of the region are spread over the waves, a lane holding
-  // column lane; the 2H halo columns of every row are one more load chain on the first threads,
-  // issued with the first batch. Per batch: the raw labels and values, then the component
-  // roots, then one LDS store. Loads are issued in branch-free batches of RB rows into
-  // registers (rows past the region re-read a valid row). The wrapped flat offset of each
-  // region row comes from a table the block fills first, kept in the candidate list's LDS
-  // (not used before the sweeps); flat offsets fit 32 bits (volumes hold fewer than 2^31
-  // voxels).
+  // column x0 + lane (rows aligned to the chunk: two 128-byte lines per row and array). The
+  // halo columns x0-1 and x0+64 are the neighbouring chunks' edge columns, whose tile labels
+  // the tile pass wrote compactly (x-face pairs): one more load chain on the first threads,
+  // issued with the first batch, with no value needed (halo entries are never swept, so an
+  // unlabelled one is just "not a label"). A partial last chunk reads its right halo column
+  // directly. Per batch: the raw labels and values, then the component roots, then one LDS
+  // store. Loads are issued in branch-free batches of RB rows into registers (rows past the
+  // region re-read a valid row). Each region row's wrapped (z*ny + y) and flat offset come from
+  // tables the block fills first, kept in the candidate list's LDS (not used before the sweep);
+  // flat offsets fit 32 bits (volumes hold fewer than 2^31 voxels).
+  static_assert(H == 1, "the x halo is one column on each side");
   const int rows = FR_Y * (nzt + 2 * H);
-  const int gx0 = wrap_any(x0 - H + lane, nx);
+  const int gx0 = wrap_any(x0 + lane, nx);
   constexpr int RB = 12;
   constexpr int MAXR = (FR_Y * FR_Z + TILE_Y - 1) / TILE_Y;  // rows per wave, full tile
-  uint32_t* rbase = reinterpret_cast<uint32_t*>(cand_list);
-  static_assert(sizeof(cand_list) >= sizeof(uint32_t) * FR_Y * FR_Z, "row table fits");
+  uint32_t* rzy = reinterpret_cast<uint32_t*>(cand_list);
+  uint32_t* rbase = rzy + FR_Y * FR_Z;
+  static_assert(sizeof(cand_list) >= 2 * sizeof(uint32_t) * FR_Y * FR_Z, "row tables fit");
   if (tid < rows) {
     const int ry = tid % FR_Y, rz = tid / FR_Y;
-    rbase[tid] = ((uint32_t)wrap_any(z0 - H + rz, nz) * (uint32_t)ny +
-                  (uint32_t)wrap_any(y0 - H + ry, ny)) * (uint32_t)nx;
+    const uint32_t zy = (uint32_t)wrap_any(z0 - H + rz, nz) * (uint32_t)ny +
+                        (uint32_t)wrap_any(y0 - H + ry, ny);
+    rzy[tid] = zy;
+    rbase[tid] = zy * (uint32_t)nx;
   }
   __syncthreads();
   auto row_base = [&](int row) -> int64_t { return (int64_t)rbase[row]; };
@@ -125,9 +133,20 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
     return l >= 0 ? root : (!(v >= 0.0f) ? -1 : -2);
   };
   const int swave = __builtin_amdgcn_readfirstlane(wave);
-  const int nhalo = rows * 2 * H;  // <= 2H * FR_Y * FR_Z < NT
-  const int hrow = tid < nhalo ? tid / (2 * H) : 0, hcol = tid % (2 * H);
-  const int64_t gh = row_base(hrow) + wrap_any(x0 - H + 64 + hcol, nx);
+  const int nhalo = rows * 2;  // <= 2 * FR_Y * FR_Z < NT
+  const int hrow = tid < nhalo ? tid >> 1 : 0, hside = tid & 1;  // 0: x0-1, 1: x0+64
+  const int cx = gridDim.x;
+  const int64_t nzy = (int64_t)nz * ny;
+  const int bxm = blockIdx.x > 0 ? (int)blockIdx.x - 1 : cx - 1;
+  const int bxp = (int)blockIdx.x + 1 < cx ? (int)blockIdx.x + 1 : 0;
+  const bool right_face = x0 + 64 < nx || nx % 64 == 0;  // x0+64 is chunk bxp's first column
+  // x-face entry (chunk, z, y) holds (first column, last column): x0-1 is chunk bxm's last
+  // column (for chunk 0 the volume's last column), x0+64 chunk bxp's first
+  const bool from_face = hside == 0 || right_face;
+  const int32_t* hsrc = from_face ? xface : L;
+  const int64_t hidx = hside == 0 ? 2 * (bxm * nzy + rzy[hrow]) + 1
+                                  : (right_face ? 2 * (bxp * nzy + rzy[hrow])
+                                                : row_base(hrow) + wrap_any(x0 + 64, nx));
 #pragma unroll 1
   for (int j0 = 0; j0 < MAXR; j0 += RB) {
     const bool halo = j0 == 0;
@@ -140,8 +159,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
       la[j] = L[g];
       va[j] = sdf[g];
     }
-    const int lh = halo ? L[gh] : -1;
-    const float vh = halo ? sdf[gh] : 0.0f;
+    const int lh = halo ? hsrc[hidx] : -1;
     int ra[RB];
 #pragma unroll
     for (int j = 0; j < RB; ++j) ra[j] = L[la[j] >= 0 ? la[j] : 0];
@@ -149,9 +167,9 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
       const int row = swave + (j0 + j) * TILE_Y;
-      if (row < rows) R[row * FR_X + lane] = region_word(la[j], ra[j], va[j]);
+      if (row < rows) R[row * FR_X + lane + 1] = region_word(la[j], ra[j], va[j]);
     }
-    if (halo && tid < nhalo) R[hrow * FR_X + 64 + hcol] = region_word(lh, rh, vh);
+    if (halo && tid < nhalo) R[hrow * FR_X + (hside ? 65 : 0)] = lh >= 0 ? rh : -2;
   }
   __syncthreads();
   // Band candidates: the tile's voxels that start unlabelled (with one sweep the exact region is
@@ -260,11 +278,12 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
 
 void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
                        int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
-                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt) {
+                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt,
+                       const int32_t* xface) {
   PNM_LAUNCH(c, "k_flood_tile", k_flood_tile,
              dim3(cdiv(v.nx, 64), cdiv(v.ny, TILE_Y), cdiv(v.nz, TILE_Z)), dim3(64, TILE_Y), 0,
              sdf, labels_ccl, labels, v.nx, v.ny, v.nz, min_solid, key_stage, key_cnt, band_stage,
-             band_cnt);
+             band_cnt, xface);
 }
 
 // ---------------- flood (pore_extraction.hpp:237-269) ----------------

@@ -137,7 +137,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   }
   // flatten + the first flood sweeps per tile; the band left over goes to the global sweeps
   launch_flood_tile(c, sdf, v, lab_ccl, labels, min_solid, stage_b, cnt_pair2, stage_c,
-                    cnt_pair2 + nseg);
+                    cnt_pair2 + nseg, xface);
   {
     const uint32_t* st[2] = {stage_b, stage_c};
     uint32_t* outs[2] = {skeys, band};

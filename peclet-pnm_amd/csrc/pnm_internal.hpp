@@ -209,10 +209,11 @@ void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint
                         uint32_t* key_cnt);
 // Flatten + the first flood sweep(s) per tile: reads the CCL labels (tile-local roots whose
 // own entries already hold their component root), writes the flooded labels to a second array
-// (segments as tile_segments / tile_segment_cap).
+// (segments as tile_segments / tile_segment_cap). xface: the tile pass's x-face labels.
 void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
                        int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
-                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt);
+                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt,
+                       const int32_t* xface);
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                 uint32_t* key_cnt);
 // One Jacobi flood sweep over the band list whose length is the device word *count (grid sized
