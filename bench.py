@@ -49,9 +49,10 @@ ALGO_BYTES_PER_VOXEL = {
 PIPELINE_BYTES_PER_VOXEL = 64  # S1..S6 compulsory traffic of extract_pore_network
 
 # HBM traffic per launch measured with rocprofv3 counters on this workload (written by
-# --pmc-summary); rocprof kernel symbols differ from the library's timing names for one kernel.
+# --pmc-summary); rocprof kernel symbols differ from the library's timing names for two kernels.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_extract_pore_network_512_polydisperse.md")
-PROFILER_NAMES = {"k_seg_topo": "k_topo_cand<true>"}
+PROFILER_NAMES = {"k_seg_topo": "k_topo_cand<true>",
+                  "k_tile": "k_tile<true>"}  # the 24-bit index variant (ny*nx, nz*ny < 2^24)
 
 
 def make_sdf(workload, size, seed):
@@ -300,7 +301,7 @@ def pmc_summary(root, copy_bytes, voxels):
             acc[k].append(v * 1024.0 * scale)
         for k, vals in acc.items():
             per.setdefault(k, {})[key] = vals
-    steps = len(per.get("k_tile", {}).get("fetch", [])) or 1
+    steps = len(per.get(PROFILER_NAMES["k_tile"], {}).get("fetch", [])) or 1
     rows = []
     for k, d in per.items():
         if k.startswith(("vectorized_elementwise", "reduce_kernel", "distribution", "elementwise")):
@@ -338,6 +339,56 @@ def pmc_summary(root, copy_bytes, voxels):
     return "\n".join(out)
 
 
+def sq_summary(root, voxels):
+    """Markdown summary of two rocprofv3 --pmc runs of SQ counters under `root` (pmc_sq_a:
+    SQ_WAVES, SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_LDS, SQ_INSTS_VMEM_RD, SQ_INSTS_VMEM_WR,
+    SQ_WAVE_CYCLES, SQ_BUSY_CYCLES; pmc_sq_b: SQ_WAIT_ANY, SQ_WAIT_INST_ANY, SQ_ACTIVE_INST_VALU,
+    SQ_ACTIVE_INST_LDS, SQ_LDS_BANK_CONFLICT, SQ_LDS_IDX_ACTIVE, SQ_WAIT_INST_LDS,
+    SQ_ACTIVE_INST_ANY): per kernel, instructions per wave and per launch, and the VALU issue
+    time per launch at one wave64 VALU instruction per CU per clock."""
+    import csv
+    from collections import defaultdict
+
+    agg = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
+    for name in ("pmc_sq_a", "pmc_sq_b"):
+        with open(os.path.join(root, name, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                k = _kernel_symbol(r["Kernel_Name"])
+                agg[k][r["Counter_Name"]] += float(r["Counter_Value"])
+                if name == "pmc_sq_a":
+                    disp[k].add(r["Dispatch_Id"])
+    cus, clock_ghz = 256, 2.4
+    rows = []
+    for k, c in agg.items():
+        if k.startswith(("vectorized_elementwise", "reduce_kernel", "distribution", "elementwise")):
+            continue  # torch's own set-up kernels outside the pipeline
+        w, nd = c.get("SQ_WAVES", 0.0), max(1, len(disp[k]))
+        if w <= 0:
+            continue
+        valu = c.get("SQ_INSTS_VALU", 0.0) / nd
+        rows.append((valu, k, nd, w / nd, c.get("SQ_INSTS_VALU", 0.0) / w,
+                     c.get("SQ_INSTS_SALU", 0.0) / w, c.get("SQ_INSTS_LDS", 0.0) / w,
+                     c.get("SQ_INSTS_VMEM_RD", 0.0) / w, c.get("SQ_INSTS_VMEM_WR", 0.0) / w,
+                     c.get("SQ_WAIT_ANY", 0.0) / max(c.get("SQ_WAVE_CYCLES", 0.0), 1.0),
+                     c.get("SQ_LDS_BANK_CONFLICT", 0.0) / max(c.get("SQ_LDS_IDX_ACTIVE", 0.0), 1.0),
+                     c.get("SQ_INSTS_SALU", 0.0) / nd))
+    rows.sort(reverse=True)
+    lines = ["| kernel | waves/launch | VALU/wave | SALU/wave | LDS/wave | VMEM rd/wave "
+             "| VMEM wr/wave | wait share | LDS conflicts | VALU/launch (M) | VALU issue ms "
+             "| SALU/launch (M) |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for valu, k, nd, wpl, vw, sw, lw, rw, ww, wait, bank, salu in rows[:16]:
+        lines.append(f"| `{k}` | {wpl:.0f} | {vw:.0f} | {sw:.0f} | {lw:.0f} | {rw:.1f} | "
+                     f"{ww:.1f} | {wait:.2f} | {bank:.2f} | {valu / 1e6:.1f} | "
+                     f"{valu / cus / clock_ghz / 1e6:.3f} | {salu / 1e6:.1f} |")
+    return "\n".join(
+        [f"SQ counters per kernel ({voxels} voxels; wait share = SQ_WAIT_ANY / SQ_WAVE_CYCLES; LDS "
+         "conflicts = SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE; VALU issue ms = VALU "
+         f"wave-instructions per launch / ({cus} CUs x {clock_ghz} GHz), one wave64 VALU "
+         "instruction per CU per clock).", ""] + lines)
+
+
 def load_traffic(kernel):
     """HBM bytes per launch (fetch + write) of `kernel` from the committed counter summary, or
     None when it is absent."""
@@ -368,6 +419,9 @@ def main():
                     help="print the markdown HBM traffic summary of the rocprofv3 --pmc runs under "
                          "DIR (pmc_copy_fetch, pmc_copy_write, pmc_net_fetch, pmc_net_write; "
                          "--copy-bytes gives the calibration size) and exit")
+    ap.add_argument("--sq-summary", default=None, metavar="DIR",
+                    help="print the markdown SQ instruction-counter summary of the rocprofv3 --pmc "
+                         "runs under DIR (pmc_sq_a, pmc_sq_b) and exit")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run an elementwise negation that reads and writes this many bytes, "
                          "5 times (a known-size copy kernel to calibrate rocprofv3 FETCH_SIZE / "
@@ -380,6 +434,9 @@ def main():
 
     if args.pmc_summary:
         print(pmc_summary(args.pmc_summary, args.copy_bytes, args.size ** 3))
+        return
+    if args.sq_summary:
+        print(sq_summary(args.sq_summary, args.size ** 3))
         return
     import torch
 
