@@ -657,125 +657,190 @@ __device__ __forceinline__ void seam_pair(unsigned long long* tab, int* L, int a
   }
   uf_unite(L, (int)lo, (int)hi);
 }
-__device__ __forceinline__ void seam_flush(unsigned long long* tab, int* L) {
-  __syncthreads();
-  for (int s = threadIdx.y * 64 + threadIdx.x; s < SEAM_TABLE; s += 256) {
-    const unsigned long long k = tab[s];
-    if (k != kNoPair) uf_unite(L, (int)(k >> 32), (int)(uint32_t)k);
+// Union of a and b (a < b) whose parents pa = L[a], pb = L[b] were read beforehand: when both
+// were roots the union is one atomic, else the general union.
+__device__ __forceinline__ void seam_unite_from(int* L, int a, int b, int pa, int pb, int old) {
+  if (pa == a && pb == b) {
+    if (old != b) uf_unite(L, a, old);  // b was hooked meanwhile: a joins what b joined
+    return;
   }
+  uf_unite(L, a, b);
+}
+// The block's collected pairs (two slots per thread of the 256), united with all first-parent
+// reads and then all root hooks of a thread in flight together.
+__device__ __forceinline__ void seam_flush(unsigned long long* tab, int* L) {
+  static_assert(SEAM_TABLE == 2 * 256, "two table slots per thread");
+  __syncthreads();
+  const int t = threadIdx.x;
+  const unsigned long long k0 = tab[t], k1 = tab[t + 256];
+  const bool v0 = k0 != kNoPair, v1 = k1 != kNoPair;
+  const int a0 = v0 ? (int)(k0 >> 32) : 0, b0 = v0 ? (int)(uint32_t)k0 : 0;
+  const int a1 = v1 ? (int)(k1 >> 32) : 0, b1 = v1 ? (int)(uint32_t)k1 : 0;
+  const int pa0 = L[a0], pb0 = L[b0], pa1 = L[a1], pb1 = L[b1];
+  const bool r0 = v0 && pa0 == a0 && pb0 == b0, r1 = v1 && pa1 == a1 && pb1 == b1;
+  int o0 = b0, o1 = b1;
+  if (r0) o0 = atomicMin(L + b0, a0);
+  if (r1) o1 = atomicMin(L + b1, a1);
+  if (v0) seam_unite_from(L, a0, b0, pa0, pb0, o0);
+  if (v1) seam_unite_from(L, a1, b1, pa1, pb1, o1);
 }
 
 // Full run-union rule on the crossing forward rows of row chunks, one row per wave at a time.
-// mode 0: rows at a y boundary (boundary row blockIdx.y), slices blockIdx.z * 16 .. + 15;
-// mode 1: slices at a z boundary (blockIdx.z), rows of y tile blockIdx.y not at a y boundary.
-__global__ __launch_bounds__(256) void k_ccl_seams_rows(int* L, int nx, int ny, int nz, int mode,
-                                                        int nsel) {
-  __shared__ unsigned long long tab[SEAM_TABLE];
-  const int lane = threadIdx.x, wave = threadIdx.y;
-  for (int s = wave * 64 + lane; s < SEAM_TABLE; s += 256) tab[s] = kNoPair;
-  __syncthreads();
-  const int x0 = blockIdx.x * 64;
+// mode 0: rows at a y boundary (boundary row by), slices bz * 16 .. + 15;
+// mode 1: slices at a z boundary (bz), rows of y tile by not at a y boundary.
+// A row's contacts with a crossing forward row: each voxel with the voxel of the same column,
+// and a run's first and last voxel with the columns just outside the run (interior columns of
+// a run reach those through their own same-column contact). Those outside columns are held by
+// the neighbouring lanes, so only the chunk's edge columns are loaded, together with the rows:
+// one round of loads per row.
+__device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L, int nx, int ny,
+                                                int nz, int mode, int nsel, int bx, int by, int bz,
+                                                int lane, int wave) {
+  const int x0 = bx * 64;
   const int x = x0 + lane;
   const bool in = x < nx;
   const int xlast = min(x0 + 63, nx - 1);
-  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
+  // the column outside the chunk this lane may need: lane 0 the one before the chunk, the last
+  // column the one after it (a one-column chunk loads the latter when it needs it)
+  const bool edge = in && (lane == 0 || x == xlast);
+  const int xe = lane == 0 ? wm1(x, nx) : wp1(x, nx);
   const int nrow = mode == 0 ? 16 : TILE_Y;
-  for (int k = wave; k < nrow; k += 4) {
+  struct Row {
+    int fy[4], fz[4], fwd[4], fe[4], my;
+    bool same[4], ok;
+  };
+  // Row k of the block's share: its crossing forward rows, and the loads (all unconditional; a
+  // row not taken, and rows not needed, re-read L[0] or the row itself).
+  auto load = [&](int k, Row& R) {
     int y, z;
+    bool ok = k < nrow;
     if (mode == 0) {
-      if ((int)blockIdx.y >= nsel) break;  // block-uniform
-      y = y_boundary_row(blockIdx.y, ny);
-      z = blockIdx.z * 16 + k;
-      if (z >= nz) break;  // wave-uniform
+      y = y_boundary_row(by < nsel ? by : 0, ny);
+      z = bz * 16 + k;
+      ok = ok && by < nsel && z < nz;
     } else {
-      y = blockIdx.y * TILE_Y + k;
-      if (y >= ny) break;  // wave-uniform
-      if (y_boundary(y, ny)) continue;
-      z = z_boundary_slice(blockIdx.z, nz);
+      y = by * TILE_Y + k;
+      ok = ok && y < ny && !y_boundary(y, ny);
+      z = z_boundary_slice(bz, nz);
     }
-    int fy[4], fz[4];
-    bool same[4];
-    seam_rows(y, z, ny, nz, fy, fz, same);
-    if (same[0] && same[1] && same[2] && same[3]) continue;
-    // the row and its crossing forward rows are loaded together (rows not needed re-read the
-    // row itself), then used
-    const int i = in ? (z * ny + y) * nx + x : 0;
-    int fwd[4];
+    if (!ok) y = z = 0;
+    seam_rows(y, z, ny, nz, R.fy, R.fz, R.same);
+    ok = ok && !(R.same[0] && R.same[1] && R.same[2] && R.same[3]);
+    R.ok = ok;  // wave-uniform
+    const int i = ok && in ? (z * ny + y) * nx + x : 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) fwd[r] = L[(same[r] || !in) ? i : (fz[r] * ny + fy[r]) * nx + x];
-    const int my = in ? L[i] : -1;
+    for (int r = 0; r < 4; ++r) {
+      const int rb = (R.fz[r] * ny + R.fy[r]) * nx;
+      const bool skip = !ok || R.same[r] || !in;
+      R.fwd[r] = L[skip ? i : rb + x];
+      R.fe[r] = L[(skip || !edge) ? i : rb + xe];
+    }
+    R.my = ok && in ? L[i] : -1;
+  };
+  auto pairs = [&](const Row& R) {
+    if (!R.ok) return;
+    const int my = R.my;
     const bool mk = my != -1;
     const uint64_t m = __ballot(mk);
-    if (m == 0) continue;
+    if (m == 0) return;
     const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
     const bool run_end = mk && (x == xlast || !((m >> (lane + 1)) & 1ull));
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      if (same[r]) continue;  // wave-uniform; edges of same-tile rows: k_ccl_seams_edges
-      const int rb = (fz[r] * ny + fy[r]) * nx;
-      const int c = in ? fwd[r] : -1;
+      if (R.same[r]) continue;  // wave-uniform; edges of same-tile rows: the x-face pass
+      const int c = in ? R.fwd[r] : -1;
       const bool cm = c != -1;
       const int prev_c = __shfl_up(c, 1, 64);
+      const int next_c = __shfl_down(c, 1, 64);
       if (mk && cm) {
         const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
         if (!dup) seam_pair(tab, L, my, c);
       }
       if (mk && !cm) {
         if (run_start) {
-          const int a = L[rb + xm];
+          const int a = lane == 0 ? R.fe[r] : prev_c;
           if (a != -1) seam_pair(tab, L, my, a);
         }
         if (run_end) {
-          const int b = L[rb + xp];
+          const int b = x != xlast ? next_c
+                                   : (lane != 0 ? R.fe[r]
+                                                : L[(R.fz[r] * ny + R.fy[r]) * nx + wp1(x, nx)]);
           if (b != -1) seam_pair(tab, L, my, b);
         }
       }
     }
+  };
+  // two rows of the wave at a time: the second row's loads are in flight while the first's
+  // pairs are collected
+  for (int k = wave; k < nrow; k += 8) {
+    Row A, B;
+    load(k, A);
+    load(k + 4, B);
+    pairs(A);
+    pairs(B);
   }
-  seam_flush(tab, L);
 }
 
 // One thread per row chunk, from the x-face labels k_tile wrote (entry (chunk, z, y): labels of
 // the chunk's first and last column): the +x relation at the chunk start, and for forward rows
 // in the same tile the chunk's first column with column x-1 and its last column with column
 // x+1. 256 consecutive rows of one chunk per block share a pair set.
-__global__ __launch_bounds__(256) void k_ccl_seams_edges(int* L, const int2* __restrict__ xf,
-                                                         int ny, int nz, int cx) {
+__device__ __forceinline__ void seam_edges_block(unsigned long long* tab, int* L,
+                                                 const int2* __restrict__ xf, int ny, int nz,
+                                                 int cx, int64_t blk, int tid) {
+  const int64_t per_chunk = (int64_t)ny * nz;
+  const int64_t t = blk * 256 + tid;
+  if (t >= (int64_t)cx * per_chunk) return;
+  const int bx = (int)(t / per_chunk);
+  const int r = (int)(t - (int64_t)bx * per_chunk);
+  const int z = r / ny, y = r - z * ny;
+  const int bxm = bx > 0 ? bx - 1 : cx - 1, bxp = bx + 1 < cx ? bx + 1 : 0;
+  auto face = [&](int b, int zz, int yy) { return xf[((int64_t)b * nz + zz) * ny + yy]; };
+  const int2 me = face(bx, z, y);
+  if (me.x == -1 && me.y == -1) return;
+  if (me.x != -1) {
+    const int left = face(bxm, z, y).y;
+    if (left != -1) seam_pair(tab, L, me.x, left);
+  }
+  int fy[4], fz[4];
+  bool same[4];
+  seam_rows(y, z, ny, nz, fy, fz, same);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!same[q]) continue;
+    const int2 f = face(bx, fz[q], fy[q]);
+    if (me.x != -1 && f.x == -1) {
+      const int a = face(bxm, fz[q], fy[q]).y;
+      if (a != -1) seam_pair(tab, L, me.x, a);
+    }
+    if (me.y != -1 && f.y == -1) {
+      const int b = face(bxp, fz[q], fy[q]).x;
+      if (b != -1) seam_pair(tab, L, me.y, b);
+    }
+  }
+}
+
+// All cross-tile and wrapping marker relations in one launch: blocks [0, n0) take the rows at
+// a y boundary, [n0, n0 + n1) the slices at a z boundary, the rest the x-face relations.
+__global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restrict__ xf, int nx,
+                                                   int ny, int nz, int cx, int ysel, int zsel,
+                                                   int64_t n0, int64_t n1) {
   __shared__ unsigned long long tab[SEAM_TABLE];
   const int tid = threadIdx.x;
   for (int s = tid; s < SEAM_TABLE; s += 256) tab[s] = kNoPair;
   __syncthreads();
-  const int64_t per_chunk = (int64_t)ny * nz;
-  const int64_t t = (int64_t)blockIdx.x * 256 + tid;
-  if (t < (int64_t)cx * per_chunk) {
-    const int bx = (int)(t / per_chunk);
-    const int r = (int)(t - (int64_t)bx * per_chunk);
-    const int z = r / ny, y = r - z * ny;
-    const int bxm = bx > 0 ? bx - 1 : cx - 1, bxp = bx + 1 < cx ? bx + 1 : 0;
-    auto face = [&](int b, int zz, int yy) { return xf[((int64_t)b * nz + zz) * ny + yy]; };
-    const int2 me = face(bx, z, y);
-    if (me.x != -1 || me.y != -1) {
-      if (me.x != -1) {
-        const int left = face(bxm, z, y).y;
-        if (left != -1) seam_pair(tab, L, me.x, left);
-      }
-      int fy[4], fz[4];
-      bool same[4];
-      seam_rows(y, z, ny, nz, fy, fz, same);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (!same[q]) continue;
-        const int2 f = face(bx, fz[q], fy[q]);
-        if (me.x != -1 && f.x == -1) {
-          const int a = face(bxm, fz[q], fy[q]).y;
-          if (a != -1) seam_pair(tab, L, me.x, a);
-        }
-        if (me.y != -1 && f.y == -1) {
-          const int b = face(bxp, fz[q], fy[q]).x;
-          if (b != -1) seam_pair(tab, L, me.y, b);
-        }
-      }
-    }
+  const int64_t b = blockIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  if (b < n0) {  // grid (cx, ysel, ceil(nz / 16))
+    const int bx = (int)(b % cx), rest = (int)(b / cx);
+    seam_rows_block(tab, L, nx, ny, nz, 0, ysel, bx, rest % ysel, rest / ysel, lane, wave);
+  } else if (b < n0 + n1) {  // grid (cx, ceil(ny / TILE_Y), zsel)
+    const int64_t bb = b - n0;
+    const int nyt = (ny + TILE_Y - 1) / TILE_Y;
+    const int bx = (int)(bb % cx), rest = (int)(bb / cx);
+    seam_rows_block(tab, L, nx, ny, nz, 1, zsel, bx, rest % nyt, rest / nyt, lane, wave);
+  } else {
+    seam_edges_block(tab, L, xf, ny, nz, cx, b - n0 - n1, tid);
   }
   seam_flush(tab, L);
 }
@@ -959,16 +1024,13 @@ void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
 }
 
 void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
-  const unsigned cx = cdiv(v.nx, 64);
+  const int cx = (int)cdiv(v.nx, 64);
   const int ysel = 2 * (int)cdiv(v.ny, TILE_Y), zsel = (int)cdiv(v.nz, TILE_Z);
-  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows, dim3(cx, (unsigned)ysel, cdiv(v.nz, 16)),
-             dim3(64, 4), 0, labels, v.nx, v.ny, v.nz, 0, ysel);
-  PNM_LAUNCH(c, "k_ccl_seams_rows", k_ccl_seams_rows,
-             dim3(cx, cdiv(v.ny, TILE_Y), (unsigned)zsel), dim3(64, 4), 0, labels, v.nx, v.ny,
-             v.nz, 1, zsel);
-  PNM_LAUNCH(c, "k_ccl_seams_edges", k_ccl_seams_edges,
-             dim3(cdiv((int64_t)cx * v.ny * v.nz, 256)), dim3(256), 0, labels,
-             reinterpret_cast<const int2*>(xface), v.ny, v.nz, (int)cx);
+  const int64_t n0 = (int64_t)cx * ysel * cdiv(v.nz, 16);
+  const int64_t n1 = (int64_t)cx * cdiv(v.ny, TILE_Y) * zsel;
+  const int64_t n2 = cdiv((int64_t)cx * v.ny * v.nz, 256);
+  PNM_LAUNCH(c, "k_ccl_seams", k_ccl_seams, dim3((unsigned)(n0 + n1 + n2)), dim3(256), 0, labels,
+             reinterpret_cast<const int2*>(xface), v.nx, v.ny, v.nz, cx, ysel, zsel, n0, n1);
 }
 
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* stage,
