@@ -99,6 +99,14 @@ __device__ __forceinline__ int cs_u(uint32_t w) { return (int)(w & 0x1fffu); }
 __device__ __forceinline__ int cs_s(uint32_t w) { return (int)((w >> 13) & 0x3ffu); }
 __device__ __forceinline__ uint32_t cs_st(uint32_t w) { return w >> 23; }
 
+// Staged walk that left its tile: its walk record (target, steps; bits 0..49) with the voxel's
+// slice and column in its tile row in bits 50..59.
+__device__ __forceinline__ uint64_t open_entry(uint64_t rec, int zl, int xl) {
+  return rec | ((uint64_t)((zl << 6) | xl) << 50);
+}
+__device__ __forceinline__ int open_slice(uint64_t e) { return (int)((e >> 56) & 15u); }
+__device__ __forceinline__ int open_column(uint64_t e) { return (int)((e >> 50) & 63u); }
+
 // direction code d = (dz+1)*9 + (dy+1)*3 + (dx+1)
 __device__ __forceinline__ void dir_delta(int d, int& dx, int& dy, int& dz) {
   dz = d / 9 - 1;
@@ -472,7 +480,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   bool capped_any = false;
   // Output of the walk of sdf>0 voxel l (slice zl, column xl of this wave's row); returns
   // whether the walk is still open.
-  auto walk_out = [&](int l, int zl, int xl) -> bool {
+  auto walk_out = [&](int l, int zl, int xl, uint64_t& rec) -> bool {
     const int xg = x0 + xl;
     const int i = srow + xl + mul(zl, plane_n);
     int target, steps, fin_u = -1;
@@ -538,13 +546,15 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     }
     const bool boundary = xg == x0 || xg == xlast || y == y0 || y == ylast || zl == 0 ||
                           zl == nzt - 1;
-    if (!fin || boundary) o.walk[i] = walk_record(target, steps, fin, capped);
+    rec = walk_record(target, steps, fin, capped);
+    if (boundary) o.walk[i] = rec;
     return !fin;
   };
   // The wave's sdf>0 voxels as a compact list in (slice, lane) order, in the key buffers (free
   // after the march): the wave then takes ceil(count / 64) turns instead of its busiest lane's
   // count of slices. A wave with more voxels than its share of the buffers uses the per-lane
-  // loop. Open walks are staged in the same (slice, lane) order either way.
+  // loop. Open walks are staged with their walk record (open_entry), so the resolver needs no
+  // record load of its own and no record is written for an open walk off the tile boundary.
   constexpr int PLIST_W = (int)(sizeof(KB) / sizeof(uint16_t)) / TILE_Y;
   uint16_t* plist = reinterpret_cast<uint16_t*>(KB) + wave * PLIST_W;
   unsigned np = 0;
@@ -560,26 +570,27 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     for (unsigned j0 = 0; j0 < np; j0 += 64) {
       const unsigned j = j0 + lane;
       const int li = j < np ? (int)plist[j] : 0;
-      const bool open = j < np && walk_out(li, li >> 9, li & 63);
+      uint64_t rec = 0;
+      const bool open = j < np && walk_out(li, li >> 9, li & 63, rec);
       const uint64_t om = __ballot(open);
       if (open)
         o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
-            (uint32_t)(srow + mul(li >> 9, plane_n) + (li & 63));
+            open_entry(rec, li >> 9, li & 63);
       nop += (unsigned)__popcll(om);
     }
   } else {
-    uint32_t open_mask = 0;
-    for (uint32_t m = pore_mask; m; m &= m - 1) {
-      const int zl = __ffs(m) - 1;
-      if (walk_out((zl * TILE_Y + wave) * 64 + lane, zl, lane)) open_mask |= 1u << zl;
-    }
-    for (int zl = 0; zl < nzt; ++zl) {
-      const bool open = (open_mask >> zl) & 1u;
+    // wave-uniform trip count (the busiest lane's), so that nop stays uniform
+    uint32_t m = pore_mask;
+    while (__any(m != 0)) {
+      const bool has = m != 0;
+      const int zl = has ? __ffs(m) - 1 : 0;
+      uint64_t rec = 0;
+      bool open = false;
+      if (has) open = walk_out((zl * TILE_Y + wave) * 64 + lane, zl, lane, rec);
       const uint64_t om = __ballot(open);
-      if (open)
-        o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
-            (uint32_t)(srow + lane + mul(zl, plane_n));
+      if (open) o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] = open_entry(rec, zl, lane);
       nop += (unsigned)__popcll(om);
+      if (has) m &= m - 1;
     }
   }
   if (__any(capped_any) && lane == 0) atomicOr(o.cap_flag, 1ull);
@@ -886,7 +897,7 @@ void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
 constexpr int RESOLVE_WALKS = 4;
 __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__ sdf, int nx,
                                                        int ny, int nz, int64_t n,
-                                                       const uint32_t* __restrict__ stage,
+                                                       const uint64_t* __restrict__ stage,
                                                        const uint32_t* __restrict__ cnt,
                                                        int64_t nseg, int seg_cap,
                                                        const uint64_t* __restrict__ walk,
@@ -898,7 +909,14 @@ __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__
   const int64_t seg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (seg >= nseg) return;  // wave-uniform; the kernel has no block barrier
   const int count = (int)cnt[seg];
-  const uint32_t* __restrict__ list = stage + seg * seg_cap;
+  const uint64_t* __restrict__ list = stage + seg * seg_cap;
+  // the segment's tile row: seg = ((bz * gy + by) * gx + bx) * TILE_Y + row
+  const int gx = (nx + 63) / 64, gy = (ny + TILE_Y - 1) / TILE_Y;
+  const int64_t tr = seg / TILE_Y;
+  const int row = (int)(seg - tr * TILE_Y);
+  const int bx = (int)(tr % gx), by = (int)((tr / gx) % gy), bz = (int)(tr / ((int64_t)gx * gy));
+  const int plane_n = nx * ny;
+  const int rowbase = ((bz * TILE_Z) * ny + by * TILE_Y + row) * nx + bx * 64;
   unsigned long long flags = 0;
   for (int base = 0; base < count; base += 64 * K) {
     int vox[K], w[K], t[K], root[K];
@@ -906,15 +924,12 @@ __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int jj = base + k * 64 + lane;
-      vox[k] = jj < count ? (int)list[jj] : 0;
+      const uint64_t e = jj < count ? list[jj] : 0ull;  // the walk record and its voxel
+      vox[k] = jj < count ? rowbase + open_slice(e) * plane_n + open_column(e) : 0;
+      w[k] = walk_target(e);
+      t[k] = walk_steps(e);
       root[k] = -1;
       if (jj < count) live |= 1u << k;
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint64_t own = walk[vox[k]];  // vox = 0 for a missing walk: a valid, ignored load
-      w[k] = walk_target(own);
-      t[k] = walk_steps(own);
     }
     while (live) {
       uint64_t rec[K];
@@ -1033,7 +1048,7 @@ void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
              reinterpret_cast<const int2*>(xface), v.nx, v.ny, v.nz, cx, ysel, zsel, n0, n1);
 }
 
-void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* stage,
+void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint64_t* stage,
                           const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
                           int32_t* roots, int32_t* min_pore, uint64_t* cap_flag) {
   if (nseg <= 0) return;

@@ -87,6 +87,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   const int64_t stage_words = std::max(nseg * scap, nb1 * 256);
   uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_c = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
+  uint64_t* stage_o = c->arena.get_n<uint64_t>((size_t)(nseg * scap));  // open walks
   // counts: open walks from the tile pass; [solid keys | band] from the flood tile, contiguous
   // for one multi-list compaction
   uint32_t* cnt_c = c->arena.get_n<uint32_t>((size_t)nseg);
@@ -118,15 +119,14 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   o.xface = reinterpret_cast<int2*>(xface);
   o.lroot_stage = stage_r;
   o.lroot_cnt = cnt_r;
-  o.open_stage = stage_c;
+  o.open_stage = stage_o;
   o.open_cnt = cnt_c;
   o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
   launch_tile(c, sdf, v, thr, o);
   bitmap_list(c, peak_map, nwords, nwx, v.nx, peaks, ctr + 0);  // peaks in voxel order
-  // the walks that left their tile, read where the tile pass staged them (before the flood
-  // tile reuses that staging)
+  // the walks that left their tile, read where the tile pass staged them
   if (!labels_only)
-    launch_roots_resolve(c, sdf, v, stage_c, cnt_c, nseg, (int)scap, walk, roots, min_pore,
+    launch_roots_resolve(c, sdf, v, stage_o, cnt_c, nseg, (int)scap, walk, roots, min_pore,
                          ctr + 3);
   launch_ccl_seams(c, v, lab_ccl, xface);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);

@@ -186,7 +186,7 @@ struct TileOut {
   int2* xface;            // labels of each row chunk's first/last column, entry (chunk, z, y)
   uint32_t* lroot_stage;  // staged tile-local marker roots
   uint32_t* lroot_cnt;
-  uint32_t* open_stage;   // staged sdf>0 voxels whose walk left the tile
+  uint64_t* open_stage;   // staged walks that left the tile: record | (slice, column) << 50
   uint32_t* open_cnt;
   unsigned long long* cap_flag;  // set when a walk stops at the cap on a non-fixed point
 };
@@ -201,7 +201,7 @@ void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
                           const uint32_t* cnt);
 // Walks that left their tile, read from the tile pass's staging (nseg segments of seg_cap slots,
 // counts in cnt): one wave per segment.
-void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint32_t* stage,
+void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint64_t* stage,
                           const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
                           int32_t* roots, int32_t* min_pore, uint64_t* cap_flag);
 void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots);
