@@ -108,6 +108,29 @@ KTimer::~KTimer() {
   c->recs.push_back({name, a, b});
 }
 
+OnSide::OnSide(Ctx* c) : ctx(c), main(c->stream) {
+  if (!c->side) {
+    PNM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    PNM_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    PNM_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  PNM_HIP(hipEventRecord(c->ev_fork, main));
+  PNM_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  c->stream = c->side;
+  c->side_pending = true;
+}
+
+OnSide::~OnSide() {
+  (void)hipEventRecord(ctx->ev_join, ctx->side);
+  ctx->stream = main;
+}
+
+void join_side(Ctx* c) {
+  if (!c->side_pending) return;
+  c->side_pending = false;
+  PNM_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+}
+
 CallScope::CallScope(Ctx* c) : ctx(c), prev_device(-1) {
   if (hipGetDevice(&prev_device) != hipSuccess) prev_device = -1;
   if (prev_device != c->device) PNM_HIP(hipSetDevice(c->device));
@@ -115,6 +138,10 @@ CallScope::CallScope(Ctx* c) : ctx(c), prev_device(-1) {
 }
 
 CallScope::~CallScope() {
+  if (ctx->side_pending) {
+    ctx->side_pending = false;
+    (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+  }
   if (prev_device >= 0 && prev_device != ctx->device) (void)hipSetDevice(prev_device);
 }
 
@@ -249,6 +276,12 @@ void pnm_ctx_destroy(pnm_ctx_t ctx) {
   if (c->err_host) (void)hipHostFree(c->err_host);
   if (c->lb_state) (void)hipFree(c->lb_state);
   if (c->lb_ctl) (void)hipFree(c->lb_ctl);
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete c;

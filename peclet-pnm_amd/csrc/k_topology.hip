@@ -16,11 +16,16 @@ constexpr int TOPO_TABLE = 512;     // LDS hash slots (4 KiB): a brick has ~50 d
 constexpr int TOPO_PROBES = 32;     // linear-probe limit before a key bypasses the table
 constexpr uint64_t kEmpty = ~0ull;
 
-__device__ __forceinline__ uint32_t hash64(uint64_t k) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdull;
-  k ^= k >> 33;
-  return (uint32_t)k;
+// Slot of a pair key in the brick's table (9 bits), from full-rate 24-bit multiplies only (a
+// 32- or 64-bit integer multiply issues at a quarter of the VALU rate). The top bits of a
+// 24 x 24-bit product's low word depend on all 24 bits of each factor. __umul24 returns a
+// signed int: the product is taken as unsigned before the shift, which must be logical (an
+// arithmetic shift would give a negative slot, an index outside the table).
+static_assert(TOPO_TABLE == 512, "slot_hash returns 9 bits");
+__device__ __forceinline__ uint32_t slot_hash(uint64_t k) {
+  uint32_t x = (uint32_t)k ^ (uint32_t)__umul24((uint32_t)(k >> 32), 0x9E3779u);
+  x ^= x >> 16;
+  return (uint32_t)__umul24(x, 0x85EBCBu) >> 23;
 }
 
 // Each block deduplicates the differing-face pairs of its brick: a lane whose key equals the
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
       const bool need = key != kEmpty && key != last[k] && !(lane > 0 && prev == key);
       last[k] = key;
       if (!__any(need)) continue;  // wave-uniform
-      const uint32_t h = hash64(key) & (TOPO_TABLE - 1);
+      const uint32_t h = slot_hash(key);
       bool placed = true;
       if (need) {
         const unsigned long long cur = table[h];

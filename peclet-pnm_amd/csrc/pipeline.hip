@@ -124,10 +124,14 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   o.cap_flag = reinterpret_cast<unsigned long long*>(ctr + 3);
   launch_tile(c, sdf, v, thr, o);
   bitmap_list(c, peak_map, nwords, nwx, v.nx, peaks, ctr + 0);  // peaks in voxel order
-  // the walks that left their tile, read where the tile pass staged them
-  if (!labels_only)
+  // the walks that left their tile, read where the tile pass staged them. The resolver shares
+  // no buffer with the CCL and flood below (it writes roots, min_pore and the cap-flag word
+  // ctr[3]), so it runs beside them on the second stream, joined before the counters are read.
+  if (!labels_only) {
+    OnSide side(c);
     launch_roots_resolve(c, sdf, v, stage_o, cnt_c, nseg, (int)scap, walk, roots, min_pore,
                          ctr + 3);
+  }
   launch_ccl_seams(c, v, lab_ccl, xface);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
   if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
@@ -144,6 +148,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
     uint64_t* tots[2] = {ctr + 2, ctr + 8};
     compact_staged_multi(c, 2, st, cnt_pair2, nseg, scap, outs, tots);
   }
+  join_side(c);
   uint64_t h[9];
   readback(c, ctr, 9, h);
   const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2], nband = (int64_t)h[8];

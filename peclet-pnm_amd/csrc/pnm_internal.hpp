@@ -76,6 +76,12 @@ struct Ctx {
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
   hipEvent_t take_event();
+  // Second stream (default priority) for work that shares no buffers with the main chain, and
+  // the events that fork it from `stream` and join it back; created on first use. side_pending:
+  // work was forked and the main stream has not been made to wait for it yet.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool side_pending = false;
   // Batched segmentation: autotuned launch shape and the cached graph of the last batch.
   bool batch_cfg_valid = false;
   int batch_block = 1024, batch_per_cu = 2, batch_groups = 0;
@@ -103,6 +109,17 @@ struct KTimer {
     hipLaunchKernelGGL(kern, grid, block, shm, (c)->stream, __VA_ARGS__);        \
     PNM_HIP(hipGetLastError());                                                  \
   } while (0)
+
+// Launches made while an OnSide lives go to the context's second stream, after everything
+// queued on the main stream so far. join_side makes the main stream wait for them (the end of
+// every call does it too, so no forked work outlives the call that forked it).
+struct OnSide {
+  explicit OnSide(Ctx* c);
+  ~OnSide();
+  Ctx* ctx;
+  hipStream_t main;
+};
+void join_side(Ctx* c);
 
 // RAII: make ctx->device current, reset the arena for a new call.
 struct CallScope {
