@@ -292,11 +292,15 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   __syncthreads();
   // peak masks: one word per (row, 64-column chunk), rows in flat order, so the set bits of the
   // whole map in word order are the peaks in ascending voxel order
+  {
+    // word (z, y, chunk): 32-bit index math (the map has fewer than 2^31 words), uniform per wave
+    const int pstep = ny * (int)gridDim.x;
+    int pw = (z0 * ny + y) * (int)gridDim.x + (int)blockIdx.x;
 #pragma unroll
-  for (int zl = 0; zl < TILE_Z; ++zl) {
-    const uint64_t pm = __ballot((peak_mask >> zl) & 1u);
-    if (lane == 0 && zl < nzt && y < ny)
-      o.peak_map[((int64_t)(z0 + zl) * ny + y) * gridDim.x + blockIdx.x] = pm;
+    for (int zl = 0; zl < TILE_Z; ++zl, pw += pstep) {
+      const uint64_t pm = __ballot((peak_mask >> zl) & 1u);
+      if (lane == 0 && zl < nzt && y < ny) o.peak_map[pw] = pm;
+    }
   }
 
   // ---- CCL inside the tile: forward rows (y+1,z), (y-1,z+1), (y,z+1), (y+1,z+1) ----
@@ -346,25 +350,28 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       for (int zl = 0; zl < TILE_Z; ++zl) p[zl] = LAB[r[zl] >= 0 ? r[zl] : 0];
     }
     unsigned nr = 0;
+    uint32_t* const lstage = o.lroot_stage + seg_id * T_SEG;
+    // x-face entry (chunk, z, y): 32-bit index math (fewer than 2^31 entries)
+    int xw = ((int)blockIdx.x * nz + z0) * ny + y;
 #pragma unroll
-    for (int zl = 0; zl < TILE_Z; ++zl) {
+    for (int zl = 0; zl < TILE_Z; ++zl, xw += ny) {
       if (zl >= nzt) break;
       const int l = (zl * TILE_Y + wave) * 64 + lane;
-      const int i = srow + lane + mul(zl, plane_n);
+      const int i = srow + lane + mul(zl, plane_n);  // = gidx(l)
       if (col_in) {
         o.labels[i] = r[zl] == -1 ? -1 : gidx(r[zl]);
         if (r[zl] == l) o.min_solid[i] = i;
       }
       const bool lroot = col_in && r[zl] == l;
       const uint64_t rm = __ballot(lroot);
-      if (lroot) o.lroot_stage[seg_id * T_SEG + nr + __popcll(rm & below)] = (uint32_t)gidx(l);
+      if (lroot) lstage[nr + __popcll(rm & below)] = (uint32_t)i;
       nr += (unsigned)__popcll(rm);
       const int a0 = __shfl(r[zl], 0), aL = __shfl(r[zl], xlast - x0);
       if (lane == 0 && y < ny) {
         int2 v;
         v.x = a0 == -1 ? -1 : gidx(a0);
         v.y = aL == -1 ? -1 : gidx(aL);
-        o.xface[((int64_t)blockIdx.x * nz + z0 + zl) * ny + y] = v;
+        o.xface[xw] = v;
       }
     }
     if (lane == 0) o.lroot_cnt[seg_id] = nr;
@@ -555,6 +562,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   // count of slices. A wave with more voxels than its share of the buffers uses the per-lane
   // loop. Open walks are staged with their walk record (open_entry), so the resolver needs no
   // record load of its own and no record is written for an open walk off the tile boundary.
+  uint64_t* const ostage = o.open_stage + seg_id * T_SEG;
   constexpr int PLIST_W = (int)(sizeof(KB) / sizeof(uint16_t)) / TILE_Y;
   uint16_t* plist = reinterpret_cast<uint16_t*>(KB) + wave * PLIST_W;
   unsigned np = 0;
@@ -574,7 +582,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const bool open = j < np && walk_out(li, li >> 9, li & 63, rec);
       const uint64_t om = __ballot(open);
       if (open)
-        o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] =
+        ostage[nop + __popcll(om & below)] =
             open_entry(rec, li >> 9, li & 63);
       nop += (unsigned)__popcll(om);
     }
@@ -588,7 +596,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       bool open = false;
       if (has) open = walk_out((zl * TILE_Y + wave) * 64 + lane, zl, lane, rec);
       const uint64_t om = __ballot(open);
-      if (open) o.open_stage[seg_id * T_SEG + nop + __popcll(om & below)] = open_entry(rec, zl, lane);
+      if (open) ostage[nop + __popcll(om & below)] = open_entry(rec, zl, lane);
       nop += (unsigned)__popcll(om);
       if (has) m &= m - 1;
     }
