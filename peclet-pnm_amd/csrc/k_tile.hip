@@ -902,7 +902,7 @@ void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
 // them: one wave per staging segment (a tile row's open walks in (slice, lane) order). Each
 // thread advances RESOLVE_WALKS walks in lockstep, so that many independent record loads are in
 // flight per thread; the rare step-by-step remainders run after all hops.
-constexpr int RESOLVE_WALKS = 4;
+constexpr int RESOLVE_WALKS = 2;
 __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__ sdf, int nx,
                                                        int ny, int nz, int64_t n,
                                                        const uint64_t* __restrict__ stage,
