@@ -127,7 +127,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   // the walks that left their tile, read where the tile pass staged them. The resolver shares
   // no buffer with the CCL and flood below (it writes roots, min_pore and the cap-flag word
   // ctr[3]) and no look-back scan state (a scan on each stream at once would race on it), so
-  // it runs beside them on the second stream, joined before the counters are read.
+  // it runs beside them on the second stream, joined before the counters are read back (so its
+  // outputs are complete when the call returns, as every output is).
   if (!labels_only) {
     OnSide side(c);
     launch_roots_resolve(c, sdf, v, stage_o, cnt_c, nseg, (int)scap, walk, roots, min_pore,
@@ -232,14 +233,8 @@ int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], cons
                        bool* pairs_overflow) {
   // the topology keys need only label bounds, so the id counts are not read back
   SegInternal s = segment_core(c, sdf, v, spc, nullptr, nullptr, nullptr, nullptr, false, false);
-  int64_t np;
-  {
-    // the pore centroids (sdf and the sorted peak list only; no sort, no scan state) run beside
-    // the seg + topology pass; the end of the call joins them
-    OnSide side(c);
-    np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, true, pores_d, cap_pores,
-                      total_pores);
-  }
+  const int64_t np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, true, pores_d,
+                                  cap_pores, total_pores);
   int64_t needed = 0;
   const int64_t nt = seg_topology(c, sdf, s.roots, s.labels, s.pore_id, s.solid_id, seg, v,
                                   -s.solid_count, s.pore_bound, pairs_d, cap_pairs, &needed);
