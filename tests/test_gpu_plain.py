@@ -230,6 +230,27 @@ def test_deterministic_and_staged_equals_fused(capi_ctx, packing128):
                                   bits(a[0]))
 
 
+def test_on_caller_stream(capi_ctx, packing128):
+    """The context on a caller's (non-default, default-priority) torch stream: the pipeline's
+    second stream is forked from and joined back to that stream, so two calls in a row, read
+    back on the same stream without any other synchronisation, equal the own-stream result."""
+    sdf = packing128
+    a = run_fused_d(capi_ctx, sdf, (0, 0, 0), (1, 1, 1))
+    st = torch.cuda.Stream()
+    try:
+        with torch.cuda.stream(st):
+            capi_ctx.set_stream(st.cuda_stream)
+            for _ in range(2):
+                b = run_fused_d(capi_ctx, sdf, (0, 0, 0), (1, 1, 1))
+                assert b[1] == a[1]
+                np.testing.assert_array_equal(bits(b[0]), bits(a[0]))
+                np.testing.assert_array_equal(b[2], a[2])
+                np.testing.assert_array_equal(b[3], a[3])
+    finally:
+        capi_ctx.set_stream(None)
+        torch.cuda.synchronize()
+
+
 def test_capacity_errors(capi_ctx):
     sdf = quantized((8, 8, 8), 5)
     s = dev(sdf)
