@@ -70,7 +70,9 @@ int pnm_device_count(void);
 pnm_status pnm_ctx_create(pnm_ctx_t* out, int device);
 void pnm_ctx_destroy(pnm_ctx_t ctx);
 /* Run this context's work on `stream` (a hipStream_t, e.g. torch's current stream); NULL
- * restores the context's own stream. */
+ * restores the context's own stream. The call first waits on the host for the work already
+ * queued on the old stream, so asynchronous calls made before the switch are done with the
+ * context's scratch before calls after it reuse that scratch. */
 pnm_status pnm_ctx_set_stream(pnm_ctx_t ctx, void* stream);
 void* pnm_ctx_stream(pnm_ctx_t ctx);
 /* Release the context's scratch arena (it regrows on the next call). */

@@ -24,6 +24,16 @@ void hip_check(hipError_t e, const char* what, const char* file, int line) {
 void* Arena::get(size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
   if (bytes == 0) bytes = 256;
+  if (base_) {
+    if (bytes > cap_ - off_)
+      fail(PNM_ECAPACITY, "fixed scratch exhausted: request of " + std::to_string(bytes) +
+                              " bytes at offset " + std::to_string(off_) + " of " +
+                              std::to_string(cap_));
+    void* p = base_ + off_;
+    off_ += bytes;
+    high_ = std::max(high_, off_);
+    return p;
+  }
   if (next_ < slots_.size()) {
     Slot& s = slots_[next_];
     if (s.size >= bytes) {
@@ -109,6 +119,8 @@ KTimer::~KTimer() {
 }
 
 OnSide::OnSide(Ctx* c) : ctx(c), main(c->stream) {
+  if (c->no_fork) return;
+  active = true;
   if (!c->side) {
     PNM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     PNM_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -121,6 +133,7 @@ OnSide::OnSide(Ctx* c) : ctx(c), main(c->stream) {
 }
 
 OnSide::~OnSide() {
+  if (!active) return;
   (void)hipEventRecord(ctx->ev_join, ctx->side);
   ctx->stream = main;
 }
@@ -290,7 +303,17 @@ void pnm_ctx_destroy(pnm_ctx_t ctx) {
 pnm_status pnm_ctx_set_stream(pnm_ctx_t ctx, void* stream) {
   return guarded([&] {
     Ctx* c = check_ctx(ctx);
-    c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    if (next == c->stream) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    PNM_HIP(hipSetDevice(c->device));
+    // The old stream is drained on the host: one event re-recorded on alternating streams and
+    // waited on by the other did not order the work on this runtime (measured: the backward
+    // pass on the new stream overlapped the forward pass still queued on the old one).
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    if (prev >= 0 && prev != c->device) (void)hipSetDevice(prev);
+    c->stream = next;
   });
 }
 
@@ -798,3 +821,4 @@ void pnm_flow_result_free(pnm_flow_result_t* r) {
 }
 
 }  // extern "C"
+

@@ -344,6 +344,9 @@ static void lb_reserve(Ctx* c, int64_t nb) {
   }
   if (nb > 0x7fffffff) fail(PNM_EINVAL, "scan: too many blocks");
   if (nb <= c->lb_cap) return;
+  if (c->lb_fixed)
+    fail(PNM_ECAPACITY, "scan: " + std::to_string(nb) + " blocks exceed the fixed look-back state (" +
+                            std::to_string(c->lb_cap) + ")");
   const int64_t cap = std::max<int64_t>(nb, std::max<int64_t>(4096, 2 * c->lb_cap));
   if (c->lb_state) {
     PNM_HIP(hipStreamSynchronize(c->stream));

@@ -155,27 +155,33 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   readback(c, ctr, 9, h);
   const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2], nband = (int64_t)h[8];
 
-  // Remaining Jacobi sweeps until a sweep labels nothing (pore_extraction.hpp:238-269). Band
-  // lengths live on the device (ctr[8 + s]); the host checks them every two sweeps.
-  // The counter block is read whole, so the walk-cap flag (set by the resolver, launched
-  // before the sweeps) comes back with the last band counts.
+  // Remaining Jacobi sweeps until a sweep labels nothing (pore_extraction.hpp:238-269), with
+  // no cap on their number. Band lengths live on the device in a ring of four counter words
+  // (ctr[8 + s % 4] is the length sweep s reads, ctr[8 + (s + 1) % 4] the one it writes); the
+  // host reads them every two sweeps and sizes the next two sweeps' grids by the last length
+  // (the band only shrinks). The counter block is read whole, so the walk-cap flag (set by the
+  // resolver, launched before the sweeps) comes back with the band lengths.
   bool have_flag = false;
   if (nband > 0) {
     uint32_t* cur = band;
     uint32_t* nxt = band2;
-    int s = 0;
+    int64_t grid = nband;
+    int64_t s = 0;
+    const size_t sweep_mark = c->arena.mark();  // every sweep pair reuses the same scratch
     while (true) {
-      if (s + 2 >= 54) fail(PNM_ENOCONVERGE, "flood did not converge within 54 sweeps");
+      c->arena.rewind(sweep_mark);
       for (int k = 0; k < 2; ++k, ++s) {
-        flood_sweep(c, v, labels, cur, ctr + 8 + s, nband, nxt, ctr + 8 + s + 1, newlab, stage_b,
-                    cnt_b, min_solid);
+        flood_sweep(c, v, labels, cur, ctr + 8 + (s & 3), grid, nxt, ctr + 8 + ((s + 1) & 3),
+                    newlab, stage_b, cnt_b, min_solid);
         std::swap(cur, nxt);
       }
-      uint64_t hh[64];
-      readback(c, ctr, 8 + s + 1, hh);
+      uint64_t hh[12];
+      readback(c, ctr, 12, hh);
       h[0] = hh[3];
       have_flag = true;
-      if (hh[8 + s] == hh[8 + s - 1] || hh[8 + s] == 0) break;
+      const uint64_t before = hh[8 + ((s - 1) & 3)], after = hh[8 + (s & 3)];
+      if (after == before || after == 0) break;
+      grid = (int64_t)after;
     }
   }
 

@@ -28,14 +28,36 @@ void hip_check(hipError_t e, const char* what, const char* file, int line);
 // Grow-only slot arena: the k-th request of a call reuses the k-th slot when it is big enough.
 // A pipeline makes the same sequence of requests every call, so after the first call of a given
 // size no hipMalloc/hipFree happens. Memory is never initialised by the arena.
+// A loop whose iterations make the same requests brackets them with mark()/rewind(), so its
+// iterations share slots (stream order keeps an iteration's reuse after the previous one).
+// Fixed mode (fix/unfix): requests are carved from one caller-given device region by a bump
+// offset instead, and a request beyond its end fails with PNM_ECAPACITY; nothing is allocated.
 class Arena {
  public:
-  void reset() { next_ = 0; }
+  void reset() {
+    next_ = 0;
+    off_ = 0;
+  }
   void* get(size_t bytes);
   template <class T>
   T* get_n(size_t count) {
     return static_cast<T*>(get(count * sizeof(T)));
   }
+  size_t mark() const { return base_ ? off_ : next_; }
+  void rewind(size_t m) {
+    if (base_)
+      off_ = m;
+    else
+      next_ = m;
+  }
+  void fix(void* base, size_t cap) {
+    base_ = static_cast<char*>(base);
+    cap_ = cap;
+    off_ = 0;
+    high_ = 0;
+  }
+  void unfix() { base_ = nullptr; }
+  size_t high_water() const { return high_; }  // largest bump offset reached in fixed mode
   void release();
   int64_t bytes() const;
 
@@ -46,6 +68,8 @@ class Arena {
   };
   std::vector<Slot> slots_;
   size_t next_ = 0;
+  char* base_ = nullptr;  // fixed mode when set
+  size_t cap_ = 0, off_ = 0, high_ = 0;
 };
 
 struct Ctx {
@@ -63,6 +87,10 @@ struct Ctx {
   uint64_t* lb_state = nullptr;
   int64_t lb_cap = 0;
   uint32_t* lb_ctl = nullptr;
+  // Set while a call runs on caller-provided fixed scratch (no allocation allowed): the
+  // look-back state must not grow and no second stream is forked.
+  bool lb_fixed = false;
+  bool no_fork = false;
   // Kernel timing: when `prof` is set every launch is bracketed by two HIP events on the
   // launch stream (only launches named in `prof_only`, newline-separated, when it is not
   // empty); pnm_ctx_profile_read aggregates them per kernel name.
@@ -113,11 +141,13 @@ struct KTimer {
 // Launches made while an OnSide lives go to the context's second stream, after everything
 // queued on the main stream so far. join_side makes the main stream wait for them (the end of
 // every call does it too, so no forked work outlives the call that forked it).
+// With ctx->no_fork set, an OnSide does nothing: its launches stay on the main stream.
 struct OnSide {
   explicit OnSide(Ctx* c);
   ~OnSide();
   Ctx* ctx;
   hipStream_t main;
+  bool active = false;
 };
 void join_side(Ctx* c);
 

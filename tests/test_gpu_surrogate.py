@@ -105,3 +105,37 @@ def test_patches_gather(capi_ctx):
         iz = (np.floor(cz).astype(int) - 7 + np.arange(16)) % 20
         want = sdf[np.ix_(iz, iy, ix)] / 4.0
         np.testing.assert_array_equal(got[b], want)
+
+
+def test_stream_switch_orders_pending_work():
+    """Forward on the context's own stream (asynchronous), then a switch to another stream and
+    the backward pass there: the switch must make the new stream wait for the queued forward, so
+    outputs and gradients equal the single-stream run bit for bit."""
+    from peclet.pnm import capi
+
+    rng = np.random.default_rng(21)
+    B = 64
+    p = capi.sur_init(5)
+    x = rng.normal(0, 1, size=(B, 16, 16, 16)).astype(np.float32)
+    s = rng.normal(0, 1, size=(B, 4)).astype(np.float32)
+    dy = rng.normal(0, 1, size=B).astype(np.float32)
+    pd, xd, sd, dyd = dev(p), dev(x), dev(s), dev(dy)
+    ctx = capi.Context(0)
+    try:
+        y1, g1 = torch.empty(B, device="cuda"), torch.zeros(p.size, device="cuda")
+        y2, g2 = torch.empty(B, device="cuda"), torch.zeros(p.size, device="cuda")
+        torch.cuda.synchronize()
+        ctx.sur_forward_d(B, xd, sd, pd, y1)
+        ctx.sur_backward_d(B, dyd, pd, g1)
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        for _ in range(3):
+            ctx.sur_forward_d(B, xd, sd, pd, y2)
+            ctx.set_stream(side.cuda_stream)
+            ctx.sur_backward_d(B, dyd, pd, g2)
+            ctx.set_stream(None)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(y2.cpu().numpy(), y1.cpu().numpy())
+            np.testing.assert_array_equal(g2.cpu().numpy(), g1.cpu().numpy())
+    finally:
+        ctx.close()

@@ -135,3 +135,26 @@ def test_walk_cap_tent():
         depth = abs(peak - start)
         want = start + min(depth, 512) * (1 if start < peak else -1)
         assert r[start] == want, start
+
+
+@pytest.mark.parametrize("kind", ["plate", "two_grains"])
+def test_flood_deep_plate_closed_form(kind):
+    """A thin plate fed by one grain floods one voxel per sweep (plate length + 1 sweeps, the
+    last one changing nothing); between two grains the equidistant middle voxel takes the
+    smaller label. Also equal to the BFS-layer flood."""
+    import deepflood as dfl
+
+    sdf = dfl.plate_sdf() if kind == "plate" else dfl.two_grain_plate_sdf()
+    lab0 = orc.markers_ccl(sdf, (1, 1, 1))
+    lab, sweeps = orc.flood(sdf, lab0)
+    np.testing.assert_array_equal(lab, bf.flood_layers(sdf, lab0))
+    lab = lab.reshape(sdf.shape)
+    plate = lab[3:5, 7:9, :]
+    if kind == "plate":
+        assert sweeps == (230 - 12) + 1
+        assert (plate[:, :, 12:230] == 0).all()  # grain A's label: its first voxel, index 0
+    else:
+        assert sweeps == 58 + 1
+        assert (plate[:, :, 12:70] == 0).all()      # x <= 69: grain A (tie at 69 -> smaller)
+        assert (plate[:, :, 70:127] == 127).all()   # grain B's first voxel is (0, 0, 127)
+    assert (lab[sdf >= 0] == -1).all()
