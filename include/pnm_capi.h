@@ -254,6 +254,23 @@ pnm_status pnm_sur_patches_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t di
 /* ---- stage-level device entry points (parity tests of single kernels; no reference
  * counterpart as separate calls: each is one step of segmentVolumeView) ---- */
 
+/* ---- segmentation without allocation (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same results as pnm_segment_volume_d, computed in
+ * a fixed scratch region the library holds per GPU (about 175 MiB, a static device array kept
+ * until the process ends) instead of the context's arena. The call allocates nothing: no
+ * hipMalloc/hipFree, no second stream, no timing events. Volumes are limited to
+ * PNM_NOALLOC_MAX_PADDED voxels after rounding Nx, Ny, Nz up to the 64 x 8 x 16 tile
+ * (PNM_NOALLOC_PADDED_VOXELS; 128^3 fits); larger ones fail with PNM_ECAPACITY before any work.
+ * Every call on a GPU shares that GPU's region: calls that overlap on one GPU (other threads,
+ * other contexts, other streams) race and must be serialised by the caller. */
+#define PNM_NOALLOC_MAX_PADDED (1 << 21)
+#define PNM_NOALLOC_PADDED_VOXELS(nx, ny, nz)                         \
+  ((((int64_t)(nx) + 63) / 64 * 64) * (((int64_t)(ny) + 7) / 8 * 8) * \
+   (((int64_t)(nz) + 15) / 16 * 16))
+pnm_status pnm_segment_volume_noalloc_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                                        const float spacing[3], int32_t* seg_d, int32_t* roots_d);
+
 /* Steps 1-3 (pore_extraction.hpp:173-269): marker labels after CCL and flood. */
 pnm_status pnm_stage_labels_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                               const float spacing[3], int32_t* labels_d, int32_t* ccl_only_d);

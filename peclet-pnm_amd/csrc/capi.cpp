@@ -4,6 +4,7 @@
 // exceptions into a pnm_status plus a thread-local message, and runs on the context's stream.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -822,3 +823,103 @@ void pnm_flow_result_free(pnm_flow_result_t* r) {
 
 }  // extern "C"
 
+// ------------------------- segmentation without allocation -------------------------
+//
+// pnm_segment_volume_noalloc_d runs the segment_volume pipeline on a fixed device region
+// instead of the context's arena, for callers that cannot call hipMalloc. The region is the
+// file-scope array `scratch` below: one copy per GPU, held until the process ends. Its device
+// address is taken once per device (hipGetSymbolAddress, with that device current) on the first
+// call there and kept in g_scratch_ptr; host code uses only that address, never the array's name
+// (which is a host-side stand-in). Layout of the region:
+//   [0, 256)                            look-back control words (ticket, done, epoch)
+//   [256, 256 + 8 * kNoAllocLbWords)    look-back status words (epoch-tagged, zeroed at set-up)
+//   [kNoAllocHeader, end)               bump arena for every other scratch array of the call
+// Every call on a device shares that device's region and leaves its contents behind; the
+// pipeline clears what it needs zeroed (counters, bitmaps) itself. Calls that overlap on one
+// device, from any thread or context, race on the region.
+namespace {
+
+constexpr int64_t kNoAllocLbWords = 16384;
+constexpr size_t kNoAllocHeader = (size_t)1 << 20;
+// Bound on the pipeline's scratch for a tile-padded volume of P voxels (n <= P): 56 B per voxel
+// of working arrays plus 25 B per padded voxel of tile staging, rounded up with headroom.
+constexpr size_t kNoAllocArena = (size_t)82 * PNM_NOALLOC_MAX_PADDED + ((size_t)2 << 20);
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocHeader + kNoAllocArena];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of this device's copy of `scratch`; the first call on a device takes it and
+// initialises the look-back words. The caller has made c->device current.
+char* noalloc_scratch(Ctx* c) {
+  need(c->device >= 0 && c->device < kNoAllocMaxDevices, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  void*& p = g_scratch_ptr[c->device];
+  if (!p) {
+    void* q = nullptr;
+    PNM_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    uint32_t* ctl = static_cast<uint32_t*>(q);
+    fill_u32(c, ctl, 0u, 64);     // ticket, done
+    fill_u32(c, ctl + 2, 1u, 1);  // epoch 1 (status words of epoch 0 read as "not ready")
+    fill_u64(c, reinterpret_cast<uint64_t*>(static_cast<char*>(q) + 256), 0u, kNoAllocLbWords);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    p = q;
+  }
+  return static_cast<char*>(p);
+}
+
+// For the life of one call: the arena carves from the region, the look-back scans use the
+// region's words, no second stream is forked and no timing events are taken.
+struct FixedScratch {
+  FixedScratch(Ctx* c, char* base)
+      : ctx(c), ctl(c->lb_ctl), state(c->lb_state), cap(c->lb_cap), prof(c->prof) {
+    c->lb_ctl = reinterpret_cast<uint32_t*>(base);
+    c->lb_state = reinterpret_cast<uint64_t*>(base + 256);
+    c->lb_cap = kNoAllocLbWords;
+    c->lb_fixed = true;
+    c->no_fork = true;
+    c->prof = false;
+    c->arena.fix(base + kNoAllocHeader, kNoAllocArena);
+  }
+  ~FixedScratch() {
+    ctx->arena.unfix();
+    ctx->lb_ctl = ctl;
+    ctx->lb_state = state;
+    ctx->lb_cap = cap;
+    ctx->lb_fixed = false;
+    ctx->no_fork = false;
+    ctx->prof = prof;
+  }
+  Ctx* ctx;
+  uint32_t* ctl;
+  uint64_t* state;
+  int64_t cap;
+  bool prof;
+};
+
+}  // namespace
+
+extern "C" pnm_status pnm_segment_volume_noalloc_d(pnm_ctx_t ctx, const float* sdf_d,
+                                                   const int32_t dims[3], const float spacing[3],
+                                                   int32_t* seg_d, int32_t* roots_d) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(spacing != nullptr, "spacing must not be NULL");
+    need(v.n == 0 || (sdf_d && seg_d), "sdf and seg must not be NULL");
+    Ctx* c = check_ctx(ctx);
+    if (v.n == 0) return;
+    const int64_t padded = PNM_NOALLOC_PADDED_VOXELS(dims[0], dims[1], dims[2]);
+    if (padded > PNM_NOALLOC_MAX_PADDED)
+      fail(PNM_ECAPACITY, "volume too large for the no-allocation entry point: " +
+                              std::to_string(padded) + " tile-padded voxels (Nx, Ny, Nz rounded " +
+                              "up to 64, 8, 16) exceed the maximum of " +
+                              std::to_string((int64_t)PNM_NOALLOC_MAX_PADDED));
+    CallScope cs(c);
+    char* base = noalloc_scratch(c);
+    FixedScratch fx(c, base);
+    segment_device(c, sdf_d, v, spacing, seg_d, roots_d);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+  });
+}

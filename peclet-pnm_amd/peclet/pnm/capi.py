@@ -30,7 +30,18 @@ EXPORTS = (
     "pnm_stage_labels_d", "pnm_stage_roots_d", "pnm_segment_batch_d", "pnm_batch_config",
     "pnm_stokes_solve_d", "pnm_sur_param_count", "pnm_sur_init", "pnm_sur_forward_d",
     "pnm_sur_backward_d", "pnm_sur_loss_d", "pnm_sur_adam_d", "pnm_sur_patches_d",
+    "pnm_segment_volume_noalloc_d",
 )
+
+# Largest tile-padded volume the no-allocation entry point accepts (PNM_NOALLOC_MAX_PADDED).
+PNM_NOALLOC_MAX_PADDED = 1 << 21
+
+
+def noalloc_padded_voxels(dims_xyz):
+    """Voxels after rounding X, Y, Z up to the 64 x 8 x 16 tile (PNM_NOALLOC_PADDED_VOXELS)."""
+    x, y, z = (int(a) for a in dims_xyz)
+    return (x + 63) // 64 * 64 * ((y + 7) // 8 * 8) * ((z + 15) // 16 * 16)
+
 
 PNM_BATCH_GRAPH = 1
 PNM_BATCH_AUTOTUNE = 2
@@ -98,6 +109,7 @@ def lib():
         L.pnm_free.restype = None
         L.pnm_extract_pores_d.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64, i64p, i64p]
         L.pnm_segment_volume_d.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.pnm_segment_volume_noalloc_d.argtypes = [vp, vp, vp, vp, vp, vp]
         L.pnm_extract_topology_d.argtypes = [vp, vp, vp, vp, ctypes.c_int64, i64p]
         L.pnm_extract_pore_network_d.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64, i64p,
                                                  i64p, vp, vp, ctypes.c_int64, i64p]
@@ -132,7 +144,8 @@ def lib():
         L.pnm_sur_patches_d.restype = st
         L.pnm_stage_labels_d.argtypes = [vp, vp, vp, vp, vp, vp]
         L.pnm_stage_roots_d.argtypes = [vp, vp, vp, ctypes.c_int, vp]
-        for name in ("pnm_extract_pores_d", "pnm_segment_volume_d", "pnm_extract_topology_d",
+        for name in ("pnm_extract_pores_d", "pnm_segment_volume_d",
+                     "pnm_segment_volume_noalloc_d", "pnm_extract_topology_d",
                      "pnm_extract_pore_network_d", "pnm_extract_network_flow_d",
                      "pnm_extract_network_flow", "pnm_stage_labels_d", "pnm_stage_roots_d"):
             getattr(L, name).restype = st
@@ -222,6 +235,12 @@ class Context:
     def segment_volume_d(self, sdf, dims_xyz, spacing_xyz, seg_out, roots_out=None):
         _check(lib().pnm_segment_volume_d(self.h, _ptr(sdf), _i3(dims_xyz), _f3(spacing_xyz),
                                           _ptr(seg_out), _ptr(roots_out)))
+
+    def segment_volume_noalloc_d(self, sdf, dims_xyz, spacing_xyz, seg_out, roots_out=None):
+        """segment_volume_d in the library's fixed per-GPU scratch (no device allocation)."""
+        _check(lib().pnm_segment_volume_noalloc_d(self.h, _ptr(sdf), _i3(dims_xyz),
+                                                  _f3(spacing_xyz), _ptr(seg_out),
+                                                  _ptr(roots_out)))
 
     def extract_topology_d(self, seg, dims_xyz, pairs_out):
         cap = pairs_out.numel() // 2
