@@ -53,3 +53,59 @@ def test_errors_and_short_data(tmp_path, capfd):
     arr, _, _ = pnm.SDFReader.read_vti(str(p))
     np.testing.assert_array_equal(arr, sdf)
     assert "does not match expected size" in capfd.readouterr().err
+
+
+def _parse_appended(path):
+    """Independent reader of a raw appended VTI: (attrs of ImageData, [(name, type, ncomp,
+    array bytes)]) following the offsets in the header."""
+    import re
+
+    raw = open(path, "rb").read()
+    us = raw.index(b'<AppendedData encoding="raw">')
+    us = raw.index(b"_", us) + 1
+    head = raw[:us].decode()
+    img = dict(re.findall(r'(\w+)="([^"]*)"', re.search(r"<ImageData[^>]*>", head).group(0)))
+    arrays = []
+    for m in re.finditer(r'<DataArray type="(\w+)" Name="([^"]*)" NumberOfComponents="(\d)" '
+                         r'format="appended" offset="(\d+)"/>', head):
+        off = us + int(m.group(4))
+        nbytes = int(np.frombuffer(raw[off:off + 8], "<u8")[0])
+        arrays.append((m.group(2), m.group(1), int(m.group(3)), raw[off + 8:off + 8 + nbytes]))
+    return img, arrays
+
+
+def test_save_vti_roundtrip_through_read_vti(tmp_path):
+    """The project's writer: SDFReader.read_vti returns the SDF, origin and spacing bit for bit
+    (point extents 0..N-1, which the reference's own script gets wrong for its reader)."""
+    import peclet.pnm as pnm
+    from peclet.pnm.vti import save_vti
+
+    rng = np.random.default_rng(1)
+    sdf = rng.standard_normal((4, 5, 6)).astype(np.float32)
+    p = tmp_path / "w.vti"
+    save_vti(p, {"sdf": sdf, "seg": np.arange(120, dtype=np.int64).reshape(4, 5, 6)},
+             spacing=(2.0, 1.0, 0.5), origin=(0.5, 2.0, -1.0))
+    arr, org, spc = pnm.SDFReader.read_vti(str(p))
+    np.testing.assert_array_equal(arr.view(np.uint32), sdf.view(np.uint32))
+    assert org == [0.5, 2.0, -1.0] and spc == [2.0, 1.0, 0.5]
+    img, arrays = _parse_appended(p)
+    assert img["WholeExtent"] == "0 5 0 4 0 3"
+    assert [a[:3] for a in arrays] == [("sdf", "Float32", 1), ("seg", "Int32", 1)]
+    np.testing.assert_array_equal(np.frombuffer(arrays[1][3], "<i4"), np.arange(120))
+
+
+def test_save_vti_vectors_cells_and_errors(tmp_path):
+    from peclet.pnm.vti import save_vti
+
+    u = np.random.default_rng(2).standard_normal((3, 4, 5, 3))
+    p = tmp_path / "v.vti"
+    save_vti(p, {"u": u}, data="cell")
+    img, arrays = _parse_appended(p)
+    assert img["WholeExtent"] == "0 5 0 4 0 3"  # cell data: N cells between N+1 points
+    name, typ, ncomp, data = arrays[0]
+    assert (name, typ, ncomp) == ("u", "Float64", 3)
+    np.testing.assert_array_equal(np.frombuffer(data, "<f8").reshape(u.shape), u)
+    with pytest.raises(ValueError, match="shape"):
+        save_vti(p, {"a": np.zeros((2, 2, 2)), "b": np.zeros((2, 2, 3))})
+    with pytest.raises(ValueError):
+        save_vti(p, {"a": np.zeros((2, 2, 2))}, data="faces")
