@@ -161,8 +161,9 @@ pnm_status pnm_extract_pores_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t 
                                int64_t* total);
 
 /* segmentVolumeView (pore_extraction.hpp:157-386) on a device SDF; seg_d: device int32[n].
- * roots_d (nullable): device int32[n] receiving the gradient-path roots of sdf>0 voxels
- * (rootsOut, :383-384; other voxels get their own index). */
+ * roots_d (nullable): device int32[n] receiving rootsOut (:383-384): the gradient-path root of
+ * every voxel, the ascent walker for sdf>0 voxels and the descent walker for the others
+ * (:273-311). */
 pnm_status pnm_segment_volume_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                                 const float spacing[3], int32_t* seg_d, int32_t* roots_d);
 
@@ -274,8 +275,9 @@ pnm_status pnm_segment_volume_noalloc_d(pnm_ctx_t ctx, const float* sdf_d, const
 /* Steps 1-3 (pore_extraction.hpp:173-269): marker labels after CCL and flood. */
 pnm_status pnm_stage_labels_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                               const float spacing[3], int32_t* labels_d, int32_t* ccl_only_d);
-/* Ascent roots: all_voxels = 0 -> step 4 (:271-311) for sdf>0 voxels (others: own index);
- * all_voxels = 1 -> nf::flow_basins walk (:621-652) from every voxel. */
+/* Gradient-walk roots: all_voxels = 0 -> step 4 (:271-311), ascent from sdf>0 voxels and
+ * descent from the others; all_voxels = 1 -> nf::flow_basins ascent (:621-652) from every
+ * voxel. */
 pnm_status pnm_stage_roots_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                              int all_voxels, int32_t* roots_d);
 

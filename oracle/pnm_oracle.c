@@ -189,7 +189,10 @@ int pnmo_flood(const float* sdf, const int32_t dims[3], int32_t* labels) {
 
 /* One ascent step: lexicographic maximum over (sdf, flat index) of the walker and its 26
  * neighbours (pore_extraction.hpp:280-304 ascent branch; :629-644 for the flow basins). */
-static inline int ascent_step(const float* sdf, int walker, int Nx, int Ny, int Nz) {
+/* One step of the gradient walk (pore_extraction.hpp:280-304): over the 26 periodic
+ * neighbours in dz -> dy -> dx order, ascent takes nv > bv, descent nv < bv, and either takes an
+ * equal value with a larger index than the current best (NaN never compares). */
+static inline int gradient_step(const float* sdf, int walker, int Nx, int Ny, int Nz, int up) {
   int best = walker;
   float bv = sdf[walker];
   const int wx = walker % Nx, wy = (walker / Nx) % Ny, wz = walker / (Nx * Ny);
@@ -199,7 +202,7 @@ static inline int ascent_step(const float* sdf, int walker, int Nx, int Ny, int 
         if (dx == 0 && dy == 0 && dz == 0) continue;
         const int ni = gidx1(wx + dx, wy + dy, wz + dz, Nx, Ny, Nz);
         const float nv = sdf[ni];
-        if (nv > bv) {
+        if (up ? (nv > bv) : (nv < bv)) {
           bv = nv;
           best = ni;
         } else if (nv == bv && ni > best)
@@ -209,21 +212,20 @@ static inline int ascent_step(const float* sdf, int walker, int Nx, int Ny, int 
 }
 
 /* ---- step 4, gradient_path: pore_extraction.hpp:271-311 (MAX_STEPS = 512, :278) ----
- * The root is the walker after min(depth, 512) steps. Only sdf > 0 roots are ever consumed by
- * the reference (:333-334, :342, :380, :530), so the descent walk is not restated: other voxels
- * get roots[i] = i. ascent_all = 1 restates nf::flow_basins (:621-652) instead. */
+ * The root is the walker after min(depth, 512) steps: ascent from sdf > 0 voxels, descent from
+ * the others (the direction is fixed by the starting voxel, :276). Only sdf > 0 roots feed the
+ * segmentation (:333-334, :342, :380, :530); the descent roots are the rest of rootsOut
+ * (:383-384). ascent_all = 1 restates nf::flow_basins (:621-652) instead: ascent from every
+ * voxel. */
 void pnmo_gradient_roots(const float* sdf, const int32_t dims[3], int ascent_all,
                          int32_t* roots) {
   const int Nx = dims[0], Ny = dims[1], Nz = dims[2];
   const int64_t n = (int64_t)Nx * Ny * Nz;
   for (int64_t i = 0; i < n; ++i) {
-    if (!ascent_all && !(sdf[i] > 0.0f)) {
-      roots[i] = (int32_t)i;
-      continue;
-    }
+    const int up = ascent_all || sdf[i] > 0.0f;
     int walker = (int)i;
     for (int s = 0; s < 512; ++s) {
-      const int best = ascent_step(sdf, walker, Nx, Ny, Nz);
+      const int best = gradient_step(sdf, walker, Nx, Ny, Nz, up);
       if (best == walker) break;
       walker = best;
     }

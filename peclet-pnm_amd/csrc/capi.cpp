@@ -548,8 +548,9 @@ pnm_status pnm_stage_roots_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t di
     if (v.n == 0) return;
     CallScope cs(c);
     int32_t* next = c->arena.get_n<int32_t>((size_t)v.n);
-    launch_next(c, sdf_d, v, next, all_voxels != 0);
-    launch_chase(c, sdf_d, v, next, roots_d, all_voxels != 0);
+    const int mode = all_voxels ? kWalkAll : kWalkPores;
+    launch_next(c, sdf_d, v, next, mode);
+    launch_chase(c, sdf_d, v, next, roots_d, mode);
     PNM_HIP(hipStreamSynchronize(c->stream));
   });
 }

@@ -72,10 +72,12 @@ __device__ __forceinline__ int64_t ld_agent64(const int64_t* p) {
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
-// Lexicographic (sdf, flat index) maximum over the voxel and its 26 periodic neighbours — the
-// ascent step of pore_extraction.hpp:280-304 (ties to the larger index; NaN never wins).
-__device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x, int y, int z,
-                                           int nx, int ny, int nz, int ci, float cv) {
+// One step of the gradient walk (pore_extraction.hpp:280-304) over the voxel and its 26 periodic
+// neighbours in dz -> dy -> dx order: ascent (kUp) takes a larger value, descent a smaller one,
+// either takes an equal value with a larger index than the current best; NaN never wins.
+template <bool kUp>
+__device__ __forceinline__ int gradient_best(const float* __restrict__ sdf, int x, int y, int z,
+                                             int nx, int ny, int nz, int ci, float cv) {
   const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
   const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
   const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
@@ -91,7 +93,7 @@ __device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x,
         if (a == 1 && b == 1 && c == 1) continue;
         const int ni = row + xs[c];
         const float nv = sdf[ni];
-        if (nv > bv) {
+        if (kUp ? (nv > bv) : (nv < bv)) {
           bv = nv;
           best = ni;
         } else if (nv == bv && ni > best) {
@@ -100,6 +102,12 @@ __device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x,
       }
     }
   return best;
+}
+
+// The ascent step (lexicographic (sdf, flat index) maximum).
+__device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x, int y, int z,
+                                           int nx, int ny, int nz, int ci, float cv) {
+  return gradient_best<true>(sdf, x, y, z, nx, ny, nz, ci, cv);
 }
 
 // (value, flat index) packed into one ordered 64-bit key: the value's bits mapped to an unsigned

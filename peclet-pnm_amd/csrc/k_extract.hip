@@ -187,47 +187,58 @@ void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const
                      out);
 }
 
-// Ascent successor of every voxel (all_voxels) or of sdf > 0 voxels (others: themselves).
+// Successor of the gradient walk. Mode kWalkPores: ascent for sdf > 0 voxels, descent for the
+// others (gradient_path, pore_extraction.hpp:273-304: the direction is fixed by the starting
+// voxel); kWalkAll: ascent for every voxel (nf::flow_basins, :621-652); kWalkSolids: descent
+// for !(sdf > 0) voxels, the others keep themselves.
 __global__ __launch_bounds__(256) void k_next(const float* __restrict__ sdf, int nx, int ny,
-                                              int nz, int32_t* __restrict__ next,
-                                              bool all_voxels) {
+                                              int nz, int32_t* __restrict__ next, int mode) {
   const int x = blockIdx.x * 64 + threadIdx.x;
   const int y = blockIdx.y * 4 + threadIdx.y;
   const int z = blockIdx.z;
   if (x >= nx || y >= ny) return;
   const int i = (z * ny + y) * nx + x;
   const float cv = sdf[i];
+  const bool pore = cv > 0.0f;
   int nxt = i;
-  if (all_voxels || cv > 0.0f) nxt = ascent_best(sdf, x, y, z, nx, ny, nz, i, cv);
+  if (mode == kWalkAll || (mode == kWalkPores && pore))
+    nxt = gradient_best<true>(sdf, x, y, z, nx, ny, nz, i, cv);
+  else if (!pore)
+    nxt = gradient_best<false>(sdf, x, y, z, nx, ny, nz, i, cv);
   next[i] = nxt;
 }
 
-void launch_next(Ctx* c, const float* sdf, Vol v, int32_t* next, bool all_voxels) {
-  PNM_LAUNCH(c, "k_next", k_next, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz, next,
-                     all_voxels);
+void launch_next(Ctx* c, const float* sdf, Vol v, int32_t* next, int mode) {
+  PNM_LAUNCH(c, "k_next", k_next, vox_grid(v), kVoxBlock, 0, sdf, v.nx, v.ny, v.nz, next, mode);
 }
 
-// Root = the walker after min(depth, 512) successor steps (pore_extraction.hpp:279-309).
+// Root = the walker after min(depth, 512) successor steps (pore_extraction.hpp:279-309). In
+// mode kWalkSolids only !(sdf > 0) voxels are written (the caller holds the pore roots).
 __global__ __launch_bounds__(256) void k_chase(const float* __restrict__ sdf, int64_t n,
                                                const int32_t* __restrict__ next,
-                                               int32_t* __restrict__ roots, bool all_voxels) {
+                                               int32_t* __restrict__ roots, int mode) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (mode == kWalkSolids && sdf[i] > 0.0f) return;
   int w = (int)i;
-  if (all_voxels || sdf[i] > 0.0f) {
-    for (int s = 0; s < kMaxSteps; ++s) {
-      const int nx = next[w];
-      if (nx == w) break;
-      w = nx;
-    }
+  for (int s = 0; s < kMaxSteps; ++s) {
+    const int nx = next[w];
+    if (nx == w) break;
+    w = nx;
   }
   roots[i] = w;
 }
 
 void launch_chase(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
-                  bool all_voxels) {
-  PNM_LAUNCH(c, "k_chase", k_chase, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, next,
-                     roots, all_voxels);
+                  int mode) {
+  PNM_LAUNCH(c, "k_chase", k_chase, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, next, roots,
+             mode);
+}
+
+void launch_descent_roots(Ctx* c, const float* sdf, Vol v, int32_t* roots) {
+  int32_t* next = c->arena.get_n<int32_t>((size_t)v.n);
+  launch_next(c, sdf, v, next, kWalkSolids);
+  launch_chase(c, sdf, v, next, roots, kWalkSolids);
 }
 
 }  // namespace pnm

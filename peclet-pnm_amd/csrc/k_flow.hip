@@ -524,7 +524,7 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   // flow basins
   int32_t* next_all = c->arena.get_n<int32_t>((size_t)n);
   int32_t* lab = c->arena.get_n<int32_t>((size_t)n);
-  launch_next(c, sdf, v, next_all, true);
+  launch_next(c, sdf, v, next_all, kWalkAll);
   PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256)), dim3(256), 0, n, next_all, seg,
              lab);
 

@@ -1016,17 +1016,6 @@ __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__
   if (flags) atomicOr(cap_flag, flags);
 }
 
-// rootsOut convention for voxels that are not sdf>0: their own index (:383-384).
-__global__ void k_roots_self(const float* __restrict__ sdf, int64_t n, int32_t* roots) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && !(sdf[i] > 0.0f)) roots[i] = (int32_t)i;
-}
-
-void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots) {
-  PNM_LAUNCH(c, "k_roots_self", k_roots_self, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
-             roots);
-}
-
 static dim3 tile_grid(Vol v) {
   return dim3(cdiv(v.nx, 64), cdiv(v.ny, TILE_Y), cdiv(v.nz, TILE_Z));
 }

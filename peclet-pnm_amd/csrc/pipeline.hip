@@ -102,7 +102,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   uint32_t* skeys = c->arena.get_n<uint32_t>((size_t)n);
   uint64_t* ctr = c->arena.get_n<uint64_t>(64);
   // ctr: [0] peaks [1] (unused) [2] solid keys [3] cap flag [4] pores [5] solids [6] full keys
-  //      [8..] band counts
+  //      [8..11] ring of flood band lengths
   fill_u64(c, ctr, 0, 64);
 
   int32_t* lab_ccl = c->arena.get_n<int32_t>((size_t)n);  // CCL labels (tile-local roots)
@@ -199,7 +199,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   }
   assign_ids_bitmap(c, sdf, v, pkeys, npk, min_pore, skeys, nskeys, min_solid, ctr + 4);
   if (seg) launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
-  if (roots_out) launch_roots_self(c, sdf, v, roots_out);
+  if (roots_out) launch_descent_roots(c, sdf, v, roots_out);  // rootsOut of sdf <= 0 voxels
   if (read_counts) {
     readback(c, ctr + 4, 2, h);
     out.r.n_pores = (int32_t)h[0];

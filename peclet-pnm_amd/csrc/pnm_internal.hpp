@@ -251,7 +251,6 @@ void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint64_t* stage,
                           const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
                           int32_t* roots, int32_t* min_pore, uint64_t* cap_flag);
-void launch_roots_self(Ctx* c, const float* sdf, Vol v, int32_t* roots);
 void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                         uint32_t* key_cnt);
 // Flatten + the first flood sweep(s) per tile: reads the CCL labels (tile-local roots whose
@@ -287,10 +286,15 @@ void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, i
                 bool solid, uint64_t* count_dev);
 void launch_seg(Ctx* c, const float* sdf, Vol v, const int32_t* roots, const int32_t* labels,
                 const int32_t* pore_id, const int32_t* solid_id, int32_t* seg);
-// Roots for every voxel (flow basins) or sdf>0 only, computed stand-alone.
-void launch_next(Ctx* c, const float* sdf, Vol v, int32_t* next, bool all_voxels);
+// Gradient-walk successors and capped chases, computed stand-alone. Modes: kWalkPores = the
+// reference's step 4 (ascent from sdf>0, descent from the rest), kWalkAll = ascent from every
+// voxel (flow basins), kWalkSolids = descent from !(sdf>0) voxels only (chase writes only those).
+constexpr int kWalkPores = 0, kWalkAll = 1, kWalkSolids = 2;
+void launch_next(Ctx* c, const float* sdf, Vol v, int32_t* next, int mode);
 void launch_chase(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
-                  bool all_voxels);
+                  int mode);
+// Descent roots of the !(sdf>0) voxels into roots (the rest of rootsOut, :383-384).
+void launch_descent_roots(Ctx* c, const float* sdf, Vol v, int32_t* roots);
 
 // Topology: block-deduplicated candidate keys, then sort + unique. key = (l1 - lo) << bits |
 // (l2 - lo). Returns count; pairs_d receives (l1,l2) int32 pairs.

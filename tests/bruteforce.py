@@ -115,22 +115,31 @@ def flood_layers(sdf, labels):
     return lab.ravel().astype(np.int32)
 
 
-def ascent_next(sdf, all_voxels):
-    """Successor: lexicographic (value, index) argmax over the closed 27-neighbourhood,
-    ignoring NaN neighbours; voxels not walked keep themselves."""
+def _walk_next(sdf, sign):
+    """Lexicographic (sign * value, index) argmax over the closed 27-neighbourhood, ignoring NaN
+    neighbours (a NaN centre keeps itself)."""
     idx = flat_index(sdf.shape)
-    bv = sdf.astype(np.float64).copy()
+    bv = sign * sdf.astype(np.float64)
     bi = idx.copy()
     for dz, dy, dx in OFFSETS26:
-        nv = _nb(sdf, dz, dy, dx).astype(np.float64)
+        nv = sign * _nb(sdf, dz, dy, dx).astype(np.float64)
         ni = _nb(idx, dz, dy, dx)
         better = (nv > bv) | ((nv == bv) & (ni > bi))
         better &= ~np.isnan(nv)
         better &= ~np.isnan(bv)
         bv = np.where(better, nv, bv)
         bi = np.where(better, ni, bi)
-    walk = np.ones(sdf.shape, bool) if all_voxels else (sdf > 0)
-    return np.where(walk, bi, idx).ravel()
+    return bi
+
+
+def ascent_next(sdf, all_voxels):
+    """Successor of the gradient walk: ascent (largest value, ties to the larger index) from
+    sdf > 0 voxels, or from every voxel when all_voxels; descent (smallest value, ties to the
+    larger index) from the others."""
+    up = _walk_next(sdf, 1.0)
+    if all_voxels:
+        return up.ravel()
+    return np.where(sdf > 0, up, _walk_next(sdf, -1.0)).ravel()
 
 
 def roots(sdf, all_voxels=False, cap=512):

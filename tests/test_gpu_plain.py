@@ -347,3 +347,20 @@ def test_512_properties(capi_ctx, kind):
     pairs2 = torch.empty_like(pairs)
     k = capi_ctx.extract_topology_d(seg2, xyz(shape), pairs2)
     assert k == npairs and torch.equal(pairs2[: 2 * k], pairs[: 2 * npairs])
+
+
+@pytest.mark.parametrize("case", ["sphere_lattice", "trig", "tent1100", "quantized", "random"])
+def test_roots_out_is_reference_roots(capi_ctx, case):
+    """segment_volume_d's roots_d is rootsOut (pore_extraction.hpp:383-384): ascent walkers of
+    sdf>0 voxels and descent walkers of the others, each capped at 512 steps."""
+    if case == "random":
+        sdf = quantized((20, 17, 65), 77, nan_frac=0.02)
+    else:
+        sdf = load(os.path.join(os.path.dirname(__file__), "golden", case + ".npz"))["sdf"]
+    n = sdf.size
+    seg = torch.empty(n, dtype=torch.int32, device="cuda")
+    rts = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    capi_ctx.segment_volume_d(dev(sdf), xyz(sdf.shape), (1.0, 1.0, 1.0), seg, rts)
+    o_seg, o_rts, _ = orc.segment_volume(sdf, (1.0, 1.0, 1.0), return_stages=True)
+    np.testing.assert_array_equal(rts.cpu().numpy(), o_rts)
+    np.testing.assert_array_equal(seg.cpu().numpy(), o_seg)

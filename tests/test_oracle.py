@@ -69,8 +69,7 @@ def test_oracle_vs_bruteforce_random(shape, seed):
     seg, rts, lab = orc.segment_volume(sdf, spc, return_stages=True)
     ref_lab = bf.flood_layers(sdf, bf.ccl(bf.markers(sdf, spc)))
     np.testing.assert_array_equal(lab, ref_lab)
-    pos = sdf.ravel() > 0
-    np.testing.assert_array_equal(rts[pos], bf.roots(sdf)[pos])
+    np.testing.assert_array_equal(rts, bf.roots(sdf))  # ascent and descent walkers
     np.testing.assert_array_equal(seg, bf.segment(sdf, spc))
     np.testing.assert_array_equal(orc.extract_topology(seg, shape), bf.topology(seg, shape))
     pores, tot = orc.extract_pores(sdf, (0, 0, 0), spc)
@@ -158,3 +157,18 @@ def test_flood_deep_plate_closed_form(kind):
         assert (plate[:, :, 12:70] == 0).all()      # x <= 69: grain A (tie at 69 -> smaller)
         assert (plate[:, :, 70:127] == 127).all()   # grain B's first voxel is (0, 0, 127)
     assert (lab[sdf >= 0] == -1).all()
+
+
+def test_descent_walk_cap_valley():
+    """rootsOut of sdf <= 0 voxels is the descent walker (pore_extraction.hpp:283-301): on a 1-D
+    valley it runs to the lowest voxel (ties to the larger index) or stops at the 512-step cap."""
+    nx = 1100
+    x = np.arange(nx)
+    sdf = (-1.0 - np.minimum(x, nx - 1 - x)).astype(np.float32).reshape(1, 1, nx)
+    r = orc.gradient_roots(sdf)
+    low = 550  # values tie at 549/550; the larger index wins
+    for start in (0, 10, 37, 38, 100, 600, 1099):
+        depth = abs(low - start)
+        want = start + min(depth, 512) * (1 if start < low else -1)
+        assert r[start] == want, start
+    np.testing.assert_array_equal(r, bf.roots(sdf))
