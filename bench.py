@@ -11,8 +11,10 @@ written to device buffers. Prints ONE JSON line.
 
 `value` is voxels per second of the whole step (Gvox/s). `roofline` is the dominant kernel's
 algorithmic bytes per launch over its average duration, timed live with HIP events on the
-launch stream. `cpu_baseline` times the serial C oracle (1 core) on a bounded z-slab of the same
-volume. Two more objects on the same line:
+launch stream. `cpu_baseline` times the OpenMP restatement of the reference's
+extract_pore_network_k (oracle/pnm_ref_omp.cpp, OMP_NUM_THREADS threads) on a bounded z-slab of
+the same volume (--cpu-full: the whole volume), median of 5 after a warm-up, with the serial C
+oracle (1 core) beside it. Two more objects on the same line:
   batched: 4096 cubic crops, edges U[48,160] at seeded offsets of the same 512^3 volume (periodic
            wrap), segmented in one persistent launch replayed from a hipGraph with the autotuned
            shape, against per-volume segment_volume calls on a sample of the same crops;
@@ -74,19 +76,58 @@ def make_sdf(workload, size, seed):
     return sdf, desc
 
 
-def cpu_baseline(sdf, slab):
-    """Serial C oracle (1 core) on the first `slab` z-slices, as their own periodic volume."""
+def _host_cpu():
+    """CPU model and the logical CPUs this process may use (the OpenMP threads are set apart)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(sdf, slab, serial_slab, reps=5, full=False):
+    """The OpenMP restatement of the reference's extract_pore_network_k (oracle/pnm_ref_omp.cpp,
+    threads from OMP_NUM_THREADS) on the first `slab` z-slices of the same SDF as their own
+    periodic volume (the whole volume with full=True): one warm-up run, then the median of `reps`.
+    The serial C oracle (1 core) on `serial_slab` slices is reported beside it."""
     import numpy as np
     import pnm_oracle as orc
 
-    crop = np.ascontiguousarray(sdf[:slab])
+    crop = np.ascontiguousarray(sdf if full else sdf[:slab])
+    orc.ref_omp_extract_pore_network(crop, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))  # warm-up
+    times, stages = [], []
+    for _ in range(reps):
+        t = time.time()
+        _, _, _, ms = orc.ref_omp_extract_pore_network(crop, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+        times.append(time.time() - t)
+        stages.append(ms)
+    k = int(np.argsort(times)[len(times) // 2])
+    dt = times[k]
+    threads = orc.ref_omp().pnmr_threads()
+    model, cpus = _host_cpu()
+    small = np.ascontiguousarray(sdf[:serial_slab])
     t = time.time()
-    orc.extract_pore_network(crop, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
-    dt = time.time() - t
-    return {"value": round(crop.size / dt / 1e9, 6), "unit": "Gvox/s", "cores": 1,
+    orc.extract_pore_network(small, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    ts = time.time() - t
+    what = "whole volume" if full else f"z-slab {crop.shape} of the same SDF as its own periodic volume"
+    return {"value": round(crop.size / dt / 1e9, 6), "unit": "Gvox/s", "cores": threads,
             "kind": "port",
-            "sample": f"serial C oracle, extract_pore_network on z-slab {crop.shape} "
-                      f"({crop.size} voxels) of the same SDF, {dt:.1f} s"}
+            "sample": (f"OpenMP restatement of the reference's extract_pore_network_k "
+                       f"(oracle/pnm_ref_omp.cpp), {what} ({crop.size} voxels), {threads} "
+                       f"threads, median of {reps} runs after one warm-up: {dt:.2f} s"),
+            "stage_ms": {name: round(float(v), 1) for name, v in
+                         zip(("upload", "pores", "segment", "topology", "download"), stages[k])},
+            "runs_s": [round(x, 3) for x in times],
+            "host_cpu": model, "host_logical_cpus": cpus,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "serial_oracle": {"value": round(small.size / ts / 1e9, 6), "unit": "Gvox/s",
+                              "cores": 1,
+                              "sample": f"serial C oracle on z-slab {small.shape}, {ts:.2f} s"}}
 
 
 def bench_batched(ctx, src, reps, seed=4001, count=4096, lo=48, hi=160, sample=256):
@@ -410,7 +451,13 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--workload", default="polydisperse", choices=["polydisperse", "boolean"])
     ap.add_argument("--seed", type=int, default=2001)
-    ap.add_argument("--cpu-slab", type=int, default=96)
+    ap.add_argument("--cpu-slab", type=int, default=64,
+                    help="z-slices of the SDF timed by the OpenMP CPU baseline")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the CPU baseline on the whole volume instead of a slab")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--serial-slab", type=int, default=32,
+                    help="z-slices timed by the serial oracle reported beside the baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
@@ -539,7 +586,9 @@ def main():
         "kernels_source": f"{nbk} untimed steps with every launch bracketed by HIP events",
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sdf, min(args.cpu_slab, shape[0]))
+        out["cpu_baseline"] = cpu_baseline(sdf, min(args.cpu_slab, shape[0]),
+                                           min(args.serial_slab, shape[0]),
+                                           reps=args.cpu_reps, full=args.cpu_full)
     else:
         out["cpu_baseline"] = None
     if not args.no_batched:

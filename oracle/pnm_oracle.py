@@ -218,3 +218,47 @@ def extract_network_flow(sdf, origin_zyx, spacing_zyx, u, v, w, p, ox=None, oy=N
         out["cz"] = arr(r.cz, nt, np.float64)
     lib().pnmo_flow_free(ctypes.byref(r))
     return out
+
+
+# ---- the OpenMP baseline (oracle/pnm_ref_omp.cpp): the reference's extract_pore_network_k
+# restated for CPU threads, timed by bench.py's cpu_baseline and checked by tests/ ----
+REF_OMP_PATH = os.path.join(HERE, "libpnm_ref_omp.so")
+_ref = None
+
+
+def ref_omp():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_OMP_PATH):
+            build()
+        L = ctypes.CDLL(REF_OMP_PATH)
+        vp = ctypes.c_void_p
+        L.pnmr_threads.restype = ctypes.c_int
+        L.pnmr_extract_pore_network.restype = ctypes.c_int64
+        L.pnmr_extract_pore_network.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, vp,
+                                                ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
+                                                ctypes.POINTER(ctypes.c_int64), vp]
+        L.pnmr_free.argtypes = [vp]
+        _ref = L
+    return _ref
+
+
+def ref_omp_extract_pore_network(sdf, origin_zyx, spacing_zyx):
+    """(pores in atomic-slot order, seg, connections, stage_ms [upload, pores, segment,
+    topology, download]) from the OpenMP baseline; threads from OMP_NUM_THREADS."""
+    s = _sdf(sdf)
+    d, o, sp = _dims(s.shape), _xyz_f32(origin_zyx), _xyz_f32(spacing_zyx)
+    cap = 1000000
+    pores = np.zeros(4 * cap, dtype=np.float32)
+    seg = np.zeros(s.size, dtype=np.int32)
+    ptr = ctypes.POINTER(ctypes.c_int32)()
+    npairs = ctypes.c_int64()
+    ms = np.zeros(5, dtype=np.float64)
+    npores = ref_omp().pnmr_extract_pore_network(_p(s), _p(d), _p(o), _p(sp), _p(pores), cap,
+                                                 _p(seg), ctypes.byref(ptr),
+                                                 ctypes.byref(npairs), _p(ms))
+    conns = (np.ctypeslib.as_array(ptr, shape=(2 * npairs.value,)).reshape(-1, 2).copy()
+             if npairs.value else np.zeros((0, 2), dtype=np.int32))
+    ref_omp().pnmr_free(ptr)
+    p = pores[: 4 * min(npores, cap)].copy().view(PORE_DTYPE)
+    return p, seg, conns, ms

@@ -172,3 +172,18 @@ def test_descent_walk_cap_valley():
         want = start + min(depth, 512) * (1 if start < low else -1)
         assert r[start] == want, start
     np.testing.assert_array_equal(r, bf.roots(sdf))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_openmp_baseline_equals_oracle(path):
+    """The OpenMP restatement timed as the CPU baseline computes what the serial oracle does:
+    seg and connections bit for bit, pores as a set (atomic-slot order, as the reference)."""
+    g = load(path)
+    sdf, org, spc = g["sdf"], g["origin_zyx"], g["spacing_zyx"]
+    p, seg, conns, _ = orc.ref_omp_extract_pore_network(sdf, org, spc)
+    o_pores, o_seg, o_conns = orc.extract_pore_network(sdf, org, spc)
+    np.testing.assert_array_equal(seg, o_seg)
+    np.testing.assert_array_equal(conns, o_conns)
+    key = lambda a: np.sort(np.ascontiguousarray(a).view(np.uint32).reshape(-1, 4).view(
+        "u4,u4,u4,u4").ravel())
+    np.testing.assert_array_equal(key(p), key(o_pores))
