@@ -86,3 +86,92 @@ def mac_divergence(f, spacing_zyx):
     u, v, w = f["u"], f["v"], f["w"]
     return ((np.roll(u, -1, 2) - u) * (sy * sz) + (np.roll(v, -1, 1) - v) * (sx * sz)
             + (np.roll(w, -1, 0) - w) * (sx * sy))
+
+
+def two_chambers_side_by_side_tubes(nx=48, ny=32, nz=24, sep=5.0, r_tube=2.5):
+    """Two chambers on a periodic x-line joined by two parallel x-tubes side by side in y: the
+    interface between the two basins crosses both tubes (two core patches of one pore pair) and
+    the solid wall between them (film faces touching both patches)."""
+    shape = (nz, ny, nx)
+    ch = [(nx * 0.25, ny / 2, nz / 2, 8.0), (nx * 0.75, ny / 2, nz / 2, 8.0)]
+    tubes = [(0, (nx * 0.25, ny / 2 + s * sep, nz / 2), r_tube, 0.3, nx / 2) for s in (-1, 1)]
+    return chambers_and_tubes(shape, ch, tubes)
+
+
+def face_tiers(sdf, flow_lab, opn=None):
+    """Brute-force patch label of every face f = 3c + d (the +d face of cell c), -1 for faces
+    that are not interface faces, following the tier rules of pore_extraction.hpp:655-818 but
+    computed as graph components instead of union-find and label propagation:
+      core faces: the minimum face id of their component of core faces;
+      film faces: per component of film faces, the minimum core-patch label over the core faces
+      adjacent to any face of that component, or (no core face adjacent) the component's
+      minimum face id.
+    Two faces are adjacent when they have the same pore pair, their cells are 26-neighbours
+    (periodic) and |2 dc + e_d2 - e_d|^2 <= 8. opn: optional (ox, oy, oz) arrays (Nz, Ny, Nx)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    shape = sdf.shape
+    n = sdf.size
+    lab = flow_lab.reshape(shape)
+    nb_axis = {0: 2, 1: 1, 2: 0}  # face direction d (x, y, z) -> numpy axis
+    itf = np.zeros((n, 3), bool)
+    core = np.zeros((n, 3), bool)
+    key = np.full((n, 3), -1, np.int64)
+    for d in range(3):
+        nbl = np.roll(lab, -1, axis=nb_axis[d])
+        nbs = np.roll(sdf, -1, axis=nb_axis[d])
+        o = np.ones(shape) if opn is None else np.roll(opn[d], -1, axis=nb_axis[d])
+        it = (lab != nbl) & (lab > 0) & (nbl > 0) & (o > 0)
+        itf[:, d] = it.ravel()
+        core[:, d] = (it & (sdf > 0) & (nbs > 0)).ravel()
+        lo, hi = np.minimum(lab, nbl).astype(np.int64), np.maximum(lab, nbl).astype(np.int64)
+        key[:, d] = np.where(it, (lo << 32) | hi, -1).ravel()
+    cell = np.arange(n).reshape(shape)
+    rows, cols = [], []
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                c2 = np.roll(cell, (-dz, -dy, -dx), axis=(0, 1, 2)).ravel()
+                for d in range(3):
+                    for d2 in range(3):
+                        D = (2 * dx + (d2 == 0) - (d == 0), 2 * dy + (d2 == 1) - (d == 1),
+                             2 * dz + (d2 == 2) - (d == 2))
+                        if D[0] ** 2 + D[1] ** 2 + D[2] ** 2 > 8:
+                            continue
+                        ok = itf[:, d] & itf[c2, d2] & (key[:, d] == key[c2, d2])
+                        f1 = 3 * np.flatnonzero(ok) + d
+                        f2 = 3 * c2[ok] + d2
+                        keep = f1 != f2
+                        rows.append(f1[keep])
+                        cols.append(f2[keep])
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    nf = 3 * n
+    itf_f, core_f = itf.ravel(), core.ravel()
+    film_f = itf_f & ~core_f
+    out = np.full(nf, -1, np.int64)
+
+    def comps(mask):
+        sel = mask[rows] & mask[cols]
+        g = coo_matrix((np.ones(sel.sum()), (rows[sel], cols[sel])), shape=(nf, nf))
+        _, cc = connected_components(g, directed=False)
+        ids = np.flatnonzero(mask)
+        root = np.full(cc.max() + 1, np.iinfo(np.int64).max, np.int64)
+        np.minimum.at(root, cc[ids], ids)
+        return cc, root
+
+    cc_core, root_core = comps(core_f)
+    out[core_f] = root_core[cc_core[core_f]]
+    cc_film, root_film = comps(film_f)
+    big = np.iinfo(np.int64).max
+    best = np.full(root_film.size, big, np.int64)
+    edge = film_f[rows] & core_f[cols]  # film face next to a core face
+    np.minimum.at(best, cc_film[rows[edge]], out[cols[edge]])
+    fl = np.flatnonzero(film_f)
+    att = best[cc_film[fl]]
+    out[fl] = np.where(att < big, att, root_film[cc_film[fl]])
+    # how many distinct core patches each film component touches (for tests that need > 1)
+    pairs = np.unique(np.stack([cc_film[rows[edge]], out[cols[edge]]], axis=1), axis=0)
+    touch = np.bincount(pairs[:, 0], minlength=root_film.size) if pairs.size else \
+        np.zeros(root_film.size, int)
+    return out, int(touch.max(initial=0))

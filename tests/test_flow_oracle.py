@@ -113,3 +113,54 @@ def test_empty_when_no_pores():
     z = np.zeros(sdf.shape)
     r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), z, z, z, z)
     assert len(r["pores"]) == 0 and len(r["throats"]) == 0
+
+
+@pytest.mark.parametrize("case", ["side_by_side", "pack2", "pack3"])
+def test_face_tiers_against_graph_components(case):
+    """Every face's patch label (core CCL, film attachment to the minimum reachable core patch,
+    leftover films) equals an independent graph-component computation, on geometries where a
+    film component touches two or three core patches of its pore pair, so that a film taking
+    any patch other than the minimum one fails."""
+    import bruteforce as bf
+    from peclet.pnm import synth
+
+    if case == "side_by_side":
+        sdf = fc.two_chambers_side_by_side_tubes()
+    else:
+        sdf, _ = synth.packing_sdf((32, 32, 32), "uniform", 3.0, 5.0, 0.55, seed=int(case[-1]))
+    f = fc.uniform_x(sdf.shape, 1.0)
+    r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"], f["p"],
+                                 debug=True)
+    flow_lab = bf.segment(sdf, (1, 1, 1))[bf.roots(sdf, all_voxels=True)]
+    np.testing.assert_array_equal(r["flow_lab"], flow_lab)
+    want, touch = fc.face_tiers(sdf, flow_lab)
+    assert touch >= 2
+    np.testing.assert_array_equal(r["fpar"], want)
+
+
+@pytest.mark.parametrize("alpha", [0.5, 0.25, 0.3])
+def test_fractional_openness_scales_flux_and_area(alpha):
+    """Openness alpha in (0, 1) on every face: the same interface patches, throat flow and area
+    alpha times their fully open values (bit for bit for powers of two), residual still zero
+    under uniform flow; openness 0 on x faces removes exactly the x-faces from the interfaces."""
+    sdf = fc.chain_x(3, 24, 7.0, 3.0)
+    U = 2.5
+    f = fc.uniform_x(sdf.shape, U)
+    one = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"], f["p"])
+    o = np.full(sdf.shape, alpha)
+    r = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"], f["p"],
+                                 ox=o, oy=o, oz=o)
+    np.testing.assert_array_equal(r["throats"], one["throats"])
+    if alpha in (0.5, 0.25):
+        np.testing.assert_array_equal(r["throat_flow"], alpha * one["throat_flow"])
+        np.testing.assert_array_equal(r["throat_area"], alpha * one["throat_area"])
+    else:
+        np.testing.assert_allclose(r["throat_flow"], alpha * one["throat_flow"], rtol=1e-14)
+        np.testing.assert_allclose(r["throat_area"], alpha * one["throat_area"], rtol=1e-14)
+    assert np.all(r["pore_residual"] == 0.0)
+    # closing every x face: no x-face interface is left, so the x-flux through throats is zero
+    z = np.zeros(sdf.shape)
+    r0 = orc.extract_network_flow(sdf, (0, 0, 0), (1, 1, 1), f["u"], f["v"], f["w"], f["p"],
+                                  ox=z, oy=o, oz=o)
+    assert np.all(r0["throat_flow"] == 0.0)
+    assert np.all(r0["pore_residual"] == 0.0)

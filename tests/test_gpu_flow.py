@@ -138,3 +138,24 @@ def test_flow_256_chamber_lattice_properties(capi_ctx):
     assert len(a["pores"]) >= 125
     scale = float(np.abs(f["u"]).max())
     assert np.abs(a["pore_residual"]).max() < 1e-9 * scale
+
+
+@pytest.mark.parametrize("case", ["side_by_side", "pack2", "pack3"])
+@pytest.mark.parametrize("alpha", [1.0, 0.3])
+def test_films_bridging_core_patches(capi_ctx, case, alpha):
+    """Geometries where one film component touches two or three core patches of its pore pair
+    (the film must attach to the smallest), with full or fractional openness: bit-exact."""
+    from peclet.pnm import synth
+
+    if case == "side_by_side":
+        sdf = fc.two_chambers_side_by_side_tubes()
+    else:
+        sdf, _ = synth.packing_sdf((32, 32, 32), "uniform", 3.0, 5.0, 0.55, seed=int(case[-1]))
+    f = fc.divergence_free(sdf.shape, (1.0, 1.0, 1.0), seed=3)
+    if alpha != 1.0:
+        rng = np.random.default_rng(4)
+        for k in ("ox", "oy", "oz"):
+            f[k] = np.where(rng.random(sdf.shape) < 0.1, 0.0, alpha)
+    o = oracle(sdf, (0, 0, 0), (1, 1, 1), f)
+    assert len(o["throats"]) > 0
+    compare(run_gpu(capi_ctx, sdf, (0, 0, 0), (1, 1, 1), f), o)

@@ -187,3 +187,18 @@ def test_openmp_baseline_equals_oracle(path):
     key = lambda a: np.sort(np.ascontiguousarray(a).view(np.uint32).reshape(-1, 4).view(
         "u4,u4,u4,u4").ravel())
     np.testing.assert_array_equal(key(p), key(o_pores))
+
+
+def test_oracle_under_sanitizers():
+    """The oracle built with AddressSanitizer + UndefinedBehaviourSanitizer (no recovery, leak
+    check on) runs every function on 40 small and degenerate cases without a report."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    here = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle")
+    r = subprocess.run(["make", "-s", "-C", here, "sanitize"], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "40 cases clean" in r.stdout
