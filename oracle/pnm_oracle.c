@@ -289,6 +289,52 @@ int pnmo_segment_volume(const float* sdf, const int32_t dims[3], const float spa
   return 0;
 }
 
+/* Fixed summation order of the per-pore and per-throat sums (shared with the device's
+ * reduction, k_key_sums in k_flow.hip): the terms t[0..m) of one key are taken in blocks of 2048;
+ * in a block starting at b, "lane" l (0..63) adds t[b + l], t[b + 64 + l], ..., t[b + 64*31 + l]
+ * (those below m) one after another starting from +0.0; the 64 lane sums are combined by a
+ * pairwise tree, s = 1, 2, 4, ..., 32: p[l] = p[l] + p[l + s] for l a multiple of 2s; the block
+ * totals p[0] are added one after another starting from +0.0. */
+static double blocked_sum(const double* t, int64_t stride, int64_t m) {
+  double acc = 0.0;
+  for (int64_t b = 0; b < m; b += 2048) {
+    double p[64];
+    for (int l = 0; l < 64; ++l) {
+      p[l] = 0.0;
+      for (int j = 0; j < 32; ++j) {
+        const int64_t i = b + 64 * (int64_t)j + l;
+        if (i < m) p[l] += t[i * stride];
+      }
+    }
+    for (int sh = 1; sh < 64; sh <<= 1)
+      for (int l = 0; l < 64; l += 2 * sh) p[l] = p[l] + p[l + sh];
+    acc += p[0];
+  }
+  return acc;
+}
+
+/* out[k][key] = blocked sum of component k of the records with that key, taken in record
+ * order (a stable counting sort by key, then blocked_sum per key). vals: K doubles per record. */
+static void keyed_blocked_sums(const int64_t* key, const double* vals, int K, int64_t cnt,
+                               int64_t nkeys, double** out) {
+  int64_t* start = (int64_t*)calloc((size_t)nkeys + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < cnt; ++i) ++start[key[i] + 1];
+  for (int64_t k = 0; k < nkeys; ++k) start[k + 1] += start[k];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nkeys + 1));
+  memcpy(fill, start, sizeof(int64_t) * (size_t)(nkeys + 1));
+  double* sorted = (double*)malloc(sizeof(double) * (size_t)(K * (cnt ? cnt : 1)));
+  for (int64_t i = 0; i < cnt; ++i) {
+    const int64_t pos = fill[key[i]]++;
+    for (int k = 0; k < K; ++k) sorted[(int64_t)K * pos + k] = vals[(int64_t)K * i + k];
+  }
+  for (int64_t k = 0; k < nkeys; ++k)
+    for (int c = 0; c < K; ++c)
+      out[c][k] = blocked_sum(sorted + (int64_t)K * start[k] + c, K, start[k + 1] - start[k]);
+  free(sorted);
+  free(fill);
+  free(start);
+}
+
 /* ---- topology: extractTopologyView, pore_extraction.hpp:401-451 ----
  * Every +x/+y/+z face (wrapped) with differing labels gives (min,max); host sort + unique in
  * signed lexicographic pair order (:448-449). */
@@ -568,7 +614,9 @@ int pnmo_extract_network_flow(const float* sdf, const int32_t dims[3], const flo
     slotOfFace[rr[t].fid] = (int32_t)t;
   }
 
-  /* throat_flux :864-944 (serial loop order z, y, x, d) */
+  /* throat_flux :864-944. The reference adds every face's contributions with unordered FP64
+   * atomics; here each key's (pore's or throat's) terms are gathered in face order (z, y, x, d)
+   * and summed in the fixed blocked order of blocked_sum, which the device reduction shares. */
   double* Q = (double*)calloc((size_t)(nt ? nt : 1), sizeof(double));
   double* A = (double*)calloc((size_t)(nt ? nt : 1), sizeof(double));
   double* Cx = (double*)calloc((size_t)(nt ? nt : 1), sizeof(double));
@@ -581,6 +629,12 @@ int pnmo_extract_network_flow(const float* sdf, const int32_t dims[3], const flo
     const float oxo = origin[0], oyo = origin[1], ozo = origin[2];
     const float sx = spacing[0], sy = spacing[1], sz = spacing[2];
     const double Lw[3] = {(double)Nx * sx, (double)Ny * sy, (double)Nz * sz};
+    /* terms in face order: residual (pore, signed q) and throat (slot, q, area, C) records */
+    int64_t nr = 0, ntr = 0;
+    int64_t* rkey = (int64_t*)malloc(sizeof(int64_t) * (size_t)(2 * nf + 1));
+    double* rq = (double*)malloc(sizeof(double) * (size_t)(2 * nf + 1));
+    int64_t* tkey = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nf + 1));
+    double* tv = (double*)malloc(sizeof(double) * 5 * (size_t)(nf + 1));
     for (int izz = 0; izz < Nz; ++izz)
       for (int iyy = 0; iyy < Ny; ++iyy)
         for (int ixx = 0; ixx < Nx; ++ixx) {
@@ -594,16 +648,20 @@ int pnmo_extract_network_flow(const float* sdf, const int32_t dims[3], const flo
             const double opn = ho ? (d == 0 ? ox[nb] : (d == 1 ? oy[nb] : oz[nb])) : 1.0;
             const double area = d == 0 ? Ax : (d == 1 ? Ay : Az);
             const double q = opn * vel * area;
-            if (a > 0) resid[a - 1] += q;
-            if (b > 0) resid[b - 1] += -q;
+            if (a > 0) {
+              rkey[nr] = a - 1;
+              rq[nr++] = q;
+            }
+            if (b > 0) {
+              rkey[nr] = b - 1;
+              rq[nr++] = -q;
+            }
             const int64_t rt = fpar[3 * (int64_t)c + d];
             if (rt >= 0) {
               const int64_t slot = slotOfFace[rt];
               const int lo = a < b ? a : b;
               if (slot >= 0) {
                 const double w0 = opn * area;
-                Q[slot] += (a < b ? q : -q);
-                A[slot] += w0;
                 const int pk = peak[lo - 1];
                 const int pkx = pk % Nx, pky = (pk / Nx) % Ny, pkz = pk / (Nx * Ny);
                 double fp[3] = {oxo + (ixx + (d == 0 ? 0.5 : 0.0)) * (double)sx,
@@ -616,13 +674,24 @@ int pnmo_extract_network_flow(const float* sdf, const int32_t dims[3], const flo
                   dv -= Lw[a2] * round(dv / Lw[a2]);
                   fp[a2] = dv;
                 }
-                Cx[slot] += w0 * fp[0];
-                Cy[slot] += w0 * fp[1];
-                Cz[slot] += w0 * fp[2];
+                tkey[ntr] = slot;
+                tv[5 * ntr + 0] = (a < b ? q : -q);
+                tv[5 * ntr + 1] = w0;
+                tv[5 * ntr + 2] = w0 * fp[0];
+                tv[5 * ntr + 3] = w0 * fp[1];
+                tv[5 * ntr + 4] = w0 * fp[2];
+                ++ntr;
               }
             }
           }
         }
+    keyed_blocked_sums(rkey, rq, 1, nr, np, &resid);
+    double* touts[5] = {Q, A, Cx, Cy, Cz};
+    keyed_blocked_sums(tkey, tv, 5, ntr, nt, touts);
+    free(rkey);
+    free(rq);
+    free(tkey);
+    free(tv);
   }
 
   /* throat_dp (host in the reference) :958-995 */

@@ -110,52 +110,108 @@ __global__ __launch_bounds__(256) void k_flow_lab(int64_t n, const int32_t* __re
 }
 
 // ---------------- faces ----------------
-// nf::face_init (:681-696). Also compacts the core faces, the film faces and every basin
-// boundary face touching a pore (the residual's faces) into lists.
-__global__ __launch_bounds__(256) void k_face_init(
+// Block-wide exclusive prefix of a per-thread count in 0..7 (three ballots per wave, one LDS
+// word per wave and list). Every thread of the block must call it; the caller separates two
+// uses of the same `wcnt` words by a barrier.
+__device__ __forceinline__ void small_count_ballots(unsigned cnt, int lane, unsigned& below,
+                                                    unsigned& wave_total) {
+  const uint64_t b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
+  const uint64_t m = lane ? (~0ull >> (64 - lane)) : 0ull;
+  below = (unsigned)(__popcll(b0 & m) + 2 * __popcll(b1 & m) + 4 * __popcll(b2 & m));
+  wave_total = (unsigned)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+}
+
+constexpr int kFaceBlock = 256;                  // cells per block of k_face_init
+constexpr int kFaceCap = 3 * kFaceBlock;         // staged faces per block and list
+constexpr int kResCap = 2 * kFaceCap;            // staged residual records per block
+
+// nf::face_init (:681-696). Also stages, per block and in face order, the core faces, the film
+// faces, every interface face and every basin boundary face touching a pore (one residual record
+// per pore side, (pore index << 32 | face)); compacting the blocks in block order gives lists in
+// ascending face order, so the records need only a stable sort on their key bits.
+__global__ __launch_bounds__(kFaceBlock) void k_face_init(
     const float* __restrict__ sdf, const int32_t* __restrict__ lab, Geo g, int64_t n,
     const double* __restrict__ ox, const double* __restrict__ oy, const double* __restrict__ oz,
     int64_t* __restrict__ fpar, int64_t* __restrict__ fpair, int64_t* __restrict__ par,
-    uint32_t* __restrict__ core_list, uint32_t* __restrict__ film_list,
-    uint64_t* __restrict__ res_rec, unsigned long long* __restrict__ ctr) {
-  const int64_t c64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+    uint32_t* __restrict__ st_core, uint32_t* __restrict__ st_film, uint32_t* __restrict__ st_itf,
+    uint32_t* __restrict__ cnt3, uint64_t* __restrict__ st_res, uint32_t* __restrict__ cnt_res) {
+  __shared__ unsigned wc[4][4];
+  const int64_t c64 = (int64_t)blockIdx.x * kFaceBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool in = c64 < n;
   const int c = (int)c64;
   int x = 0, y = 0, z = 0;
   if (in) cell_xyz(c, g, x, y, z);
   const int a = in ? lab[c] : 0;
   const float sc = in ? sdf[c] : 0.0f;
-  for (int d = 0; d < 3; ++d) {
-    bool is_core = false, is_film = false;
-    int nres = 0;
-    int64_t f = 0;
-    int b = 0;
-    if (in) {
-      f = 3 * c64 + d;
+  unsigned core_m = 0, film_m = 0;  // bit d: the +d face is a core / film face
+  int rk0[3] = {-1, -1, -1}, rk1[3] = {-1, -1, -1};  // pore indices of the residual records
+  if (in) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int64_t f = 3 * c64 + d;
       const int nb = nb_cell(x, y, z, d, g);
-      b = lab[nb];
+      const int b = lab[nb];
       const double opn = ox ? (d == 0 ? ox[nb] : (d == 1 ? oy[nb] : oz[nb])) : 1.0;
       const bool itf = (a != b && a > 0 && b > 0 && opn > 0.0);
-      is_core = itf && sc > 0.0f && sdf[nb] > 0.0f;
-      is_film = itf && !is_core;
+      const bool is_core = itf && sc > 0.0f && sdf[nb] > 0.0f;
       par[f] = is_core ? f : -1;
       fpar[f] = itf ? (is_core ? f : kSent) : -1;
       fpair[f] = itf ? (((int64_t)(a < b ? a : b) << 32) | (int64_t)(a < b ? b : a)) : -1;
-      if (a != b) nres = (a > 0) + (b > 0);
+      core_m |= (unsigned)is_core << d;
+      film_m |= (unsigned)(itf && !is_core) << d;
+      if (a != b) {
+        if (a > 0) {
+          rk0[d] = a - 1;
+          if (b > 0) rk1[d] = b - 1;
+        } else if (b > 0) {
+          rk0[d] = b - 1;
+        }
+      }
     }
-    const uint64_t s1 = wave_append(&ctr[0], is_core, lane);
-    if (is_core) core_list[s1] = (uint32_t)f;
-    const uint64_t s2 = wave_append(&ctr[1], is_film, lane);
-    if (is_film) film_list[s2] = (uint32_t)f;
-    // residual records: (pore index << 32 | face)
-    const uint64_t s3 = wave_append(&ctr[2], nres > 0, lane);
-    const uint64_t s4 = wave_append(&ctr[2], nres > 1, lane);
-    if (nres > 0) {
-      const int k0 = a > 0 ? a - 1 : b - 1;
-      res_rec[s3] = ((uint64_t)k0 << 32) | (uint64_t)f;
+  }
+  const unsigned n_core = __popc(core_m), n_film = __popc(film_m), n_itf = __popc(core_m | film_m);
+  unsigned n_res = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) n_res += (rk0[d] >= 0) + (rk1[d] >= 0);
+  unsigned below[4], wt[4];
+  small_count_ballots(n_core, lane, below[0], wt[0]);
+  small_count_ballots(n_film, lane, below[1], wt[1]);
+  small_count_ballots(n_itf, lane, below[2], wt[2]);
+  small_count_ballots(n_res, lane, below[3], wt[3]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wc[k][wave] = wt[k];
+  __syncthreads();
+  unsigned pos[4], tot[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    unsigned base = 0, t = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const unsigned cw = wc[k][w];
+      if (w < wave) base += cw;
+      t += cw;
     }
-    if (nres > 1) res_rec[s4] = ((uint64_t)(b - 1) << 32) | (uint64_t)f;
+    pos[k] = base + below[k];
+    tot[k] = t;
+  }
+  const int64_t b32 = (int64_t)blockIdx.x * kFaceCap;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const uint32_t f = (uint32_t)(3 * c64 + d);
+    if (core_m >> d & 1u) st_core[b32 + pos[0]++] = f;
+    if (film_m >> d & 1u) st_film[b32 + pos[1]++] = f;
+    if ((core_m | film_m) >> d & 1u) st_itf[b32 + pos[2]++] = f;
+    if (rk0[d] >= 0) st_res[(int64_t)blockIdx.x * kResCap + pos[3]++] = ((uint64_t)rk0[d] << 32) | f;
+    if (rk1[d] >= 0) st_res[(int64_t)blockIdx.x * kResCap + pos[3]++] = ((uint64_t)rk1[d] << 32) | f;
+  }
+  if (threadIdx.x == 0) {
+    const int64_t nb = gridDim.x;
+    cnt3[blockIdx.x] = tot[0];
+    cnt3[nb + blockIdx.x] = tot[1];
+    cnt3[2 * nb + blockIdx.x] = tot[2];
+    cnt_res[blockIdx.x] = tot[3];
   }
 }
 
@@ -375,64 +431,113 @@ __device__ __forceinline__ double face_q(const FluxArgs& A, int64_t f, int& a, i
   return opn * vel * area;
 }
 
-// Per-pore residual: the pore's boundary faces in face order (:898-901).
-__global__ void k_resid_sum(const uint64_t* __restrict__ rec, const int64_t* __restrict__ start,
-                            const int64_t* __restrict__ end, int64_t np, FluxArgs A,
-                            double* __restrict__ resid) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= np) return;
-  double acc = 0.0;
-  for (int64_t i = start[k]; i < end[k]; ++i) {
-    const int64_t f = (int64_t)(rec[i] & 0xffffffffull);
-    int a, b, x, y, z, d;
-    double w0;
-    const double q = face_q(A, f, a, b, w0, x, y, z, d);
-    acc += (a - 1 == (int)k) ? q : -q;
-  }
-  resid[k] = acc;
+// Per-record terms, computed in parallel; the sums below add them in record order.
+// Residual (:898-901): the signed q of each (pore, face) record.
+__global__ __launch_bounds__(256) void k_resid_terms(const uint64_t* __restrict__ rec,
+                                                     int64_t cnt, FluxArgs A,
+                                                     double* __restrict__ term) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint64_t r = rec[i];
+  const int64_t f = (int64_t)(r & 0xffffffffull);
+  const int k = (int)(r >> 32);
+  int a, b, x, y, z, d;
+  double w0;
+  const double q = face_q(A, f, a, b, w0, x, y, z, d);
+  term[i] = (a - 1 == k) ? q : -q;
 }
 
-// Per-throat Q, A, area-weighted min-imaged face centre (:903-939), face order.
-__global__ void k_throat_sum(const uint64_t* __restrict__ rec, const int64_t* __restrict__ start,
-                             const int64_t* __restrict__ end, int64_t nt, FluxArgs A,
-                             const int32_t* __restrict__ peak, float oxo, float oyo, float ozo,
-                             float sx, float sy, float sz, double* __restrict__ Q,
-                             double* __restrict__ Ar, double* __restrict__ Cx,
-                             double* __restrict__ Cy, double* __restrict__ Cz) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
+// Throat (:903-939): the oriented q, the open area, and the area-weighted face centre relative
+// to the lower pore's peak-voxel centre, min-imaged per axis.
+__global__ __launch_bounds__(256) void k_throat_terms(
+    const uint64_t* __restrict__ rec, int64_t cnt, FluxArgs A, const int32_t* __restrict__ peak,
+    float oxo, float oyo, float ozo, float sx, float sy, float sz, double* __restrict__ tq,
+    double* __restrict__ ta, double* __restrict__ tcx, double* __restrict__ tcy,
+    double* __restrict__ tcz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const int64_t f = (int64_t)(rec[i] & 0xffffffffull);
+  int a, b, ixx, iyy, izz, d;
+  double w0;
+  const double q = face_q(A, f, a, b, w0, ixx, iyy, izz, d);
+  const int lo = a < b ? a : b;
   const double Lw[3] = {(double)A.g.nx * sx, (double)A.g.ny * sy, (double)A.g.nz * sz};
-  double q_acc = 0.0, a_acc = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-  for (int64_t i = start[t]; i < end[t]; ++i) {
-    const int64_t f = (int64_t)(rec[i] & 0xffffffffull);
-    int a, b, ixx, iyy, izz, d;
-    double w0;
-    const double q = face_q(A, f, a, b, w0, ixx, iyy, izz, d);
-    const int lo = a < b ? a : b;
-    q_acc += (a < b ? q : -q);
-    a_acc += w0;
-    const int pk = peak[lo - 1];
-    int pkx, pky, pkz;
-    cell_xyz(pk, A.g, pkx, pky, pkz);
-    double fp[3] = {oxo + (ixx + (d == 0 ? 0.5 : 0.0)) * (double)sx,
-                    oyo + (iyy + (d == 1 ? 0.5 : 0.0)) * (double)sy,
-                    ozo + (izz + (d == 2 ? 0.5 : 0.0)) * (double)sz};
-    const double pc[3] = {oxo + pkx * (double)sx, oyo + pky * (double)sy,
-                          ozo + pkz * (double)sz};
-    for (int a2 = 0; a2 < 3; ++a2) {
-      double dv = fp[a2] - pc[a2];
-      dv -= Lw[a2] * round(dv / Lw[a2]);
-      fp[a2] = dv;
-    }
-    cx += w0 * fp[0];
-    cy += w0 * fp[1];
-    cz += w0 * fp[2];
+  const int pk = peak[lo - 1];
+  int pkx, pky, pkz;
+  cell_xyz(pk, A.g, pkx, pky, pkz);
+  double fp[3] = {oxo + (ixx + (d == 0 ? 0.5 : 0.0)) * (double)sx,
+                  oyo + (iyy + (d == 1 ? 0.5 : 0.0)) * (double)sy,
+                  ozo + (izz + (d == 2 ? 0.5 : 0.0)) * (double)sz};
+  const double pc[3] = {oxo + pkx * (double)sx, oyo + pky * (double)sy, ozo + pkz * (double)sz};
+  for (int a2 = 0; a2 < 3; ++a2) {
+    double dv = fp[a2] - pc[a2];
+    dv -= Lw[a2] * round(dv / Lw[a2]);
+    fp[a2] = dv;
   }
-  Q[t] = q_acc;
-  Ar[t] = a_acc;
-  Cx[t] = cx;
-  Cy[t] = cy;
-  Cz[t] = cz;
+  tq[i] = a < b ? q : -q;
+  ta[i] = w0;
+  tcx[i] = w0 * fp[0];
+  tcy[i] = w0 * fp[1];
+  tcz[i] = w0 * fp[2];
+}
+
+template <int K>
+struct TermSets {
+  const double* in[K];
+  double* out[K];
+};
+
+// Keyed sums in a fixed order (the reference adds with unordered FP64 atomics, :898-938; the
+// oracle sums in this same order, so the results are bit-identical): one wave per key takes the
+// key's terms [start, end) in blocks of 2048; lane l adds the block's terms l, l + 64, ...,
+// l + 64*31 one after another from +0.0 (each step a coalesced 64-lane load), the 64 lane sums
+// are combined by a pairwise tree (xor butterfly, s = 1, 2, ..., 32), and the block totals are
+// added one after another. K term sequences run as independent chains.
+template <int K>
+__global__ __launch_bounds__(256) void k_key_sums(const int64_t* __restrict__ start,
+                                                  const int64_t* __restrict__ end, int64_t nkeys,
+                                                  TermSets<K> T) {
+  const int lane = threadIdx.x & 63;
+  const int64_t key = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (key >= nkeys) return;  // whole wave
+  const int64_t s = start[key], e = end[key];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int64_t b = s; b < e; b += 2048) {
+    double p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) p[k] = 0.0;
+    if (b + 2048 <= e) {
+      double v[8][K];
+      for (int j0 = 0; j0 < 32; j0 += 8) {  // eight loads in flight per chain
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+          for (int k = 0; k < K; ++k) v[j][k] = T.in[k][b + 64 * (j0 + j) + lane];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+          for (int k = 0; k < K; ++k) p[k] += v[j][k];
+      }
+    } else {
+      for (int j = 0; j < 32; ++j) {
+        const int64_t i = b + 64 * j + lane;
+        if (i < e)
+#pragma unroll
+          for (int k = 0; k < K; ++k) p[k] += T.in[k][i];
+      }
+    }
+#pragma unroll
+    for (int sh = 1; sh < 64; sh <<= 1)
+#pragma unroll
+      for (int k = 0; k < K; ++k) p[k] = p[k] + __shfl_xor(p[k], sh, 64);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += p[k];
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) T.out[k][key] = acc[k];
 }
 
 // throat_dp (host in the reference, :958-995).
@@ -498,9 +603,10 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
                  pnm_flow_result_t* out) {
   const int64_t n = v.n;
   const Geo g{v.nx, v.ny, v.nz};
+  if (3 * n > (int64_t)0xffffffffLL)
+    fail(PNM_EINVAL, "extract_network_flow: 3*Nx*Ny*Nz must fit in 32 bits (face ids)");
   int32_t* seg = c->arena.get_n<int32_t>((size_t)n);
-  int32_t* roots = c->arena.get_n<int32_t>((size_t)n);
-  SegPeaks sp = segment_with_peaks(c, sdf, v, spc, seg, roots);
+  SegPeaks sp = segment_with_peaks(c, sdf, v, spc, seg, nullptr);
   const int64_t np = sp.n_pores;
   if (np <= 0) {  // np == 0 returns empty in the reference; np < 0 cannot happen here
     PNM_HIP(hipStreamSynchronize(c->stream));
@@ -528,33 +634,54 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256)), dim3(256), 0, n, next_all, seg,
              lab);
 
-  // faces
+  // faces: labels, pair keys and the face-ordered lists (core, film, all interface faces,
+  // residual records), staged per block and compacted in block order
   const int64_t nf = 3 * n;
   int64_t* fpar = c->arena.get_n<int64_t>((size_t)nf);
   int64_t* fpair = c->arena.get_n<int64_t>((size_t)nf);
   int64_t* par = c->arena.get_n<int64_t>((size_t)nf);
+  const int64_t nbf = cdiv(n, kFaceBlock);
+  uint32_t* st_core = c->arena.get_n<uint32_t>((size_t)(nbf * kFaceCap));
+  uint32_t* st_film = c->arena.get_n<uint32_t>((size_t)(nbf * kFaceCap));
+  uint32_t* st_itf = c->arena.get_n<uint32_t>((size_t)(nbf * kFaceCap));
+  uint64_t* st_res = c->arena.get_n<uint64_t>((size_t)(nbf * kResCap));
+  uint32_t* cnt3 = c->arena.get_n<uint32_t>((size_t)(3 * nbf));
+  uint32_t* cnt_res = c->arena.get_n<uint32_t>((size_t)nbf);
+  PNM_LAUNCH(c, "k_face_init", k_face_init, dim3((unsigned)nbf), dim3(kFaceBlock), 0, sdf, lab,
+             g, n, ox, oy, oz, fpar, fpair, par, st_core, st_film, st_itf, cnt3, st_res, cnt_res);
   uint32_t* core_list = c->arena.get_n<uint32_t>((size_t)nf);
   uint32_t* film_list = c->arena.get_n<uint32_t>((size_t)nf);
+  uint32_t* itf_list = c->arena.get_n<uint32_t>((size_t)nf);
   uint64_t* res_rec = c->arena.get_n<uint64_t>((size_t)(2 * nf));
-  PNM_LAUNCH(c, "k_face_init", k_face_init, dim3(cdiv(n, 256)), dim3(256), 0, sdf, lab, g, n, ox,
-             oy, oz, fpar, fpair, par, core_list, film_list, res_rec, uctr);
-  uint64_t h[3];
-  readback(c, ctr, 3, h);
+  {
+    const uint32_t* st[3] = {st_core, st_film, st_itf};
+    uint32_t* outs[3] = {core_list, film_list, itf_list};
+    uint64_t* tots[3] = {ctr + 0, ctr + 1, ctr + 6};
+    compact_staged_multi(c, 3, st, cnt3, nbf, kFaceCap, outs, tots);
+    compact_staged64(c, st_res, cnt_res, nbf, kResCap, res_rec, ctr + 2);
+  }
+  uint64_t h[8];
+  readback(c, ctr, 7, h);
   const int64_t ncore = (int64_t)h[0], nfilm = (int64_t)h[1], nres = (int64_t)h[2];
+  const int64_t nitf = (int64_t)h[6];
 
   // core tier
   face_ccl(c, core_list, ncore, g, fpair, par, fpar);
   // film attachment to a fixpoint
+  // (two sweeps per read-back: a sweep at the fixpoint changes nothing, and every sweep that
+  // changes something lowers at least one film label, so nfilm + 2 sweeps always suffice)
   if (nfilm > 0) {
     int64_t sweeps = 0;
     while (true) {
       fill_u64(c, ctr + 3, 0, 1);
-      PNM_LAUNCH(c, "k_film_attach", k_film_attach, dim3(cdiv(nfilm, 256)), dim3(256), 0,
-                 film_list, nfilm, g, fpair, fpar, uctr + 3);
-      uint64_t ch;
-      readback(c, ctr + 3, 1, &ch);
-      if (ch == 0) break;
-      if (++sweeps > nfilm + 2) fail(PNM_ENOCONVERGE, "film attachment did not converge");
+      for (int k = 0; k < 2; ++k)
+        PNM_LAUNCH(c, "k_film_attach", k_film_attach, dim3(cdiv(nfilm, 256)), dim3(256), 0,
+                   film_list, nfilm, g, fpair, fpar, uctr + 3);
+      uint64_t ch[2];
+      readback(c, ctr + 3, 1, ch);
+      if (ch[0] == 0) break;
+      sweeps += 2;
+      if (sweeps > nfilm + 2) fail(PNM_ENOCONVERGE, "film attachment did not converge");
     }
   }
   // leftover tier
@@ -611,41 +738,51 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
     radix_sort_u64(c, key, order, nt, 0, 2 * B);
     PNM_LAUNCH(c, "k_throat_slots", k_throat_slots, dim3(cdiv(nt, 256)), dim3(256), 0, root_fid,
                order, nt, fpair, par, throats);
-    // throat records over every interface face
-    const int64_t nitf = ncore + nfilm;
+    // throat records over every interface face, in face order; a stable sort on the slot
+    // keeps face order within each throat
     uint64_t* trec = c->arena.get_n<uint64_t>((size_t)nitf);
-    if (ncore > 0)
-      PNM_LAUNCH(c, "k_throat_records", k_throat_records, dim3(cdiv(ncore, 256)), dim3(256), 0,
-                 core_list, ncore, fpar, par, trec);
-    if (nfilm > 0)
-      PNM_LAUNCH(c, "k_throat_records", k_throat_records, dim3(cdiv(nfilm, 256)), dim3(256), 0,
-                 film_list, nfilm, fpar, par, trec + ncore);
-    radix_sort_u64(c, trec, nullptr, nitf, 0, 32 + nbits64((uint64_t)nt));
+    PNM_LAUNCH(c, "k_throat_records", k_throat_records, dim3(cdiv(nitf, 256)), dim3(256), 0,
+               itf_list, nitf, fpar, par, trec);
+    radix_sort_u64(c, trec, nullptr, nitf, 32, 32 + nbits64((uint64_t)nt));
     int64_t* ts = c->arena.get_n<int64_t>((size_t)nt);
     int64_t* te = c->arena.get_n<int64_t>((size_t)nt);
     fill_u64(c, reinterpret_cast<uint64_t*>(ts), 0, nt);
     fill_u64(c, reinterpret_cast<uint64_t*>(te), 0, nt);
     PNM_LAUNCH(c, "k_seg_bounds", k_seg_bounds, dim3(cdiv(nitf, 256)), dim3(256), 0, trec, nitf,
                ts, te);
-    PNM_LAUNCH(c, "k_throat_sum", k_throat_sum, dim3(cdiv(nt, 64)), dim3(64), 0, trec, ts, te, nt,
-               fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], Q, Ar, Cx, Cy, Cz);
+    double* terms = c->arena.get_n<double>((size_t)(5 * nitf));
+    PNM_LAUNCH(c, "k_throat_terms", k_throat_terms, dim3(cdiv(nitf, 256)), dim3(256), 0, trec,
+               nitf, fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], terms,
+               terms + nitf, terms + 2 * nitf, terms + 3 * nitf, terms + 4 * nitf);
+    TermSets<5> T{};
+    double* outs[5] = {Q, Ar, Cx, Cy, Cz};
+    for (int k = 0; k < 5; ++k) {
+      T.in[k] = terms + k * nitf;
+      T.out[k] = outs[k];
+    }
+    PNM_LAUNCH(c, "k_key_sums", k_key_sums<5>, dim3(cdiv(nt, 4)), dim3(256), 0, ts, te, nt, T);
     PNM_LAUNCH(c, "k_throat_dp", k_throat_dp, dim3(cdiv(nt, 256)), dim3(256), 0, throats, nt,
                pores, pp, peak, Ar, Cx, Cy, Cz, g, org[0], org[1], org[2], spc[0], spc[1],
                spc[2], grad_p[0], grad_p[1], grad_p[2], dp, tc);
   }
-  // residual
+  // residual: records in face order, stably sorted by pore
   {
     int64_t* rs = c->arena.get_n<int64_t>((size_t)np);
     int64_t* re = c->arena.get_n<int64_t>((size_t)np);
     fill_u64(c, reinterpret_cast<uint64_t*>(rs), 0, np);
     fill_u64(c, reinterpret_cast<uint64_t*>(re), 0, np);
+    double* rterm = c->arena.get_n<double>((size_t)std::max<int64_t>(nres, 1));
     if (nres > 0) {
-      radix_sort_u64(c, res_rec, nullptr, nres, 0, 32 + nbits64((uint64_t)np));
+      radix_sort_u64(c, res_rec, nullptr, nres, 32, 32 + nbits64((uint64_t)np));
       PNM_LAUNCH(c, "k_seg_bounds", k_seg_bounds, dim3(cdiv(nres, 256)), dim3(256), 0, res_rec,
                  nres, rs, re);
+      PNM_LAUNCH(c, "k_resid_terms", k_resid_terms, dim3(cdiv(nres, 256)), dim3(256), 0, res_rec,
+                 nres, fa, rterm);
     }
-    PNM_LAUNCH(c, "k_resid_sum", k_resid_sum, dim3(cdiv(np, 64)), dim3(64), 0, res_rec, rs, re,
-               np, fa, resid);
+    TermSets<1> T{};
+    T.in[0] = rterm;
+    T.out[0] = resid;
+    PNM_LAUNCH(c, "k_key_sums", k_key_sums<1>, dim3(cdiv(np, 4)), dim3(256), 0, rs, re, np, T);
   }
 
   out->np = np;

@@ -164,3 +164,63 @@ def test_fractional_openness_scales_flux_and_area(alpha):
                                   ox=z, oy=o, oz=o)
     assert np.all(r0["throat_flow"] == 0.0)
     assert np.all(r0["pore_residual"] == 0.0)
+
+
+def _blocked_sum(t):
+    """The fixed blocked summation order, restated: blocks of 2048; lane l adds terms l, l + 64,
+    ... serially from +0.0; pairwise tree over the 64 lanes; block totals added serially."""
+    acc = 0.0
+    for b in range(0, len(t), 2048):
+        blk = t[b:b + 2048]
+        p = []
+        for lane in range(64):
+            s = 0.0
+            for v in blk[lane::64]:
+                s += v
+            p.append(s)
+        sh = 1
+        while sh < 64:
+            for lane in range(0, 64, 2 * sh):
+                p[lane] = p[lane] + p[lane + sh]
+            sh *= 2
+        acc += p[0]
+    return acc
+
+
+def test_residual_sum_order_and_accuracy():
+    """The pore residuals equal an independent numpy statement of the per-face terms (q of the
+    +d face, read at the +d neighbour, + for the pore on the cell side, - on the neighbour side)
+    summed per pore in face order in the blocked order, bit for bit, and the exactly rounded sum
+    (math.fsum) within 1e-12 of the largest term."""
+    import math
+
+    sdf = lattice48()
+    spc = (1.5, 1.0, 0.5)
+    f = fc.divergence_free(sdf.shape, spc, seed=7)
+    r = orc.extract_network_flow(sdf, (0, 0, 0), spc, f["u"], f["v"], f["w"], f["p"],
+                                 debug=True)
+    lab = r["flow_lab"].reshape(sdf.shape)
+    sz, sy, sx = spc
+    area = (sy * sz, sx * sz, sx * sy)
+    vel = (f["u"], f["v"], f["w"])
+    axis = (2, 1, 0)
+    n = sdf.size
+    keys, terms, order = [], [], []
+    for d in range(3):
+        nbl = np.roll(lab, -1, axis=axis[d]).ravel()
+        q = (1.0 * np.roll(vel[d], -1, axis=axis[d]) * area[d]).ravel()
+        a = lab.ravel()
+        face = 3 * np.arange(n) + d
+        m = (a != nbl) & (a > 0)
+        keys.append(a[m] - 1), terms.append(q[m]), order.append(2 * face[m])
+        m = (a != nbl) & (nbl > 0)
+        keys.append(nbl[m] - 1), terms.append(-q[m]), order.append(2 * face[m] + 1)
+    keys, terms, order = (np.concatenate(x) for x in (keys, terms, order))
+    srt = np.lexsort((order, keys))
+    keys, terms = keys[srt], terms[srt]
+    bounds = np.searchsorted(keys, np.arange(len(r["pore_residual"]) + 1))
+    big = np.abs(terms).max()
+    for k, got in enumerate(r["pore_residual"]):
+        t = terms[bounds[k]:bounds[k + 1]].tolist()
+        assert got == _blocked_sum(t), k
+        assert abs(got - math.fsum(t)) <= 1e-12 * big

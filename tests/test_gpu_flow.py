@@ -1,8 +1,8 @@
 """HIP extract_network_flow vs the oracle (GPU).
 
-Discrete outputs (pores, throat list) must be bit-exact. The FP64 sums are accumulated per
-throat / pore in ascending face order on the device — the oracle's serial order — so Q, A, the
-residual, pore pressures and dp are expected bit-identical too; the stated bar is 1e-12
+Discrete outputs (pores, throat list) must be bit-exact. The FP64 sums are taken per throat /
+pore over the faces in ascending face order, in the fixed blocked order the oracle uses too, so
+Q, A, the residual, pore pressures and dp are expected bit-identical; the stated bar is 1e-12
 relative (to the largest |Q| for the flux sums), checked first, then exactness.
 """
 from __future__ import annotations
