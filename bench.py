@@ -182,8 +182,10 @@ def bench_batched(ctx, src, reps, seed=4001, count=4096, lo=48, hi=160, sample=2
             "sample_labels_identical": bool(same), "launch": ctx.batch_config()}
 
 
-def bench_flow(ctx, reps, size=256):
-    """extract_network_flow on the chamber lattice with fields from the project Stokes solver."""
+def bench_flow(ctx, reps, size=256, stokes_iters=40000):
+    """extract_network_flow on the chamber lattice with fields from the project Stokes solver
+    (at most `stokes_iters` MINRES iterations: a profiling run may cut the solve short, which
+    changes the field values, not the work of the extraction)."""
     import numpy as np
     import torch
 
@@ -200,7 +202,7 @@ def bench_flow(ctx, reps, size=256):
     torch.cuda.synchronize()
     t = time.time()
     it, rr = ctx.stokes_solve_d(s, dims, (1, 1, 1), (fx, 0.0, 0.0), *f, mu=mu, rtol=1e-10,
-                                max_iter=40000)
+                                max_iter=stokes_iters)
     torch.cuda.synchronize()
     t_stokes = time.time() - t
     args = (s, dims, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), *f[:4])
@@ -318,7 +320,7 @@ def _counter_rows(path):
                 for r in csv.DictReader(f)]
 
 
-def pmc_summary(root, copy_bytes, voxels):
+def pmc_summary(root, copy_bytes, voxels, workload="extract_pore_network"):
     """Markdown summary of four rocprofv3 --pmc runs under `root` (pmc_copy_fetch,
     pmc_copy_write: the calibration kernel; pmc_net_fetch, pmc_net_write: the pipeline), each with
     run_counter_collection.csv. FETCH_SIZE and WRITE_SIZE are KiB per dispatch; each counter is
@@ -347,6 +349,9 @@ def pmc_summary(root, copy_bytes, voxels):
     for k, d in per.items():
         if k.startswith(("vectorized_elementwise", "reduce_kernel", "distribution", "elementwise")):
             continue  # torch's own set-up kernels outside the pipeline
+        if workload != "extract_pore_network" and k.startswith(("k_ms_", "k_st_",
+                                                                 "__amd_rocclr_fillBuffer")):
+            continue  # the Stokes solve (its kernels and memsets) that generates the flow fields
         f, w = d.get("fetch", []), d.get("write", [])
         rows.append((sum(f) / steps + sum(w) / steps, k, len(f) / steps,
                      sum(f) / max(1, len(f)), sum(w) / max(1, len(w)),
@@ -370,7 +375,7 @@ def pmc_summary(root, copy_bytes, voxels):
            "| counter | reads (KiB) | true (KiB) | scale factor |", "|---|---:|---:|---:|",
            f"| FETCH_SIZE | {rf / 1024:.1f} | {copy_bytes / 1024:.1f} | {kf:.4f} |",
            f"| WRITE_SIZE | {rw / 1024:.1f} | {copy_bytes / 1024:.1f} | {kw:.4f} |", "",
-           f"## Pipeline ({steps} passes of extract_pore_network over {voxels} voxels; "
+           f"## Pipeline ({steps} passes of {workload} over {voxels} voxels; "
            "bytes are scaled)", ""] + lines + [
            "", f"Total per step: fetch {tf:.2f} B/voxel, write {tw:.2f} B/voxel "
                f"({tf + tw:.2f} B/voxel; {(tf + tw) * voxels / 1e9:.3f} GB).", "",
@@ -404,6 +409,9 @@ def sq_summary(root, voxels):
     for k, c in agg.items():
         if k.startswith(("vectorized_elementwise", "reduce_kernel", "distribution", "elementwise")):
             continue  # torch's own set-up kernels outside the pipeline
+        if workload != "extract_pore_network" and k.startswith(("k_ms_", "k_st_",
+                                                                 "__amd_rocclr_fillBuffer")):
+            continue  # the Stokes solve (its kernels and memsets) that generates the flow fields
         w, nd = c.get("SQ_WAVES", 0.0), max(1, len(disp[k]))
         if w <= 0:
             continue
@@ -462,6 +470,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--flow-only", action="store_true",
+                    help="run only the extract_network_flow section (profiling)")
+    ap.add_argument("--flow-stokes-iters", type=int, default=40000,
+                    help="MINRES iteration limit of the flow section's Stokes solve")
+    ap.add_argument("--pmc-workload", default="extract_pore_network",
+                    help="workload named in the --pmc-summary table title")
     ap.add_argument("--pmc-summary", default=None, metavar="DIR",
                     help="print the markdown HBM traffic summary of the rocprofv3 --pmc runs under "
                          "DIR (pmc_copy_fetch, pmc_copy_write, pmc_net_fetch, pmc_net_write; "
@@ -480,7 +494,7 @@ def main():
         sys.exit(2)
 
     if args.pmc_summary:
-        print(pmc_summary(args.pmc_summary, args.copy_bytes, args.size ** 3))
+        print(pmc_summary(args.pmc_summary, args.copy_bytes, args.size ** 3, args.pmc_workload))
         return
     if args.sq_summary:
         print(sq_summary(args.sq_summary, args.size ** 3))
@@ -497,6 +511,12 @@ def main():
             torch.neg(src, out=dst)  # a shader kernel; copy_ may go to a DMA engine
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 4, "reps": 5}), flush=True)
+        return
+    if args.flow_only:  # profiling runs of the flow section alone
+        ctx = capi.Context(0)
+        print(json.dumps({"flow": bench_flow(ctx, reps=3, stokes_iters=args.flow_stokes_iters)}),
+              flush=True)
+        ctx.close()
         return
     sdf, desc = make_sdf(args.workload, args.size, args.seed)
     shape = sdf.shape
@@ -597,7 +617,7 @@ def main():
         out["batched"] = bench_batched(ctx, s.view(shape), reps=3)
     if not args.no_flow:
         torch.cuda.empty_cache()
-        out["flow"] = bench_flow(ctx, reps=3)
+        out["flow"] = bench_flow(ctx, reps=3, stokes_iters=args.flow_stokes_iters)
     if not args.no_surrogate:
         torch.cuda.empty_cache()
         out["surrogate"] = bench_surrogate(ctx)
