@@ -117,6 +117,10 @@ struct Ctx {
   float batch_tuned_ms = 0.0f;
   void* batch_graph = nullptr;  // hipGraphExec_t
   uint64_t batch_key = 0;
+  // Workgroup count the scratch was sized for, per batch shape: later calls of the same shape
+  // keep it (same arena requests, same buffers, the same captured graph).
+  int batch_scratch_groups = 0;
+  uint64_t batch_scratch_shape = 0;
   // Throat surrogate: activations of the last forward pass, kept for its backward pass.
   Arena sur;
   void* sur_acts = nullptr;
