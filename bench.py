@@ -256,8 +256,7 @@ def bench_surrogate(ctx, size=256, seed=9999):
     def surrogate_path():
         s2, geo, _ = S.flow_sample(ctx, sdf, geometry_only=True)
         centers, raw = S.throat_inputs(geo, sdf.shape)
-        patches = S.gather_patches(ctx, s2, sdf.shape, centers)
-        pred = model.predict(patches, model.standardize(raw))
+        pred = model.predict_sdf(s2, sdf.shape, centers, model.standardize(raw))
         torch.cuda.synchronize()
         return geo, pred
     surrogate_path()
@@ -271,21 +270,21 @@ def bench_surrogate(ctx, size=256, seed=9999):
     c = float(np.load(wpath)["geometric_c"])
     _, raw = S.throat_inputs(net, sdf.shape)
     err_geo = S.median_rel_error(S.geometric_baseline_predict(raw[ok], c), logg[ok])
-    # inference alone, timed with events on the current stream
+    # inference alone (the fused kernel: patches gathered from the SDF inside it), timed with
+    # HIP events around its launches on the context's stream
     nt = len(pred)
-    _, raw = S.throat_inputs(geo, sdf.shape)
-    centers, _ = S.throat_inputs(geo, sdf.shape)
-    patches = S.gather_patches(ctx, s_dev, sdf.shape, centers)
+    centers, raw = S.throat_inputs(geo, sdf.shape)
     sc = model.standardize(raw)
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    model.predict(patches, sc)
-    e0.record(stream)
+    model.predict_sdf(s_dev, sdf.shape, centers, sc)
+    ctx.profile(True)
+    ctx.profile_only(["k_sur_infer"])
+    ctx.profile_read()
     for _ in range(5):
-        model.predict(patches, sc)
-    e1.record(stream)
+        model.predict_sdf(s_dev, sdf.shape, centers, sc)
     torch.cuda.synchronize()
-    t_inf = e0.elapsed_time(e1) / 5 / 1e3
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    t_inf = prof["k_sur_infer"][0] / prof["k_sur_infer"][1] / 1e3
     conv_flops = sum(2 * co * 27 * ci * d ** 3 for d, ci, co in ((16, 1, 16), (8, 16, 32),
                                                                    (4, 32, 32)))
     tflops = conv_flops * nt / t_inf / 1e12

@@ -251,6 +251,13 @@ pnm_status pnm_sur_adam_d(pnm_ctx_t ctx, float* params_d, const float* grads_d, 
  * (periodic); centers_d: float[batch][3], X,Y,Z in voxel units. */
 pnm_status pnm_sur_patches_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                              const float* centers_d, int32_t batch, float scale, float* out_d);
+/* Fused inference: out_d[b] = the network's output for throat b (the same value as
+ * pnm_sur_forward_d on that throat's pnm_sur_patches_d patch, up to FP32 summation order inside
+ * the MFMA), computed straight from the device SDF without materialising patches or
+ * activations. scalars_d: float[batch][4] (standardised throat scalars). */
+pnm_status pnm_sur_infer_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                           const float* centers_d, const float* scalars_d, int32_t batch,
+                           float scale, const float* params_d, float* out_d);
 
 /* ---- stage-level device entry points (parity tests of single kernels; no reference
  * counterpart as separate calls: each is one step of segmentVolumeView) ---- */

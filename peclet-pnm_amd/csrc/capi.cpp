@@ -653,6 +653,21 @@ pnm_status pnm_sur_adam_d(pnm_ctx_t ctx, float* params_d, const float* grads_d, 
   });
 }
 
+pnm_status pnm_sur_infer_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
+                           const float* centers_d, const float* scalars_d, int32_t batch,
+                           float scale, const float* params_d, float* out_d) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(v.n > 0 && sdf_d && centers_d && scalars_d && params_d && out_d && batch >= 0 &&
+             scale > 0.0f,
+         "non-empty volume, buffers and scale > 0 are required");
+    Ctx* c = check_ctx(ctx);
+    if (batch == 0) return;
+    CallScope cs(c);
+    surrogate_infer(c, sdf_d, v, centers_d, scalars_d, batch, scale, params_d, out_d);
+  });
+}
+
 pnm_status pnm_sur_patches_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                              const float* centers_d, int32_t batch, float scale, float* out_d) {
   return guarded([&] {

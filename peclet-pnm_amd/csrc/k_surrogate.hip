@@ -538,6 +538,251 @@ void surrogate_patches(Ctx* c, const float* sdf, Vol v, const float* centers, in
              v.nx, v.ny, v.nz, centers, B, scale, out);
 }
 
+// ---------------- fused inference ----------------
+// The whole network for one throat at a time in one workgroup (8 waves), persistent over the
+// throats: the 16^3 patch is gathered from the SDF (the next throat's patch is loaded while the
+// current one is computed), and every convolution is an implicit GEMM on the MFMA with both
+// operands in LDS — BF16 weights of all three layers (84 KB, loaded once per workgroup) and the
+// throat's activations with their zero halo — so nothing but the patch, the scalars and the
+// output touches HBM. Bias, ReLU and the 2x pooling are applied to the accumulators and the
+// pooled values rounded to BF16 into the next layer's input; the global average and the dense
+// head run on one wave. Rounding points, the K order of the MFMA chains and the pooling, global
+// average and head summation orders are those of the training forward (surrogate_forward at
+// the batch sizes where its GEMMs need no split over K), so both give the same log g up to the
+// order of the FP32 additions inside an MFMA instruction.
+struct SurOffs {
+  int w0, b0, w1, b1, w2, b2, W4, b4, W5, b5;
+};
+
+constexpr int SI_THREADS = 512;
+
+__device__ __forceinline__ bf16x4 lds4(const uint16_t* p) {
+  return *reinterpret_cast<const bf16x4*>(p);
+}
+__device__ __forceinline__ float lane_f(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+__global__ __launch_bounds__(SI_THREADS) void k_sur_infer(
+    const float* __restrict__ sdf, int nx, int ny, int nz, const float* __restrict__ centers,
+    const float* __restrict__ scal, int nt, float scale, const float* __restrict__ P, SurOffs L,
+    float* __restrict__ out) {
+  __shared__ __align__(16) uint16_t W0s[16 * 32];
+  __shared__ __align__(16) uint16_t W1s[32 * 432];
+  __shared__ __align__(16) uint16_t W2s[32 * 864];
+  __shared__ float Bs[80];
+  __shared__ __align__(16) uint16_t X0[18 * 18 * 18];  // patch with its zero halo
+  __shared__ __align__(16) uint16_t A1[10 * 10 * 10 * 16];  // pooled layer 0, zero halo
+  __shared__ __align__(16) uint16_t A2[6 * 6 * 6 * 32];  // pooled layer 1, zero halo
+  __shared__ float G2[64 * 32];  // layer 2 after ReLU, [position][channel]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 15, g = lane >> 4;  // MFMA operand row / column and K quarter
+  for (int i = tid; i < 16 * 32; i += SI_THREADS) W0s[i] = f2bf(P[L.w0 + i]);
+  for (int i = tid; i < 32 * 432; i += SI_THREADS) W1s[i] = f2bf(P[L.w1 + i]);
+  for (int i = tid; i < 32 * 864; i += SI_THREADS) W2s[i] = f2bf(P[L.w2 + i]);
+  if (tid < 16) Bs[tid] = P[L.b0 + tid];
+  else if (tid < 48) Bs[tid] = P[L.b1 + tid - 16];
+  else if (tid < 80) Bs[tid] = P[L.b2 + tid - 48];
+  for (int i = tid; i < 18 * 18 * 18; i += SI_THREADS) X0[i] = 0;
+  for (int i = tid; i < 10 * 10 * 10 * 16; i += SI_THREADS) A1[i] = 0;
+  for (int i = tid; i < 6 * 6 * 6 * 32; i += SI_THREADS) A2[i] = 0;
+  __syncthreads();  // the halos are zero before any interior is written
+
+  float pre[8];  // this thread's 8 patch values of the next throat
+  auto fetch = [&](int t) {
+    if (t >= nt) return;
+    const int cx = (int)floorf(centers[3 * t + 0]) - 7;
+    const int cy = (int)floorf(centers[3 * t + 1]) - 7;
+    const int cz = (int)floorf(centers[3 * t + 2]) - 7;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + SI_THREADS * j;
+      const int x = wrap_any(cx + (e & 15), nx), y = wrap_any(cy + ((e >> 4) & 15), ny);
+      const int z = wrap_any(cz + (e >> 8), nz);
+      pre[j] = sdf[((int64_t)z * ny + y) * nx + x] / scale;
+    }
+  };
+  int t = blockIdx.x;
+  fetch(t);
+  for (; t < nt; t += gridDim.x) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + SI_THREADS * j;
+      X0[(((e >> 8) + 1) * 18 + ((e >> 4) & 15) + 1) * 18 + (e & 15) + 1] = f2bf(pre[j]);
+    }
+    __syncthreads();
+    fetch(t + (int)gridDim.x);
+
+    // ---- layer 0: 1 -> 16 channels on 16^3, K = 27 taps (+5 zero); a tile is one x-row ----
+    {
+      const bf16x4 b0 = lds4(W0s + n * 32 + 4 * g), b1 = lds4(W0s + n * 32 + 16 + 4 * g);
+      const float bias = Bs[n];
+      for (int jr = 0; jr < 8; ++jr) {  // pooled rows (pz, py) = wave * 8 + jr
+        const int q = wave * 8 + jr, pz = q >> 3, py = q & 7;
+        float v[2][2][4];
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy) {
+            const int z = 2 * pz + dz, y = 2 * py + dy, x = n;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+              bf16x4 a;
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const int tap = 16 * ks + 4 * g + j;
+                short val = 0;
+                if (tap < 27) {
+                  const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
+                  val = (short)X0[((z + tz) * 18 + (y + ty)) * 18 + x + tx];
+                }
+                a[j] = val;
+              }
+              acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ks ? b1 : b0, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {  // row 4g + i = x, column n = channel
+              float c = acc[i] + bias;
+              v[dz][dy][i] = c > 0.f ? c : 0.f;
+            }
+          }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {  // pooled x = 2g + h
+          float s = 0.f;
+#pragma unroll
+          for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+              s += v[dz][dy][2 * h];
+              s += v[dz][dy][2 * h + 1];
+            }
+          A1[(((pz + 1) * 10 + py + 1) * 10 + 2 * g + h + 1) * 16 + n] = f2bf(s * 0.125f);
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- layer 1: 16 -> 32 channels on 8^3, K = 27 taps x 16 channels ----
+    // A tile is 16 positions (two 8-voxel rows y, y+1 of one slice); this wave owns the tiles
+    // (z, rows 2yt, 2yt+1) for z in {2pz, 2pz+1} and yt in {py0, py0+1}, both column tiles.
+    {
+      const int pz = wave >> 1, py0 = 2 * (wave & 1);
+      const int ry = n >> 3, rx = n & 7;  // this lane's A row inside a tile
+      f32x4 acc[2][2][2];
+#pragma unroll
+      for (int a0 = 0; a0 < 2; ++a0)
+#pragma unroll
+        for (int a1 = 0; a1 < 2; ++a1)
+#pragma unroll
+          for (int a2 = 0; a2 < 2; ++a2) acc[a0][a1][a2] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int tap = 0; tap < 27; ++tap) {
+        const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
+        const bf16x4 bf0 = lds4(W1s + n * 432 + tap * 16 + 4 * g);
+        const bf16x4 bf1 = lds4(W1s + (16 + n) * 432 + tap * 16 + 4 * g);
+#pragma unroll
+        for (int jy = 0; jy < 2; ++jy)
+#pragma unroll
+          for (int dz = 0; dz < 2; ++dz) {
+            const int z = 2 * pz + dz, y = 2 * (py0 + jy) + ry;
+            const bf16x4 a =
+                lds4(A1 + (((z + tz) * 10 + y + ty) * 10 + rx + tx) * 16 + 4 * g);
+            acc[jy][dz][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf0, acc[jy][dz][0],
+                                                                      0, 0, 0);
+            acc[jy][dz][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf1, acc[jy][dz][1],
+                                                                      0, 0, 0);
+          }
+      }
+      // C row 4g + i: y offset g >> 1, x = 4 (g & 1) + i; the y + 1 row sits 32 lanes up
+#pragma unroll
+      for (int jy = 0; jy < 2; ++jy)
+#pragma unroll
+        for (int nt2 = 0; nt2 < 2; ++nt2) {
+          const float bias = Bs[16 + 16 * nt2 + n];
+          float v[2][4], w[2][4];
+#pragma unroll
+          for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const float c = acc[jy][dz][nt2][i] + bias;
+              v[dz][i] = c > 0.f ? c : 0.f;
+              w[dz][i] = __shfl_xor(v[dz][i], 32, 64);
+            }
+          if (g < 2) {
+            const int py = py0 + jy;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {  // pooled x = 2g + h
+              float s = 0.f;
+#pragma unroll
+              for (int dz = 0; dz < 2; ++dz) {
+                s += v[dz][2 * h];
+                s += v[dz][2 * h + 1];
+                s += w[dz][2 * h];
+                s += w[dz][2 * h + 1];
+              }
+              A2[(((pz + 1) * 6 + py + 1) * 6 + 2 * g + h + 1) * 32 + 16 * nt2 + n] =
+                  f2bf(s * 0.125f);
+            }
+          }
+        }
+    }
+    __syncthreads();
+
+    // ---- layer 2: 32 -> 32 channels on 4^3, K = 27 taps x 32 channels; one tile per wave
+    //      (a whole slice z, one column tile) ----
+    {
+      const int z = wave >> 1, nt2 = wave & 1;
+      const int ry = n >> 2, rx = n & 3;
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < 54; ++ks) {
+        const int tap = ks >> 1, c0 = (ks & 1) * 16;
+        const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
+        const bf16x4 a = lds4(A2 + (((z + tz) * 6 + ry + ty) * 6 + rx + tx) * 32 + c0 + 4 * g);
+        const bf16x4 b = lds4(W2s + (16 * nt2 + n) * 864 + ks * 16 + 4 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      }
+      const float bias = Bs[48 + 16 * nt2 + n];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {  // position z*16 + 4g + i (y = g, x = i)
+        const float c = acc[i] + bias;
+        G2[(z * 16 + 4 * g + i) * 32 + 16 * nt2 + n] = c > 0.f ? c : 0.f;
+      }
+    }
+    __syncthreads();
+
+    // ---- global average, scalars, dense head (wave 0) ----
+    if (wave == 0) {
+      float h = 0.f;
+      if (lane < 32) {
+        float s = 0.f;
+        for (int k = 0; k < 64; ++k) s += G2[k * 32 + lane];
+        h = s / 64.0f;
+      } else if (lane < 36) {
+        h = scal[4 * t + lane - 32];
+      }
+      float s = P[L.b4 + lane];
+      for (int k = 0; k < 36; ++k) s += P[L.W4 + lane * 36 + k] * lane_f(h, k);
+      s = s > 0.f ? s : 0.f;
+      const float sh = P[L.W5 + lane] * s;
+      float y = P[L.b5];
+      for (int k = 0; k < 64; ++k) y += lane_f(sh, k);
+      if (lane == 0) out[t] = y;
+    }
+  }
+}
+
+void surrogate_infer(Ctx* c, const float* sdf, Vol v, const float* centers, const float* scalars,
+                     int B, float scale, const float* params, float* out) {
+  const SurLayout L = sur_layout();
+  const SurOffs o{(int)L.w[0], (int)L.b[0], (int)L.w[1], (int)L.b[1], (int)L.w[2], (int)L.b[2],
+                  (int)L.W4, (int)L.b4, (int)L.W5, (int)L.b5};
+  int cus = 0;
+  PNM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int groups = std::min(B, std::max(cus, 1));
+  PNM_LAUNCH(c, "k_sur_infer", k_sur_infer, dim3((unsigned)groups), dim3(SI_THREADS), 0, sdf,
+             v.nx, v.ny, v.nz, centers, scalars, B, scale, params, o, out);
+}
+
 void surrogate_release(Ctx* c) {
   delete static_cast<SurActs*>(c->sur_acts);
   c->sur_acts = nullptr;

@@ -373,5 +373,9 @@ void surrogate_adam(Ctx* c, float* params, const float* grads, float* m, float* 
 void surrogate_patches(Ctx* c, const float* sdf, Vol v, const float* centers, int B, float scale,
                        float* out);
 void surrogate_release(Ctx* c);
+// Fused inference: log g (before the target offset) of B throats straight from the SDF: patch
+// gather around centers (xyz voxel coordinates), the three convolutions, pooling and the head.
+void surrogate_infer(Ctx* c, const float* sdf, Vol v, const float* centers, const float* scalars,
+                     int B, float scale, const float* params, float* out);
 
 }  // namespace pnm

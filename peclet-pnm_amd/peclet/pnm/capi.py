@@ -30,7 +30,7 @@ EXPORTS = (
     "pnm_stage_labels_d", "pnm_stage_roots_d", "pnm_segment_batch_d", "pnm_batch_config",
     "pnm_stokes_solve_d", "pnm_sur_param_count", "pnm_sur_init", "pnm_sur_forward_d",
     "pnm_sur_backward_d", "pnm_sur_loss_d", "pnm_sur_adam_d", "pnm_sur_patches_d",
-    "pnm_segment_volume_noalloc_d",
+    "pnm_segment_volume_noalloc_d", "pnm_sur_infer_d",
 )
 
 # Largest tile-padded volume the no-allocation entry point accepts (PNM_NOALLOC_MAX_PADDED).
@@ -142,6 +142,8 @@ def lib():
         L.pnm_sur_adam_d.restype = st
         L.pnm_sur_patches_d.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp]
         L.pnm_sur_patches_d.restype = st
+        L.pnm_sur_infer_d.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp, vp]
+        L.pnm_sur_infer_d.restype = st
         L.pnm_stage_labels_d.argtypes = [vp, vp, vp, vp, vp, vp]
         L.pnm_stage_roots_d.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         for name in ("pnm_extract_pores_d", "pnm_segment_volume_d",
@@ -333,6 +335,12 @@ class Context:
     def sur_patches_d(self, sdf, dims_xyz, centers, batch, scale, out):
         _check(lib().pnm_sur_patches_d(self.h, _ptr(sdf), _i3(dims_xyz), _ptr(centers),
                                        int(batch), float(scale), _ptr(out)))
+
+    def sur_infer_d(self, sdf, dims_xyz, centers, scalars, batch, scale, params, out):
+        """Fused surrogate inference from the device SDF (no patches, no activations in HBM)."""
+        _check(lib().pnm_sur_infer_d(self.h, _ptr(sdf), _i3(dims_xyz), _ptr(centers),
+                                     _ptr(scalars), int(batch), float(scale), _ptr(params),
+                                     _ptr(out)))
 
 
 def sur_param_count():

@@ -121,6 +121,19 @@ class ThroatSurrogate:
                                    self.params, out[a:b])
         return out[:N] + self.t_mean
 
+    def predict_sdf(self, sdf_dev, shape_zyx, centers, scalars_std):
+        """log g straight from the device SDF (fused kernel: patch gather, convolutions and head
+        in one launch; no patches or activations in HBM). centers: host [N, 3] xyz voxels."""
+        torch = _torch()
+        N = centers.shape[0]
+        out = torch.empty(max(N, 1), device="cuda")
+        if N:
+            c = torch.from_numpy(np.ascontiguousarray(centers, dtype=np.float32)).cuda()
+            s = torch.from_numpy(np.ascontiguousarray(scalars_std, dtype=np.float32)).cuda()
+            self.ctx.sur_infer_d(sdf_dev, tuple(shape_zyx)[::-1], c, s, N, PATCH_SCALE,
+                                 self.params, out)
+        return out[:N] + self.t_mean
+
     def train_step(self, patches, scalars_std, targets, lr, delta=0.25):
         """One Adam step on a device minibatch; returns the loss tensor (device)."""
         torch = _torch()

@@ -139,3 +139,39 @@ def test_stream_switch_orders_pending_work():
             np.testing.assert_array_equal(g2.cpu().numpy(), g1.cpu().numpy())
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("B", [300, 1000])
+def test_fused_inference_matches_forward(capi_ctx, B):
+    """pnm_sur_infer_d (patch gather + network in one kernel) against pnm_sur_patches_d +
+    pnm_sur_forward_d on the same throats, centres spread over the whole periodic volume
+    (patches wrap), more throats than workgroups (the persistent loop), and against the FP64
+    oracle on a subset."""
+    from peclet.pnm import capi
+
+    rng = np.random.default_rng(7 + B)
+    shape = (40, 48, 56)  # z, y, x
+    sdf = (rng.normal(0, 4, size=shape)).astype(np.float32)
+    p = capi.sur_init(3)
+    P = so.unpack(p)
+    for k in ("b0", "b1", "b2", "b4"):
+        P[k] += rng.normal(0, 0.05, P[k].shape)
+    p = so.pack(P).astype(np.float32)
+    centers = (rng.random((B, 3)) * np.array([56, 48, 40])).astype(np.float32)
+    scal = rng.normal(0, 1, size=(B, 4)).astype(np.float32)
+    sd, cd, scd, pd = dev(sdf), dev(centers), dev(scal), dev(p)
+    fused = torch.empty(B, device="cuda")
+    capi_ctx.sur_infer_d(sd, (56, 48, 40), cd, scd, B, 8.0, pd, fused)
+    patches = torch.empty(B * 4096, device="cuda")
+    capi_ctx.sur_patches_d(sd, (56, 48, 40), cd, B, 8.0, patches)
+    ref = torch.empty(B, device="cuda")
+    capi_ctx.sur_forward_d(B, patches, scd, pd, ref)
+    f, r = fused.cpu().numpy(), ref.cpu().numpy()
+    scale = max(1.0, float(np.abs(r).max()))
+    print(f"[fused B={B}] max |fused - forward| = {np.abs(f - r).max():.3e}, "
+          f"bitwise equal {np.array_equal(f, r)}", flush=True)
+    assert np.abs(f - r).max() <= 1e-5 * scale
+    k = 24
+    y_ref, _ = so.forward(p.astype(np.float64), patches[: k * 4096].cpu().numpy().reshape(
+        k, 16, 16, 16), scal[:k], rounding=True)
+    assert np.abs(f[:k] - y_ref).max() <= 2e-3 * max(1.0, np.abs(y_ref).max())
