@@ -22,7 +22,7 @@ oracle (1 core) beside it. Two more objects on the same line:
            tubes r ~ U[3,8], seed 3001) with FP64 fields from the project Stokes solver;
   surrogate: throat conductance g = Q/dp on a held-out 256^3 grain pack: the plain path (Stokes
            solve + extract_network_flow) against the surrogate path (throat geometry from the
-           SDF alone + 16^3 patches + CNN inference with the packaged trained weights).
+           SDF alone, then the fused CNN inference with the packaged trained weights).
 """
 from __future__ import annotations
 
@@ -101,11 +101,12 @@ def cpu_baseline(sdf, slab, serial_slab, reps=5, full=False):
     crop = np.ascontiguousarray(sdf if full else sdf[:slab])
     orc.ref_omp_extract_pore_network(crop, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))  # warm-up
     times, stages = [], []
-    for _ in range(reps):
+    for i in range(reps):
         t = time.time()
         _, _, _, ms = orc.ref_omp_extract_pore_network(crop, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
         times.append(time.time() - t)
         stages.append(ms)
+        print(f"[cpu_baseline] run {i + 1}/{reps}: {times[-1]:.2f} s", file=sys.stderr, flush=True)
     k = int(np.argsort(times)[len(times) // 2])
     dt = times[k]
     threads = orc.ref_omp().pnmr_threads()
@@ -251,20 +252,27 @@ def bench_surrogate(ctx, size=256, seed=9999):
     torch.cuda.synchronize()
     t_plain = time.time() - t0
     logg, ok = S.conductance_targets(net)
-    # bookkeeping alone on the solved fields (second baseline)
-    # surrogate path: geometry from the SDF only, patches, inference (one warm-up, then timed)
+    # bookkeeping alone on the solved fields (second baseline): the plain path minus the solve
+    # surrogate path from the same host SDF: upload, throat geometry without fields (segmentation
+    # and interface faces, surrogate inputs computed on the device), fused inference (one
+    # warm-up, then the median of 5)
     def surrogate_path():
-        s2, geo, _ = S.flow_sample(ctx, sdf, geometry_only=True)
-        centers, raw = S.throat_inputs(geo, sdf.shape)
-        pred = model.predict_sdf(s2, sdf.shape, centers, model.standardize(raw))
-        torch.cuda.synchronize()
+        s2 = torch.from_numpy(np.ascontiguousarray(sdf, dtype=np.float32)).cuda()
+        geo, cen, raw = S.throat_geometry(ctx, s2, sdf.shape)
+        pred = model.predict_device(s2, sdf.shape, cen, raw).cpu()
         return geo, pred
     surrogate_path()
     torch.cuda.synchronize()
-    t0 = time.time()
-    geo, pred = surrogate_path()
-    t_sur = time.time() - t0
+    ts = []
+    for _ in range(5):
+        t0 = time.time()
+        geo, pred = surrogate_path()
+        ts.append(time.time() - t0)
+    t_sur = float(np.median(ts))
     same = np.array_equal(np.asarray(geo["throats"]), np.asarray(net["throats"]))
+    same_geo = same and all(np.array_equal(np.asarray(geo[k]).view(np.uint64),
+                                           np.asarray(net[k]).view(np.uint64))
+                            for k in ("throat_area", "throat_center"))
     pred = pred.cpu().numpy()
     err = S.median_rel_error(pred[ok], logg[ok]) if same and ok.any() else None
     c = float(np.load(wpath)["geometric_c"])
@@ -273,18 +281,19 @@ def bench_surrogate(ctx, size=256, seed=9999):
     # inference alone (the fused kernel: patches gathered from the SDF inside it), timed with
     # HIP events around its launches on the context's stream
     nt = len(pred)
-    centers, raw = S.throat_inputs(geo, sdf.shape)
-    sc = model.standardize(raw)
-    model.predict_sdf(s_dev, sdf.shape, centers, sc)
+    _, cen, raw = S.throat_geometry(ctx, s_dev, sdf.shape)
+    model.predict_device(s_dev, sdf.shape, cen, raw)
     ctx.profile(True)
-    ctx.profile_only(["k_sur_infer"])
+    ctx.profile_only(None)
     ctx.profile_read()
     for _ in range(5):
-        model.predict_sdf(s_dev, sdf.shape, centers, sc)
+        S.throat_geometry(ctx, s_dev, sdf.shape)
+        model.predict_device(s_dev, sdf.shape, cen, raw)
     torch.cuda.synchronize()
     prof = ctx.profile_read()
     ctx.profile(False)
     t_inf = prof["k_sur_infer"][0] / prof["k_sur_infer"][1] / 1e3
+    t_geo = sum(v[0] for k, v in prof.items() if k != "k_sur_infer") / 5 / 1e3
     conv_flops = sum(2 * co * 27 * ci * d ** 3 for d, ci, co in ((16, 1, 16), (8, 16, 32),
                                                                    (4, 32, 32)))
     tflops = conv_flops * nt / t_inf / 1e12
@@ -293,10 +302,11 @@ def bench_surrogate(ctx, size=256, seed=9999):
             "throats": int(nt), "throats_with_q_dp_positive": int(ok.sum()),
             "plain_s": round(t_plain, 3), "stokes_s": round(t_stokes, 3),
             "surrogate_s": round(t_sur, 4),
+            "surrogate_device_s": {"geometry": round(t_geo, 5), "inference": round(t_inf, 5)},
             "speedup_vs_plain": round(t_plain / t_sur, 1),
             "speedup_vs_bookkeeping_only": round((t_plain - t_stokes) / t_sur, 2),
             "median_rel_error_g": err, "median_rel_error_g_geometric_A2_over_L": err_geo,
-            "same_throats": bool(same),
+            "same_throats": bool(same), "same_areas_and_centroids": bool(same_geo),
             "inference": {"seconds": round(t_inf, 5), "throats_per_s": round(nt / t_inf),
                           "roofline": {"bound": "mfma", "achieved": round(tflops, 3),
                                        "peak": 2500.0, "unit": "TFLOP/s",
@@ -384,7 +394,7 @@ def pmc_summary(root, copy_bytes, voxels, workload="extract_pore_network"):
     return "\n".join(out)
 
 
-def sq_summary(root, voxels):
+def sq_summary(root, voxels, workload="extract_pore_network"):
     """Markdown summary of two rocprofv3 --pmc runs of SQ counters under `root` (pmc_sq_a:
     SQ_WAVES, SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_INSTS_LDS, SQ_INSTS_VMEM_RD, SQ_INSTS_VMEM_WR,
     SQ_WAVE_CYCLES, SQ_BUSY_CYCLES; pmc_sq_b: SQ_WAIT_ANY, SQ_WAIT_INST_ANY, SQ_ACTIVE_INST_VALU,
@@ -469,6 +479,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched-segmentation section (profiling runs)")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="only the surrogate section (profiling runs)")
     ap.add_argument("--flow-only", action="store_true",
                     help="run only the extract_network_flow section (profiling)")
     ap.add_argument("--flow-stokes-iters", type=int, default=40000,
@@ -496,7 +510,7 @@ def main():
         print(pmc_summary(args.pmc_summary, args.copy_bytes, args.size ** 3, args.pmc_workload))
         return
     if args.sq_summary:
-        print(sq_summary(args.sq_summary, args.size ** 3))
+        print(sq_summary(args.sq_summary, args.size ** 3, args.pmc_workload))
         return
     import torch
 
@@ -517,7 +531,18 @@ def main():
               flush=True)
         ctx.close()
         return
+    if args.surrogate_only:
+        ctx = capi.Context(0)
+        print(json.dumps({"surrogate": bench_surrogate(ctx)}), flush=True)
+        ctx.close()
+        return
     sdf, desc = make_sdf(args.workload, args.size, args.seed)
+    if args.batched_only:
+        ctx = capi.Context(0)
+        src = torch.from_numpy(sdf).to("cuda:0")
+        print(json.dumps({"batched": bench_batched(ctx, src, reps=3)}), flush=True)
+        ctx.close()
+        return
     shape = sdf.shape
     n = sdf.size
     dims = (shape[2], shape[1], shape[0])

@@ -190,6 +190,21 @@ pnm_status pnm_extract_network_flow_d(pnm_ctx_t ctx, const float* sdf_d, const i
                                       const double* oz_d, const double grad_p[3],
                                       pnm_flow_result_t* out);
 
+/* Throat geometry of extract_network_flow without flow fields (project extension: the reference
+ * computes it inside extract_network_flow_k, pore_extraction.hpp:655-944, and has no call for it
+ * alone). Returns the pores, throats, throat_area and throat_center that
+ * pnm_extract_network_flow_d returns under binary openness (a face is open iff both of its cells
+ * have sdf > 0, as pnm_stokes_solve_d writes it); pore_pressure, pore_residual, throat_flow and
+ * throat_dp stay NULL. seg_d (nullable): a segmentation of this SDF (pnm_segment_volume_d, or
+ * one volume of pnm_segment_batch_d); NULL segments here. centers_d / raw_d (both or neither,
+ * nullable): device float[cap][3] / float[cap][4] receiving each throat's surrogate inputs when
+ * nt <= cap: the interface centroid in voxel units (X,Y,Z) and log area, log radius of each
+ * pore, log min-image separation of the pore centres. Release with pnm_flow_result_free. */
+pnm_status pnm_throat_geometry_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t* seg_d,
+                                 const int32_t dims[3], const float origin[3],
+                                 const float spacing[3], float* centers_d, float* raw_d,
+                                 int64_t cap, pnm_flow_result_t* out);
+
 /* ---- batched segmentation (project extension; no single-call counterpart in the reference:
  * the result equals nvol separate segment_volume calls, pore_extraction.hpp:389-397, each volume
  * its own periodic box numbered on its own) ---- */

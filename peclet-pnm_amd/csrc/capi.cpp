@@ -575,6 +575,31 @@ pnm_status pnm_extract_network_flow_d(pnm_ctx_t ctx, const float* sdf_d, const i
   });
 }
 
+pnm_status pnm_throat_geometry_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t* seg_d,
+                                 const int32_t dims[3], const float origin[3],
+                                 const float spacing[3], float* centers_d, float* raw_d,
+                                 int64_t cap, pnm_flow_result_t* out) {
+  return guarded([&] {
+    const Vol v = check_dims(dims);
+    need(origin && spacing && out, "origin, spacing and out are required");
+    need(v.n == 0 || sdf_d, "sdf must not be NULL");
+    need((centers_d == nullptr) == (raw_d == nullptr) && cap >= 0,
+         "pass both feature buffers (centers, raw) or neither, with cap >= 0");
+    Ctx* c = check_ctx(ctx);
+    std::memset(out, 0, sizeof(*out));
+    if (v.n == 0) return;
+    CallScope cs(c);
+    GeoMode geo;
+    geo.seg = seg_d;
+    geo.centers = centers_d;
+    geo.raw = raw_d;
+    geo.cap = cap;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    flow_device(c, sdf_d, v, origin, spacing, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, nullptr, zero, out, &geo);
+  });
+}
+
 pnm_status pnm_stokes_solve_d(pnm_ctx_t ctx, const float* sdf_d, const int32_t dims[3],
                               const float spacing[3], const double force[3], double mu,
                               double rtol, int32_t max_iter, double* u_d, double* v_d,

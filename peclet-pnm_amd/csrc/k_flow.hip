@@ -60,7 +60,20 @@ __global__ void k_flow_peaks(const float* __restrict__ sdf, const int32_t* __res
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npeaks) return;
   const uint32_t r = peaks[j];
-  if (sdf[r] > 0.0f) peak[seg[r] - 1] = (int32_t)r;
+  const int32_t l = seg[r];
+  if (sdf[r] > 0.0f && l > 0) peak[l - 1] = (int32_t)r;
+}
+
+// Largest label of a given segmentation (its pore count) into *mx (starts at 0).
+__global__ __launch_bounds__(256) void k_label_max(const int32_t* __restrict__ seg, int64_t n,
+                                                   int* __restrict__ mx) {
+  int m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, seg[i]);
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1) m = max(m, __shfl_xor(m, sh, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(mx, m);
 }
 
 // nf::pore_pressure (:582-613): grid coordinate in FP32, then FP64 trilinear weights in the
@@ -129,6 +142,9 @@ constexpr int kResCap = 2 * kFaceCap;            // staged residual records per 
 // faces, every interface face and every basin boundary face touching a pore (one residual record
 // per pore side, (pore index << 32 | face)); compacting the blocks in block order gives lists in
 // ascending face order, so the records need only a stable sort on their key bits.
+// kGeo: throat geometry without fields: the openness is the binary one of the project solver
+// (a face is open iff both of its cells have sdf > 0), and no residual records are staged.
+template <bool kGeo>
 __global__ __launch_bounds__(kFaceBlock) void k_face_init(
     const float* __restrict__ sdf, const int32_t* __restrict__ lab, Geo g, int64_t n,
     const double* __restrict__ ox, const double* __restrict__ oy, const double* __restrict__ oz,
@@ -152,15 +168,17 @@ __global__ __launch_bounds__(kFaceBlock) void k_face_init(
       const int64_t f = 3 * c64 + d;
       const int nb = nb_cell(x, y, z, d, g);
       const int b = lab[nb];
-      const double opn = ox ? (d == 0 ? ox[nb] : (d == 1 ? oy[nb] : oz[nb])) : 1.0;
+      const float snb = sdf[nb];
+      const double opn = kGeo ? ((sc > 0.0f && snb > 0.0f) ? 1.0 : 0.0)
+                              : (ox ? (d == 0 ? ox[nb] : (d == 1 ? oy[nb] : oz[nb])) : 1.0);
       const bool itf = (a != b && a > 0 && b > 0 && opn > 0.0);
-      const bool is_core = itf && sc > 0.0f && sdf[nb] > 0.0f;
+      const bool is_core = itf && sc > 0.0f && snb > 0.0f;
       par[f] = is_core ? f : -1;
       fpar[f] = itf ? (is_core ? f : kSent) : -1;
       fpair[f] = itf ? (((int64_t)(a < b ? a : b) << 32) | (int64_t)(a < b ? b : a)) : -1;
       core_m |= (unsigned)is_core << d;
       film_m |= (unsigned)(itf && !is_core) << d;
-      if (a != b) {
+      if (!kGeo && a != b) {
         if (a > 0) {
           rk0[d] = a - 1;
           if (b > 0) rk1[d] = b - 1;
@@ -416,6 +434,8 @@ struct FluxArgs {
 };
 
 // q of face f (throat_flux :889-897): the shared face is the -d face of the +d neighbour.
+// kGeo: no velocity field (q = 0); only the open area w0 is wanted.
+template <bool kGeo = false>
 __device__ __forceinline__ double face_q(const FluxArgs& A, int64_t f, int& a, int& b,
                                          double& w0, int& x, int& y, int& z, int& d) {
   const int c = (int)(f / 3);
@@ -424,7 +444,7 @@ __device__ __forceinline__ double face_q(const FluxArgs& A, int64_t f, int& a, i
   const int nb = nb_cell(x, y, z, d, A.g);
   a = A.lab[c];
   b = A.lab[nb];
-  const double vel = d == 0 ? A.u[nb] : (d == 1 ? A.v[nb] : A.w[nb]);
+  const double vel = kGeo ? 0.0 : (d == 0 ? A.u[nb] : (d == 1 ? A.v[nb] : A.w[nb]));
   const double opn = A.ox ? (d == 0 ? A.ox[nb] : (d == 1 ? A.oy[nb] : A.oz[nb])) : 1.0;
   const double area = d == 0 ? A.Ax : (d == 1 ? A.Ay : A.Az);
   w0 = opn * area;
@@ -448,7 +468,8 @@ __global__ __launch_bounds__(256) void k_resid_terms(const uint64_t* __restrict_
 }
 
 // Throat (:903-939): the oriented q, the open area, and the area-weighted face centre relative
-// to the lower pore's peak-voxel centre, min-imaged per axis.
+// to the lower pore's peak-voxel centre, min-imaged per axis. kGeo: no q (tq is not written).
+template <bool kGeo>
 __global__ __launch_bounds__(256) void k_throat_terms(
     const uint64_t* __restrict__ rec, int64_t cnt, FluxArgs A, const int32_t* __restrict__ peak,
     float oxo, float oyo, float ozo, float sx, float sy, float sz, double* __restrict__ tq,
@@ -459,7 +480,7 @@ __global__ __launch_bounds__(256) void k_throat_terms(
   const int64_t f = (int64_t)(rec[i] & 0xffffffffull);
   int a, b, ixx, iyy, izz, d;
   double w0;
-  const double q = face_q(A, f, a, b, w0, ixx, iyy, izz, d);
+  const double q = face_q<kGeo>(A, f, a, b, w0, ixx, iyy, izz, d);
   const int lo = a < b ? a : b;
   const double Lw[3] = {(double)A.g.nx * sx, (double)A.g.ny * sy, (double)A.g.nz * sz};
   const int pk = peak[lo - 1];
@@ -474,7 +495,7 @@ __global__ __launch_bounds__(256) void k_throat_terms(
     dv -= Lw[a2] * round(dv / Lw[a2]);
     fp[a2] = dv;
   }
-  tq[i] = a < b ? q : -q;
+  if (!kGeo) tq[i] = a < b ? q : -q;
   ta[i] = w0;
   tcx[i] = w0 * fp[0];
   tcy[i] = w0 * fp[1];
@@ -569,12 +590,42 @@ __global__ void k_throat_dp(const int32_t* __restrict__ throats, int64_t nt,
     const double ancj = o[a] + pkj3[a] * (double)s[a];
     const double ct = (Cx[t] * (a == 0) + Cy[t] * (a == 1) + Cz[t] * (a == 2)) / aw;
     tc[3 * t + a] = anc + ct;  // interface centroid: anchor + min-image offset
+    if (!pp) continue;  // throat geometry only
     const long long Dc = (long long)round((ancj - anc) / (double)s[a]) -
                          (long long)round(ct / (double)s[a] + 1e-6);
     const long long k = (long long)round((double)Dc / N[a]);
     macro += gp[a] * ((pjp[a] - pip[a]) - L[a] * (double)k);
   }
-  dp[t] = (pp[li - 1] - pp[lj - 1]) - macro;
+  if (pp) dp[t] = (pp[li - 1] - pp[lj - 1]) - macro;
+}
+
+// Per-throat inputs of the conductance surrogate (the host statement is throat_inputs in
+// surrogate.py): the interface centroid in voxel coordinates (X, Y, Z) and the raw scalars
+// log area, log radius of each pore, log min-image separation of the pore centres, computed in
+// FP64 and rounded to FP32.
+__global__ void k_throat_features(const int32_t* __restrict__ throats, int64_t nt,
+                                  const pnm_pore_t* __restrict__ pores,
+                                  const double* __restrict__ Ar, const double* __restrict__ tc,
+                                  Geo g, float ox, float oy, float oz, float sx, float sy,
+                                  float sz, float* __restrict__ centers,
+                                  float* __restrict__ raw) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const pnm_pore_t pi = pores[throats[2 * t] - 1], pj = pores[throats[2 * t + 1] - 1];
+  const double o[3] = {ox, oy, oz}, s[3] = {sx, sy, sz};
+  const double L[3] = {(double)g.nx * s[0], (double)g.ny * s[1], (double)g.nz * s[2]};
+  const double dpos[3] = {(double)pj.x - (double)pi.x, (double)pj.y - (double)pi.y,
+                          (double)pj.z - (double)pi.z};
+  double q = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double d = dpos[a] - L[a] * rint(dpos[a] / L[a]);  // numpy round: half to even
+    q += d * d;
+    centers[3 * t + a] = (float)((tc[3 * t + a] - o[a]) / s[a]);
+  }
+  raw[4 * t + 0] = (float)log(fmax(Ar[t], 1e-12));
+  raw[4 * t + 1] = (float)log(fmax((double)pi.radius, 1e-6));
+  raw[4 * t + 2] = (float)log(fmax((double)pj.radius, 1e-6));
+  raw[4 * t + 3] = (float)log(fmax(sqrt(q), 1e-6));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -600,39 +651,77 @@ int64_t face_ccl(Ctx* c, const uint32_t* list, int64_t cnt, Geo g, const int64_t
 void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                  const double* u, const double* vv, const double* w, const double* p,
                  const double* ox, const double* oy, const double* oz, const double grad_p[3],
-                 pnm_flow_result_t* out) {
+                 pnm_flow_result_t* out, const GeoMode* geo) {
   const int64_t n = v.n;
   const Geo g{v.nx, v.ny, v.nz};
+  const bool kgeo = geo != nullptr;
   if (3 * n > (int64_t)0xffffffffLL)
     fail(PNM_EINVAL, "extract_network_flow: 3*Nx*Ny*Nz must fit in 32 bits (face ids)");
-  int32_t* seg = c->arena.get_n<int32_t>((size_t)n);
-  SegPeaks sp = segment_with_peaks(c, sdf, v, spc, seg, nullptr);
-  const int64_t np = sp.n_pores;
+  uint64_t* ctr = c->arena.get_n<uint64_t>(8);
+  fill_u64(c, ctr, 0, 8);
+  unsigned long long* uctr = reinterpret_cast<unsigned long long*>(ctr);
+  const int32_t* seg = nullptr;
+  const uint32_t* peaks = nullptr;
+  int64_t np = 0, npeaks = 0;
+  if (kgeo && geo->seg) {
+    // a given segmentation: its largest label and the peak list of the extract_pores predicate
+    seg = geo->seg;
+    const int64_t nb = pre_blocks(v), scap = pre_stage_cap();
+    uint32_t* stage = c->arena.get_n<uint32_t>((size_t)(nb * scap));
+    uint32_t* cnt = c->arena.get_n<uint32_t>((size_t)nb);
+    uint32_t* pk = c->arena.get_n<uint32_t>((size_t)n);
+    PreOut o{};
+    o.peak_stage = stage;
+    o.peak_cnt = cnt;
+    launch_pre(c, sdf, v, 0.0f, o);
+    compact_staged(c, stage, cnt, nb, scap, pk, ctr + 1);
+    PNM_LAUNCH(c, "k_label_max", k_label_max, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 4096)),
+               dim3(256), 0, seg, n, reinterpret_cast<int*>(ctr));
+    uint64_t h2[2];
+    readback(c, ctr, 2, h2);
+    np = (int64_t)(int32_t)(uint32_t)h2[0];
+    peaks = pk;
+    npeaks = (int64_t)h2[1];
+    fill_u64(c, ctr, 0, 2);
+  } else {
+    int32_t* sg = c->arena.get_n<int32_t>((size_t)n);
+    const SegPeaks sp = segment_with_peaks(c, sdf, v, spc, sg, nullptr);
+    seg = sg;
+    np = sp.n_pores;
+    peaks = sp.peaks;
+    npeaks = sp.npeaks;
+  }
   if (np <= 0) {  // np == 0 returns empty in the reference; np < 0 cannot happen here
     PNM_HIP(hipStreamSynchronize(c->stream));
     return;
   }
-  uint64_t* ctr = c->arena.get_n<uint64_t>(8);
-  fill_u64(c, ctr, 0, 8);
-  unsigned long long* uctr = reinterpret_cast<unsigned long long*>(ctr);
 
   // peaks and centres
   int32_t* peak = c->arena.get_n<int32_t>((size_t)np);
   fill_u32(c, reinterpret_cast<uint32_t*>(peak), 0u, np);
-  PNM_LAUNCH(c, "k_flow_peaks", k_flow_peaks, dim3(cdiv(std::max<int64_t>(sp.npeaks, 1), 256)),
-             dim3(256), 0, sdf, seg, sp.peaks, sp.npeaks, peak);
+  PNM_LAUNCH(c, "k_flow_peaks", k_flow_peaks, dim3(cdiv(std::max<int64_t>(npeaks, 1), 256)),
+             dim3(256), 0, sdf, seg, peaks, npeaks, peak);
   pnm_pore_t* pores = c->arena.get_n<pnm_pore_t>((size_t)np);
   launch_centroids(c, sdf, v, org, spc, reinterpret_cast<const uint32_t*>(peak), np, pores);
-  double* pp = c->arena.get_n<double>((size_t)np);
-  PNM_LAUNCH(c, "k_pore_pressure", k_pore_pressure, dim3(cdiv(np, 256)), dim3(256), 0, pores, np,
-             g, org[0], org[1], org[2], spc[0], spc[1], spc[2], p, pp);
+  double* pp = nullptr;
+  if (!kgeo) {
+    pp = c->arena.get_n<double>((size_t)np);
+    PNM_LAUNCH(c, "k_pore_pressure", k_pore_pressure, dim3(cdiv(np, 256)), dim3(256), 0, pores,
+               np, g, org[0], org[1], org[2], spc[0], spc[1], spc[2], p, pp);
+  }
 
-  // flow basins
-  int32_t* next_all = c->arena.get_n<int32_t>((size_t)n);
-  int32_t* lab = c->arena.get_n<int32_t>((size_t)n);
-  launch_next(c, sdf, v, next_all, kWalkAll);
-  PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256)), dim3(256), 0, n, next_all, seg,
-             lab);
+  // flow basins. Geometry only: every open face joins two sdf>0 cells, and the basin of an
+  // sdf>0 cell is its segmentation label (both are the label at the end of its ascent), so the
+  // segmentation stands for the basins.
+  const int32_t* lab = seg;
+  if (!kgeo) {
+    int32_t* next_all = c->arena.get_n<int32_t>((size_t)n);
+    int32_t* fl = c->arena.get_n<int32_t>((size_t)n);
+    launch_next(c, sdf, v, next_all, kWalkAll);
+    PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256)), dim3(256), 0, n, next_all, seg,
+               fl);
+    lab = fl;
+  }
 
   // faces: labels, pair keys and the face-ordered lists (core, film, all interface faces,
   // residual records), staged per block and compacted in block order
@@ -644,21 +733,27 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   uint32_t* st_core = c->arena.get_n<uint32_t>((size_t)(nbf * kFaceCap));
   uint32_t* st_film = c->arena.get_n<uint32_t>((size_t)(nbf * kFaceCap));
   uint32_t* st_itf = c->arena.get_n<uint32_t>((size_t)(nbf * kFaceCap));
-  uint64_t* st_res = c->arena.get_n<uint64_t>((size_t)(nbf * kResCap));
+  uint64_t* st_res = kgeo ? nullptr : c->arena.get_n<uint64_t>((size_t)(nbf * kResCap));
   uint32_t* cnt3 = c->arena.get_n<uint32_t>((size_t)(3 * nbf));
   uint32_t* cnt_res = c->arena.get_n<uint32_t>((size_t)nbf);
-  PNM_LAUNCH(c, "k_face_init", k_face_init, dim3((unsigned)nbf), dim3(kFaceBlock), 0, sdf, lab,
-             g, n, ox, oy, oz, fpar, fpair, par, st_core, st_film, st_itf, cnt3, st_res, cnt_res);
+  if (kgeo)
+    PNM_LAUNCH(c, "k_face_init", k_face_init<true>, dim3((unsigned)nbf), dim3(kFaceBlock), 0,
+               sdf, lab, g, n, nullptr, nullptr, nullptr, fpar, fpair, par, st_core, st_film,
+               st_itf, cnt3, st_res, cnt_res);
+  else
+    PNM_LAUNCH(c, "k_face_init", k_face_init<false>, dim3((unsigned)nbf), dim3(kFaceBlock), 0,
+               sdf, lab, g, n, ox, oy, oz, fpar, fpair, par, st_core, st_film, st_itf, cnt3,
+               st_res, cnt_res);
   uint32_t* core_list = c->arena.get_n<uint32_t>((size_t)nf);
   uint32_t* film_list = c->arena.get_n<uint32_t>((size_t)nf);
   uint32_t* itf_list = c->arena.get_n<uint32_t>((size_t)nf);
-  uint64_t* res_rec = c->arena.get_n<uint64_t>((size_t)(2 * nf));
+  uint64_t* res_rec = kgeo ? nullptr : c->arena.get_n<uint64_t>((size_t)(2 * nf));
   {
     const uint32_t* st[3] = {st_core, st_film, st_itf};
     uint32_t* outs[3] = {core_list, film_list, itf_list};
     uint64_t* tots[3] = {ctr + 0, ctr + 1, ctr + 6};
     compact_staged_multi(c, 3, st, cnt3, nbf, kFaceCap, outs, tots);
-    compact_staged64(c, st_res, cnt_res, nbf, kResCap, res_rec, ctr + 2);
+    if (!kgeo) compact_staged64(c, st_res, cnt_res, nbf, kResCap, res_rec, ctr + 2);
   }
   uint64_t h[8];
   readback(c, ctr, 7, h);
@@ -711,16 +806,16 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   readback(c, ctr + 5, 1, h);
   const int64_t nt = (int64_t)h[0];
   int32_t* throats = c->arena.get_n<int32_t>((size_t)std::max<int64_t>(2 * nt, 2));
-  double* Q = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* Q = kgeo ? nullptr : c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* Ar = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* Cx = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* Cy = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* Cz = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
-  double* dp = c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
+  double* dp = kgeo ? nullptr : c->arena.get_n<double>((size_t)std::max<int64_t>(nt, 1));
   double* tc = c->arena.get_n<double>((size_t)std::max<int64_t>(3 * nt, 1));
-  double* resid = c->arena.get_n<double>((size_t)np);
-  FluxArgs fa{g, lab, u, vv, w, ox, oy, oz, (double)spc[1] * spc[2], (double)spc[0] * spc[2],
-              (double)spc[0] * spc[1]};
+  double* resid = kgeo ? nullptr : c->arena.get_n<double>((size_t)np);
+  FluxArgs fa{g, lab, u, vv, w, kgeo ? nullptr : ox, kgeo ? nullptr : oy, kgeo ? nullptr : oz,
+              (double)spc[1] * spc[2], (double)spc[0] * spc[2], (double)spc[0] * spc[1]};
   if (nt > 0) {
     uint32_t* order = c->arena.get_n<uint32_t>((size_t)nt);
     uint64_t* key = c->arena.get_n<uint64_t>((size_t)nt);
@@ -751,19 +846,46 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
     PNM_LAUNCH(c, "k_seg_bounds", k_seg_bounds, dim3(cdiv(nitf, 256)), dim3(256), 0, trec, nitf,
                ts, te);
     double* terms = c->arena.get_n<double>((size_t)(5 * nitf));
-    PNM_LAUNCH(c, "k_throat_terms", k_throat_terms, dim3(cdiv(nitf, 256)), dim3(256), 0, trec,
-               nitf, fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], terms,
-               terms + nitf, terms + 2 * nitf, terms + 3 * nitf, terms + 4 * nitf);
-    TermSets<5> T{};
-    double* outs[5] = {Q, Ar, Cx, Cy, Cz};
-    for (int k = 0; k < 5; ++k) {
-      T.in[k] = terms + k * nitf;
-      T.out[k] = outs[k];
+    if (kgeo) {
+      PNM_LAUNCH(c, "k_throat_terms", k_throat_terms<true>, dim3(cdiv(nitf, 256)), dim3(256), 0,
+                 trec, nitf, fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], nullptr,
+                 terms + nitf, terms + 2 * nitf, terms + 3 * nitf, terms + 4 * nitf);
+      TermSets<4> T{};
+      double* outs[4] = {Ar, Cx, Cy, Cz};
+      for (int k = 0; k < 4; ++k) {
+        T.in[k] = terms + (k + 1) * nitf;
+        T.out[k] = outs[k];
+      }
+      PNM_LAUNCH(c, "k_key_sums", k_key_sums<4>, dim3(cdiv(nt, 4)), dim3(256), 0, ts, te, nt, T);
+    } else {
+      PNM_LAUNCH(c, "k_throat_terms", k_throat_terms<false>, dim3(cdiv(nitf, 256)), dim3(256), 0,
+                 trec, nitf, fa, peak, org[0], org[1], org[2], spc[0], spc[1], spc[2], terms,
+                 terms + nitf, terms + 2 * nitf, terms + 3 * nitf, terms + 4 * nitf);
+      TermSets<5> T{};
+      double* outs[5] = {Q, Ar, Cx, Cy, Cz};
+      for (int k = 0; k < 5; ++k) {
+        T.in[k] = terms + k * nitf;
+        T.out[k] = outs[k];
+      }
+      PNM_LAUNCH(c, "k_key_sums", k_key_sums<5>, dim3(cdiv(nt, 4)), dim3(256), 0, ts, te, nt, T);
     }
-    PNM_LAUNCH(c, "k_key_sums", k_key_sums<5>, dim3(cdiv(nt, 4)), dim3(256), 0, ts, te, nt, T);
     PNM_LAUNCH(c, "k_throat_dp", k_throat_dp, dim3(cdiv(nt, 256)), dim3(256), 0, throats, nt,
                pores, pp, peak, Ar, Cx, Cy, Cz, g, org[0], org[1], org[2], spc[0], spc[1],
                spc[2], grad_p[0], grad_p[1], grad_p[2], dp, tc);
+    if (kgeo && geo->centers && geo->raw && nt <= geo->cap)
+      PNM_LAUNCH(c, "k_throat_features", k_throat_features, dim3(cdiv(nt, 256)), dim3(256), 0,
+                 throats, nt, pores, Ar, tc, g, org[0], org[1], org[2], spc[0], spc[1], spc[2],
+                 geo->centers, geo->raw);
+  }
+  if (kgeo) {
+    out->np = np;
+    out->nt = nt;
+    out->pores = host_array(c, pores, np);
+    out->throats = host_array(c, throats, 2 * nt);
+    out->throat_area = host_array(c, Ar, nt);
+    out->throat_center = host_array(c, tc, 3 * nt);
+    PNM_HIP(hipStreamSynchronize(c->stream));
+    return;
   }
   // residual: records in face order, stably sorted by pore
   {

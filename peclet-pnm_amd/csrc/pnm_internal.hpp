@@ -345,11 +345,22 @@ void segment_batch_device(Ctx* c, int nvol, const float* const* sdf, int32_t* co
                           int32_t* out_ns);
 
 // ---------- network flow (k_flow.hip) ----------
+// Throat geometry without fields (binary openness from the SDF: a face is open iff both cells
+// have sdf > 0): pores, throats, areas and centroids only, equal to extract_network_flow's under
+// that openness.
+struct GeoMode {
+  const int32_t* seg = nullptr;  // a given segmentation of the SDF (nullable: segment here)
+  float* centers = nullptr;      // nullable: per-throat surrogate inputs when nt <= cap,
+  float* raw = nullptr;          //   centroids [nt][3] (voxel X,Y,Z) and raw scalars [nt][4]
+  int64_t cap = 0;
+};
 // Fields are device pointers; ox/oy/oz all null = fully open. Results are malloc'ed host arrays.
+// geo != nullptr: throat geometry only (fields unused; the flow, dp, pressure and residual
+// arrays of `out` stay null).
 void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                  const double* u, const double* vv, const double* w, const double* p,
                  const double* ox, const double* oy, const double* oz, const double grad_p[3],
-                 pnm_flow_result_t* out);
+                 pnm_flow_result_t* out, const GeoMode* geo = nullptr);
 
 // ---------- project MAC Stokes solver (k_stokes.hip) ----------
 struct StokesInfo {

@@ -30,7 +30,7 @@ EXPORTS = (
     "pnm_stage_labels_d", "pnm_stage_roots_d", "pnm_segment_batch_d", "pnm_batch_config",
     "pnm_stokes_solve_d", "pnm_sur_param_count", "pnm_sur_init", "pnm_sur_forward_d",
     "pnm_sur_backward_d", "pnm_sur_loss_d", "pnm_sur_adam_d", "pnm_sur_patches_d",
-    "pnm_segment_volume_noalloc_d", "pnm_sur_infer_d",
+    "pnm_segment_volume_noalloc_d", "pnm_sur_infer_d", "pnm_throat_geometry_d",
 )
 
 # Largest tile-padded volume the no-allocation entry point accepts (PNM_NOALLOC_MAX_PADDED).
@@ -117,6 +117,9 @@ def lib():
                                                  vp, ctypes.POINTER(FlowResult)]
         L.pnm_extract_network_flow.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                                vp, ctypes.POINTER(FlowResult)]
+        L.pnm_throat_geometry_d.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64,
+                                            ctypes.POINTER(FlowResult)]
+        L.pnm_throat_geometry_d.restype = st
         L.pnm_flow_result_free.argtypes = [ctypes.POINTER(FlowResult)]
         L.pnm_flow_result_free.restype = None
         L.pnm_segment_batch_d.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp, vp, vp,
@@ -298,6 +301,26 @@ class Context:
             self.h, _ptr(sdf), _i3(dims_xyz), _f3(origin_xyz), _f3(spacing_xyz), _ptr(u), _ptr(v),
             _ptr(w), _ptr(p), _ptr(ox), _ptr(oy), _ptr(oz), gp, ctypes.byref(r)))
         return _flow_to_numpy(r)
+
+    def throat_geometry_d(self, sdf, dims_xyz, origin_xyz, spacing_xyz, seg=None, centers=None,
+                          raw=None):
+        """Pores, throats, throat_area and throat_center of extract_network_flow under binary
+        openness, without fields. seg: optional device int32 segmentation of this SDF.
+        centers / raw: optional device float32 [cap, 3] / [cap, 4] receiving the surrogate
+        inputs when the throat count fits."""
+        r = FlowResult()
+        cap = 0
+        if centers is not None or raw is not None:
+            if centers is None or raw is None:
+                raise ValueError("pass both centers and raw, or neither")
+            cap = min(centers.numel() // 3, raw.numel() // 4)
+        _check(lib().pnm_throat_geometry_d(
+            self.h, _ptr(sdf), _ptr(seg), _i3(dims_xyz), _f3(origin_xyz), _f3(spacing_xyz),
+            _ptr(centers), _ptr(raw), ctypes.c_int64(cap), ctypes.byref(r)))
+        out = _flow_to_numpy(r)
+        for k in ("pore_pressure", "pore_residual", "throat_flow", "throat_dp"):
+            del out[k]
+        return out
 
 
     def stokes_solve_d(self, sdf, dims_xyz, spacing_xyz, force_xyz, u, v, w, p, ox=None, oy=None,

@@ -8,8 +8,10 @@ Per throat the inputs are
   - a 16^3 SDF patch centred on the interface centroid (throat_center), scaled by 1/8;
   - 4 scalars: log throat area, log radius of each pore (SDF at its peak), log min-image pore
     separation, standardised with the training set's mean and spread.
-The "throat geometry" (throats, areas, centroids, pore positions and radii) comes from the same
-bookkeeping with zero fields, so inference needs only the SDF.
+The throat geometry (throats, areas, centroids, pore positions and radii) comes from
+pnm_throat_geometry_d: the segmentation (computed there, or one volume of a batched segmentation)
+and the interface faces under the solver's binary openness, with no fields, so inference needs
+only the SDF. Its throats are exactly those extract_network_flow returns on the solved fields.
 """
 from __future__ import annotations
 
@@ -48,6 +50,26 @@ def throat_inputs(net, shape_zyx, spacing_zyx=(1.0, 1.0, 1.0), origin_zyx=(0.0, 
                     np.log(np.maximum(rad[th[:, 1] - 1], 1e-6)),
                     np.log(np.maximum(sep, 1e-6))], axis=1)
     return centers.astype(np.float32), raw.astype(np.float32)
+
+
+def throat_geometry(ctx, sdf_dev, shape_zyx, seg=None):
+    """Throat geometry of a device SDF (zyx, unit spacing, origin 0) without fields, with the
+    surrogate inputs computed on the device: (geometry dict, centers [nt,3] and raw scalars
+    [nt,4] as device float32 tensors). seg: optional device segmentation of the same SDF (e.g.
+    one volume of a batched segmentation)."""
+    torch = _torch()
+    n = int(np.prod(shape_zyx))
+    cap = max(4096, n // 256)
+    centers = torch.empty(3 * cap, dtype=torch.float32, device="cuda")
+    raw = torch.empty(4 * cap, dtype=torch.float32, device="cuda")
+    dims = tuple(shape_zyx)[::-1]
+    geo = ctx.throat_geometry_d(sdf_dev, dims, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), seg=seg,
+                                centers=centers, raw=raw)
+    nt = len(geo["throats"])
+    if nt > cap:  # more throats than the buffers hold: the same inputs from the host arrays
+        c, r = throat_inputs(geo, shape_zyx)
+        return geo, torch.from_numpy(c).cuda(), torch.from_numpy(r).cuda()
+    return geo, centers[: 3 * nt].view(nt, 3), raw[: 4 * nt].view(nt, 4)
 
 
 def conductance_targets(net):
@@ -132,6 +154,20 @@ class ThroatSurrogate:
             s = torch.from_numpy(np.ascontiguousarray(scalars_std, dtype=np.float32)).cuda()
             self.ctx.sur_infer_d(sdf_dev, tuple(shape_zyx)[::-1], c, s, N, PATCH_SCALE,
                                  self.params, out)
+        return out[:N] + self.t_mean
+
+    def predict_device(self, sdf_dev, shape_zyx, centers_d, raw_d):
+        """log g from device inputs (throat_geometry's centers and raw scalars): standardised on
+        the device, then the fused kernel. Returns a device tensor."""
+        torch = _torch()
+        N = centers_d.shape[0]
+        out = torch.empty(max(N, 1), device="cuda")
+        if N:
+            mu = torch.from_numpy(self.mu).cuda()
+            sd = torch.from_numpy(self.sd).cuda()
+            s = ((raw_d - mu) / sd).contiguous()
+            self.ctx.sur_infer_d(sdf_dev, tuple(shape_zyx)[::-1], centers_d.contiguous(), s, N,
+                                 PATCH_SCALE, self.params, out)
         return out[:N] + self.t_mean
 
     def train_step(self, patches, scalars_std, targets, lr, delta=0.25):

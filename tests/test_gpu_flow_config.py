@@ -8,7 +8,8 @@ body force along x and binary openness. At full size the checks are size-indepen
   closed, so every open boundary face belongs to a throat);
 - the residual itself sits at the solver tolerance relative to the largest throat flow;
 - the network's dissipation sum(Q * dp) is positive and every throat has g = Q/dp > 0;
-- two calls are bit-identical.
+- two calls are bit-identical;
+- the throat geometry without fields (pnm_throat_geometry_d) equals the flow path's.
 
 A 96^3 chamber lattice with Stokes fields and openness is compared with the oracle bit for bit.
 """
@@ -70,6 +71,13 @@ def test_chamber_lattice_256_stokes_openness(capi_ctx):
     print(f"[flow_config] {len(q)} throats, g > 0 on {(g > 0).mean():.4f}, "
           f"max residual / max Q {np.abs(r['pore_residual']).max() / qmax:.2e}", flush=True)
     assert (g > 0).all(), (g > 0).mean()
+    # the throat geometry without fields is the flow path's under this openness, bit for bit
+    geo = capi_ctx.throat_geometry_d(s, shape[::-1], (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    np.testing.assert_array_equal(geo["throats"], r["throats"])
+    np.testing.assert_array_equal(np.ascontiguousarray(geo["pores"]).view(np.uint32),
+                                  np.ascontiguousarray(r["pores"]).view(np.uint32))
+    for k in ("throat_area", "throat_center"):
+        np.testing.assert_array_equal(geo[k].view(np.uint64), r[k].view(np.uint64), err_msg=k)
 
 
 def test_chamber_lattice_96_stokes_vs_oracle(capi_ctx):
