@@ -16,6 +16,8 @@
 // all as one "NT" GEMM C[M][N] = A[M][K] . Bt[N][K]^T with K-contiguous BF16 operands.
 // Activations are channels-last: [batch][z][y][x][channel]. The dense head, pooling, loss and
 // Adam run in FP32.
+#include <cmath>
+
 #include "k_common.hpp"
 
 namespace pnm {
@@ -540,115 +542,210 @@ void surrogate_patches(Ctx* c, const float* sdf, Vol v, const float* centers, in
 
 // ---------------- fused inference ----------------
 // The whole network for one throat at a time in one workgroup (8 waves), persistent over the
-// throats: the 16^3 patch is gathered from the SDF (the next throat's patch is loaded while the
-// current one is computed), and every convolution is an implicit GEMM on the MFMA with both
-// operands in LDS — BF16 weights of all three layers (84 KB, loaded once per workgroup) and the
-// throat's activations with their zero halo — so nothing but the patch, the scalars and the
-// output touches HBM. Bias, ReLU and the 2x pooling are applied to the accumulators and the
-// pooled values rounded to BF16 into the next layer's input; the global average and the dense
-// head run on one wave. Rounding points, the K order of the MFMA chains and the pooling, global
-// average and head summation orders are those of the training forward (surrogate_forward at
-// the batch sizes where its GEMMs need no split over K), so both give the same log g up to the
-// order of the FP32 additions inside an MFMA instruction.
+// throats. Every convolution is an implicit GEMM on v_mfma_f32_16x16x32_bf16 (lane l holds
+// rows/columns l % 16 and K elements 8 (l / 16) .. + 7) with both operands in LDS:
+//   - weights (84 KB, converted once per workgroup) stored in MFMA fragment order, so a lane
+//     reads its 16 bytes of a K-step contiguously;
+//   - the throat's activations with their zero halo, channels-last in planes of 8 channels
+//     (A1: 2 planes of [10^3][8], A2: 4 planes of [6^3][8]), so the 8 lanes a 16-byte read
+//     serves at once cover 128 contiguous bytes.
+// The 16^3 patch of the next throat is fetched into registers while the current one is
+// computed and stored into LDS as soon as layer 0 has read the current one. Bias, ReLU and the
+// 2x pooling are applied to the accumulators and the pooled values rounded to BF16 into the
+// next layer's input; layer 2's global average is summed from its accumulators (lane shuffles,
+// one partial per z-slice) and the dense head runs on one wave. Rounding points are those of
+// the training forward (surrogate_forward); the FP32 sums differ from it only in order.
 struct SurOffs {
   int w0, b0, w1, b1, w2, b2, W4, b4, W5, b5;
 };
 
 constexpr int SI_THREADS = 512;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(p));
+}
 __device__ __forceinline__ bf16x4 lds4(const uint16_t* p) {
   return *reinterpret_cast<const bf16x4*>(p);
 }
-__device__ __forceinline__ float lane_f(float v, int j) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+// Sum over the wave by DPP: row_shr 1, 2, 4, 8 within each 16-lane row, then row_bcast 15 and
+// 31 across rows; lane 63 receives the total.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask,
+                                                        0xf, false));
 }
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v = dpp_add<0x111, 0xf>(v);  // row_shr:1
+  v = dpp_add<0x112, 0xf>(v);  // row_shr:2
+  v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+  v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+  v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+// FP32 -> BF16 bits with the gfx950 conversion instruction (round to nearest even, as f2bf for
+// every finite value)
+__device__ __forceinline__ uint16_t f2bf_hw(float f) {
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// offset of tap t (z-major 3x3x3) in a haloed grid of edge E, in voxels
+__host__ __device__ constexpr int tap_off(int t, int E) {
+  return ((t / 9) * E + (t / 3) % 3) * E + t % 3;
+}
+
+// One 8-channel plane of the haloed 10^3 layer-0 output and of the haloed 6^3 layer-1 output
+// (unpadded: 64 bytes of padding between planes measured 25% slower in layer 1).
+constexpr int kA1Plane = 1000 * 8;
+constexpr int kA2Plane = 216 * 8;
+// The patch with its zero halo: 18 slabs of 18 rows of 18 voxels, rows 19 and slabs 418 BF16
+// apart, so that the four lane groups' 16-voxel reads of the layer-0 gather (four different
+// taps) fall on different LDS banks.
+constexpr int kX0Row = 19, kX0Slab = 418;
 
 __global__ __launch_bounds__(SI_THREADS) void k_sur_infer(
     const float* __restrict__ sdf, int nx, int ny, int nz, const float* __restrict__ centers,
-    const float* __restrict__ scal, int nt, float scale, const float* __restrict__ P, SurOffs L,
-    float* __restrict__ out) {
-  __shared__ __align__(16) uint16_t W0s[16 * 32];
-  __shared__ __align__(16) uint16_t W1s[32 * 432];
-  __shared__ __align__(16) uint16_t W2s[32 * 864];
-  __shared__ float Bs[80];
-  __shared__ __align__(16) uint16_t X0[18 * 18 * 18];  // patch with its zero halo
-  __shared__ __align__(16) uint16_t A1[10 * 10 * 10 * 16];  // pooled layer 0, zero halo
-  __shared__ __align__(16) uint16_t A2[6 * 6 * 6 * 32];  // pooled layer 1, zero halo
-  __shared__ float G2[64 * 32];  // layer 2 after ReLU, [position][channel]
+    const float* __restrict__ scal, int nt, float scale, float inv_scale,
+    const float* __restrict__ P, SurOffs L, float* __restrict__ out) {
+  __shared__ __align__(16) uint16_t W0f[64 * 8];            // [lane][8]
+  __shared__ __align__(16) uint16_t W1f[2 * 13 * 64 * 8];   // [col tile][K-step][lane][8]
+  __shared__ __align__(16) uint16_t W1t[2 * 64 * 4];        // tap 26 (K = 16): [tile][lane][4]
+  __shared__ __align__(16) uint16_t W2f[2 * 27 * 64 * 8];   // [col tile][K-step][lane][8]
+  __shared__ __align__(16) uint16_t X0[18 * kX0Slab];
+  __shared__ __align__(16) uint16_t A1[2 * kA1Plane];
+  __shared__ __align__(16) uint16_t A2[4 * kA2Plane];
+  __shared__ __align__(16) float W4t[36 * 64];              // dense 36 -> 64, [in][out]
+  __shared__ float Bs[208];  // conv biases 0..79, dense bias 80..143, output weights 144..207
+  __shared__ float part[4 * 32];  // layer 2 after ReLU: per z-slice channel sums
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = lane & 15, g = lane >> 4;  // MFMA operand row / column and K quarter
-  for (int i = tid; i < 16 * 32; i += SI_THREADS) W0s[i] = f2bf(P[L.w0 + i]);
-  for (int i = tid; i < 32 * 432; i += SI_THREADS) W1s[i] = f2bf(P[L.w1 + i]);
-  for (int i = tid; i < 32 * 864; i += SI_THREADS) W2s[i] = f2bf(P[L.w2 + i]);
+  const int n = lane & 15, g = lane >> 4;
+
+  // weights as B fragments: lane (n, g) of K-step s holds W[tile * 16 + n][32 s + 8 g .. + 7]
+  for (int i = tid; i < 64 * 8; i += SI_THREADS) {
+    const int l = i >> 3, k = 8 * (l >> 4) + (i & 7);
+    W0f[i] = k < 27 ? f2bf(P[L.w0 + (l & 15) * 32 + k]) : (uint16_t)0;
+  }
+  for (int i = tid; i < 2 * 13 * 512; i += SI_THREADS) {
+    const int l = (i >> 3) & 63, s = (i >> 9) % 13, tile = i / (13 * 512);
+    W1f[i] = f2bf(P[L.w1 + (16 * tile + (l & 15)) * 432 + 32 * s + 8 * (l >> 4) + (i & 7)]);
+  }
+  for (int i = tid; i < 2 * 256; i += SI_THREADS) {
+    const int l = (i >> 2) & 63, tile = i >> 8;
+    W1t[i] = f2bf(P[L.w1 + (16 * tile + (l & 15)) * 432 + 416 + 4 * (l >> 4) + (i & 3)]);
+  }
+  for (int i = tid; i < 2 * 27 * 512; i += SI_THREADS) {
+    const int l = (i >> 3) & 63, s = (i >> 9) % 27, tile = i / (27 * 512);
+    W2f[i] = f2bf(P[L.w2 + (16 * tile + (l & 15)) * 864 + 32 * s + 8 * (l >> 4) + (i & 7)]);
+  }
+  for (int i = tid; i < 36 * 64; i += SI_THREADS) W4t[i] = P[L.W4 + (i & 63) * 36 + (i >> 6)];
   if (tid < 16) Bs[tid] = P[L.b0 + tid];
   else if (tid < 48) Bs[tid] = P[L.b1 + tid - 16];
   else if (tid < 80) Bs[tid] = P[L.b2 + tid - 48];
-  for (int i = tid; i < 18 * 18 * 18; i += SI_THREADS) X0[i] = 0;
-  for (int i = tid; i < 10 * 10 * 10 * 16; i += SI_THREADS) A1[i] = 0;
-  for (int i = tid; i < 6 * 6 * 6 * 32; i += SI_THREADS) A2[i] = 0;
-  __syncthreads();  // the halos are zero before any interior is written
+  else if (tid < 144) Bs[tid] = P[L.b4 + tid - 80];
+  else if (tid < 208) Bs[tid] = P[L.W5 + tid - 144];
+  for (int i = tid; i < 18 * kX0Slab; i += SI_THREADS) X0[i] = 0;
+  for (int i = tid; i < 2 * kA1Plane; i += SI_THREADS) A1[i] = 0;
+  for (int i = tid; i < 4 * kA2Plane; i += SI_THREADS) A2[i] = 0;
+  const float b5 = P[L.b5];
 
-  float pre[8];  // this thread's 8 patch values of the next throat
+  // patch gather: this thread's 8 voxels are (x, y) = (fx, fy), z = fz + 2 j
+  const int fx = tid & 15, fy = (tid >> 4) & 15, fz = tid >> 8;
+  float pre[8];
+  // centre of the throat the next fetch gathers, loaded one phase ahead of it
+  float cxyz[3] = {0.f, 0.f, 0.f};
+  auto load_center = [&](int t) {
+    if (t >= nt) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cxyz[k] = centers[3 * t + k];
+  };
   auto fetch = [&](int t) {
     if (t >= nt) return;
-    const int cx = (int)floorf(centers[3 * t + 0]) - 7;
-    const int cy = (int)floorf(centers[3 * t + 1]) - 7;
-    const int cz = (int)floorf(centers[3 * t + 2]) - 7;
+    // the patch corner is wrapped once; the thread's offset (< 16) wraps at most once more
+    int x = wrap_any((int)floorf(cxyz[0]) - 7, nx) + fx;
+    int y = wrap_any((int)floorf(cxyz[1]) - 7, ny) + fy;
+    int z = wrap_any((int)floorf(cxyz[2]) - 7, nz) + fz;
+    x = nx >= 16 ? (x >= nx ? x - nx : x) : wrap_any(x, nx);
+    y = ny >= 16 ? (y >= ny ? y - ny : y) : wrap_any(y, ny);
+    z = nz >= 16 ? (z >= nz ? z - nz : z) : wrap_any(z, nz);
+    const float* row = sdf + (int64_t)y * nx + x;
+    const int64_t slab = (int64_t)nx * ny;
+    int zz[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int e = tid + SI_THREADS * j;
-      const int x = wrap_any(cx + (e & 15), nx), y = wrap_any(cy + ((e >> 4) & 15), ny);
-      const int z = wrap_any(cz + (e >> 8), nz);
-      pre[j] = sdf[((int64_t)z * ny + y) * nx + x] / scale;
+      zz[j] = z;
+      z += 2;
+      if (nz >= 16) z = z >= nz ? z - nz : z;  // one wrap at most: z < nz + 2
+      else z = wrap_any(z, nz);
+    }
+    // raw values only: the loads stay in flight until the stash one throat later
+#pragma unroll
+    for (int j = 0; j < 8; ++j) pre[j] = row[(int64_t)zz[j] * slab];
+  };
+  auto stash = [&](int t) {
+    if (t >= nt) return;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float v = inv_scale != 0.0f ? pre[j] * inv_scale : pre[j] / scale;  // 2^-k: exact
+      X0[(fz + 2 * j + 1) * kX0Slab + (fy + 1) * kX0Row + fx + 1] = f2bf_hw(v);
     }
   };
-  int t = blockIdx.x;
-  fetch(t);
-  for (; t < nt; t += gridDim.x) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = tid + SI_THREADS * j;
-      X0[(((e >> 8) + 1) * 18 + ((e >> 4) & 15) + 1) * 18 + (e & 15) + 1] = f2bf(pre[j]);
-    }
-    __syncthreads();
-    fetch(t + (int)gridDim.x);
 
-    // ---- layer 0: 1 -> 16 channels on 16^3, K = 27 taps (+5 zero); a tile is one x-row ----
+  // layer-0 A operand: taps 8 g + j of the lane's voxel; taps >= 27 (zero weights) read tap 0
+  int off0[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tp = 8 * g + j;
+    off0[j] = tp < 27 ? (tp / 9) * kX0Slab + ((tp / 3) % 3) * kX0Row + tp % 3 : 0;
+  }
+  // layer-1 A operand: K-step s covers taps 2 s and 2 s + 1; lane group g reads tap
+  // 2 s + (g >> 1), channel plane g & 1
+  const int t1 = g >> 1;
+  const uint16_t* a1lane = A1 + (g & 1) * kA1Plane + ((n >> 3) * 10 + (n & 7)) * 8;
+  const uint16_t* a1last = A1 + (g >> 1) * kA1Plane + ((n >> 3) * 10 + (n & 7)) * 8 + 4 * (g & 1);
+
+  const int G = (int)gridDim.x;
+  int t = blockIdx.x;
+  load_center(t);
+  fetch(t);
+  load_center(t + G);
+  __syncthreads();  // halos are zero before any interior is written
+  stash(t);
+  fetch(t + G);
+  for (; t < nt; t += G) {
+    __syncthreads();  // X0 holds throat t; A1, A2 and part are free
+    load_center(t + 2 * G);  // used by this iteration's fetch
+    // the throat scalars for this iteration's dense head (uniform loads)
+    const float sc4[4] = {scal[4 * t], scal[4 * t + 1], scal[4 * t + 2], scal[4 * t + 3]};
+
+    // ---- layer 0: 1 -> 16 channels on 16^3, K = 27 taps (+5 zero); a tile is one x-row.
+    //      Wave w computes the 8 pooled rows of pooled slice w. ----
     {
-      const bf16x4 b0 = lds4(W0s + n * 32 + 4 * g), b1 = lds4(W0s + n * 32 + 16 + 4 * g);
+      const bf16x8 b = lds8(W0f + lane * 8);
       const float bias = Bs[n];
-      for (int jr = 0; jr < 8; ++jr) {  // pooled rows (pz, py) = wave * 8 + jr
-        const int q = wave * 8 + jr, pz = q >> 3, py = q & 7;
+      const f32x4 bias4 = {bias, bias, bias, bias};  // the MFMA adds the bias
+      const int pz = wave;
+#pragma unroll
+      for (int py = 0; py < 8; ++py) {
         float v[2][2][4];
 #pragma unroll
         for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
           for (int dy = 0; dy < 2; ++dy) {
-            const int z = 2 * pz + dz, y = 2 * py + dy, x = n;
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            const uint16_t* base = X0 + (2 * pz + dz) * kX0Slab + (2 * py + dy) * kX0Row + n;
+            u16x8 a;
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-              bf16x4 a;
+            for (int j = 0; j < 8; ++j) a[j] = base[off0[j]];
+            const f32x4 acc = mfma32(__builtin_bit_cast(bf16x8, a), b, bias4);
 #pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                const int tap = 16 * ks + 4 * g + j;
-                short val = 0;
-                if (tap < 27) {
-                  const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
-                  val = (short)X0[((z + tz) * 18 + (y + ty)) * 18 + x + tx];
-                }
-                a[j] = val;
-              }
-              acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ks ? b1 : b0, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {  // row 4g + i = x, column n = channel
-              float c = acc[i] + bias;
-              v[dz][dy][i] = c > 0.f ? c : 0.f;
-            }
+            for (int i = 0; i < 4; ++i)  // row 4 g + i = x, column n = channel
+              v[dz][dy][i] = acc[i] > 0.f ? acc[i] : 0.f;
           }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {  // pooled x = 2g + h
+        for (int h = 0; h < 2; ++h) {  // pooled x = 2 g + h
           float s = 0.f;
 #pragma unroll
           for (int dz = 0; dz < 2; ++dz)
@@ -657,18 +754,22 @@ __global__ __launch_bounds__(SI_THREADS) void k_sur_infer(
               s += v[dz][dy][2 * h];
               s += v[dz][dy][2 * h + 1];
             }
-          A1[(((pz + 1) * 10 + py + 1) * 10 + 2 * g + h + 1) * 16 + n] = f2bf(s * 0.125f);
+          A1[(n >> 3) * kA1Plane + (((pz + 1) * 10 + py + 1) * 10 + 2 * g + h + 1) * 8 + (n & 7)] =
+              f2bf_hw(s * 0.125f);
         }
       }
     }
-    __syncthreads();
+    __syncthreads();  // A1 complete; X0 free
+    stash(t + G);
+    fetch(t + 2 * G);
 
-    // ---- layer 1: 16 -> 32 channels on 8^3, K = 27 taps x 16 channels ----
-    // A tile is 16 positions (two 8-voxel rows y, y+1 of one slice); this wave owns the tiles
-    // (z, rows 2yt, 2yt+1) for z in {2pz, 2pz+1} and yt in {py0, py0+1}, both column tiles.
+    // ---- layer 1: 16 -> 32 channels on 8^3, K = 27 taps x 16 channels. A tile is 16
+    //      positions (two 8-voxel rows y, y + 1 of one slice); this wave owns the tiles
+    //      (z, rows 2 yt, 2 yt + 1) for z in {2 pz, 2 pz + 1} and yt in {py0, py0 + 1}, both
+    //      column tiles. ----
     {
       const int pz = wave >> 1, py0 = 2 * (wave & 1);
-      const int ry = n >> 3, rx = n & 7;  // this lane's A row inside a tile
+      const int wbase = (2 * pz * 10 + 2 * py0) * 10 * 8;
       f32x4 acc[2][2][2];
 #pragma unroll
       for (int a0 = 0; a0 < 2; ++a0)
@@ -676,24 +777,37 @@ __global__ __launch_bounds__(SI_THREADS) void k_sur_infer(
         for (int a1 = 0; a1 < 2; ++a1)
 #pragma unroll
           for (int a2 = 0; a2 < 2; ++a2) acc[a0][a1][a2] = f32x4{0.f, 0.f, 0.f, 0.f};
-      for (int tap = 0; tap < 27; ++tap) {
-        const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
-        const bf16x4 bf0 = lds4(W1s + n * 432 + tap * 16 + 4 * g);
-        const bf16x4 bf1 = lds4(W1s + (16 + n) * 432 + tap * 16 + 4 * g);
+#pragma unroll
+      for (int s = 0; s < 13; ++s) {
+        const int toff = (t1 ? tap_off(2 * s + 1, 10) : tap_off(2 * s, 10)) * 8;
+        const uint16_t* ap = a1lane + wbase + toff;
+        const bf16x8 bf0 = lds8(W1f + (0 * 13 + s) * 512 + lane * 8);
+        const bf16x8 bf1 = lds8(W1f + (1 * 13 + s) * 512 + lane * 8);
 #pragma unroll
         for (int jy = 0; jy < 2; ++jy)
 #pragma unroll
           for (int dz = 0; dz < 2; ++dz) {
-            const int z = 2 * pz + dz, y = 2 * (py0 + jy) + ry;
-            const bf16x4 a =
-                lds4(A1 + (((z + tz) * 10 + y + ty) * 10 + rx + tx) * 16 + 4 * g);
+            const bf16x8 a = lds8(ap + (dz * 100 + jy * 20) * 8);
+            acc[jy][dz][0] = mfma32(a, bf0, acc[jy][dz][0]);
+            acc[jy][dz][1] = mfma32(a, bf1, acc[jy][dz][1]);
+          }
+      }
+      {  // tap 26 alone: K = 16, lane group g reads channels 4 g .. 4 g + 3
+        const uint16_t* ap = a1last + wbase + tap_off(26, 10) * 8;
+        const bf16x4 bf0 = lds4(W1t + lane * 4);
+        const bf16x4 bf1 = lds4(W1t + 256 + lane * 4);
+#pragma unroll
+        for (int jy = 0; jy < 2; ++jy)
+#pragma unroll
+          for (int dz = 0; dz < 2; ++dz) {
+            const bf16x4 a = lds4(ap + (dz * 100 + jy * 20) * 8);
             acc[jy][dz][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf0, acc[jy][dz][0],
                                                                       0, 0, 0);
             acc[jy][dz][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf1, acc[jy][dz][1],
                                                                       0, 0, 0);
           }
       }
-      // C row 4g + i: y offset g >> 1, x = 4 (g & 1) + i; the y + 1 row sits 32 lanes up
+      // C row 4 g + i: y offset g >> 1, x = 4 (g & 1) + i; the y + 1 row sits 32 lanes up
 #pragma unroll
       for (int jy = 0; jy < 2; ++jy)
 #pragma unroll
@@ -709,9 +823,9 @@ __global__ __launch_bounds__(SI_THREADS) void k_sur_infer(
               w[dz][i] = __shfl_xor(v[dz][i], 32, 64);
             }
           if (g < 2) {
-            const int py = py0 + jy;
+            const int py = py0 + jy, c = 16 * nt2 + n;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {  // pooled x = 2g + h
+            for (int h = 0; h < 2; ++h) {  // pooled x = 2 g + h
               float s = 0.f;
 #pragma unroll
               for (int dz = 0; dz < 2; ++dz) {
@@ -720,53 +834,52 @@ __global__ __launch_bounds__(SI_THREADS) void k_sur_infer(
                 s += w[dz][2 * h];
                 s += w[dz][2 * h + 1];
               }
-              A2[(((pz + 1) * 6 + py + 1) * 6 + 2 * g + h + 1) * 32 + 16 * nt2 + n] =
-                  f2bf(s * 0.125f);
+              A2[(c >> 3) * kA2Plane + (((pz + 1) * 6 + py + 1) * 6 + 2 * g + h + 1) * 8 +
+                 (c & 7)] = f2bf_hw(s * 0.125f);
             }
           }
         }
     }
-    __syncthreads();
+    __syncthreads();  // A2 complete
 
-    // ---- layer 2: 32 -> 32 channels on 4^3, K = 27 taps x 32 channels; one tile per wave
-    //      (a whole slice z, one column tile) ----
+    // ---- layer 2: 32 -> 32 channels on 4^3, K = 27 taps x 32 channels (one tap per K-step);
+    //      one tile per wave: slice z, column tile nt2; then its channel sums over the slice ----
     {
+      // A row n is the voxel (y, x) = (n & 3, n >> 2) of slice z: the 8 lanes of a 16-byte
+      // read then cover 8 different 16-byte bank groups
       const int z = wave >> 1, nt2 = wave & 1;
-      const int ry = n >> 2, rx = n & 3;
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int ks = 0; ks < 54; ++ks) {
-        const int tap = ks >> 1, c0 = (ks & 1) * 16;
-        const int tz = tap / 9, ty = (tap / 3) % 3, tx = tap % 3;
-        const bf16x4 a = lds4(A2 + (((z + tz) * 6 + ry + ty) * 6 + rx + tx) * 32 + c0 + 4 * g);
-        const bf16x4 b = lds4(W2s + (16 * nt2 + n) * 864 + ks * 16 + 4 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-      }
+      const uint16_t* ap = A2 + g * kA2Plane + ((z * 6 + (n & 3)) * 6 + (n >> 2)) * 8;
+      const uint16_t* bp = W2f + nt2 * 27 * 512 + lane * 8;
       const float bias = Bs[48 + 16 * nt2 + n];
+      f32x4 acc = {bias, bias, bias, bias};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {  // position z*16 + 4g + i (y = g, x = i)
-        const float c = acc[i] + bias;
-        G2[(z * 16 + 4 * g + i) * 32 + 16 * nt2 + n] = c > 0.f ? c : 0.f;
-      }
+      for (int s = 0; s < 27; ++s)
+        acc = mfma32(lds8(ap + tap_off(s, 6) * 8), lds8(bp + s * 512), acc);
+      float sum = 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sum += acc[i] > 0.f ? acc[i] : 0.f;  // 4 voxels of the slice
+      sum += __shfl_xor(sum, 16, 64);
+      sum += __shfl_xor(sum, 32, 64);
+      if (g == 0) part[z * 32 + 16 * nt2 + n] = sum;
     }
-    __syncthreads();
+    __syncthreads();  // part complete
 
     // ---- global average, scalars, dense head (wave 0) ----
+    // every lane forms the 36 inputs itself (broadcast reads of the slice sums); lane j
+    // computes hidden unit j, and the output is the wave's sum
     if (wave == 0) {
-      float h = 0.f;
-      if (lane < 32) {
-        float s = 0.f;
-        for (int k = 0; k < 64; ++k) s += G2[k * 32 + lane];
-        h = s / 64.0f;
-      } else if (lane < 36) {
-        h = scal[4 * t + lane - 32];
+      float q[4] = {Bs[80 + lane], 0.f, 0.f, 0.f};  // four independent chains
+#pragma unroll
+      for (int k = 0; k < 32; ++k) {
+        const float h = (((part[k] + part[32 + k]) + part[64 + k]) + part[96 + k]) / 64.0f;
+        q[k & 3] += W4t[k * 64 + lane] * h;
       }
-      float s = P[L.b4 + lane];
-      for (int k = 0; k < 36; ++k) s += P[L.W4 + lane * 36 + k] * lane_f(h, k);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[k] += W4t[(32 + k) * 64 + lane] * sc4[k];
+      float s = (q[0] + q[1]) + (q[2] + q[3]);
       s = s > 0.f ? s : 0.f;
-      const float sh = P[L.W5 + lane] * s;
-      float y = P[L.b5];
-      for (int k = 0; k < 64; ++k) y += lane_f(sh, k);
-      if (lane == 0) out[t] = y;
+      const float y = wave_sum_dpp(Bs[144 + lane] * s);
+      if (lane == 63) out[t] = b5 + y;
     }
   }
 }
@@ -779,8 +892,11 @@ void surrogate_infer(Ctx* c, const float* sdf, Vol v, const float* centers, cons
   int cus = 0;
   PNM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
   const int groups = std::min(B, std::max(cus, 1));
+  // a power-of-two scale divides exactly as a multiplication by its inverse
+  int ex = 0;
+  const float inv = std::frexp(scale, &ex) == 0.5f ? std::ldexp(1.0f, 1 - ex) : 0.0f;
   PNM_LAUNCH(c, "k_sur_infer", k_sur_infer, dim3((unsigned)groups), dim3(SI_THREADS), 0, sdf,
-             v.nx, v.ny, v.nz, centers, scalars, B, scale, params, o, out);
+             v.nx, v.ny, v.nz, centers, scalars, B, scale, inv, params, o, out);
 }
 
 void surrogate_release(Ctx* c) {

@@ -141,12 +141,15 @@ def test_stream_switch_orders_pending_work():
         ctx.close()
 
 
-@pytest.mark.parametrize("B", [300, 1000])
-def test_fused_inference_matches_forward(capi_ctx, B):
+@pytest.mark.parametrize("B,scale", [(300, 8.0), (1000, 8.0), (300, 6.0)])
+def test_fused_inference_matches_forward(capi_ctx, B, scale):
     """pnm_sur_infer_d (patch gather + network in one kernel) against pnm_sur_patches_d +
     pnm_sur_forward_d on the same throats, centres spread over the whole periodic volume
     (patches wrap), more throats than workgroups (the persistent loop), and against the FP64
-    oracle on a subset."""
+    oracle on a subset. Both round the same values to BF16 at the same points; the FP32 sums
+    differ in order (K-steps of 32 in the fused kernel, 16 in the forward GEMM; the global
+    average per z-slice), so the bar is 5e-5 of the output scale. scale 8 takes the exact
+    multiply-by-inverse path, scale 6 the division."""
     from peclet.pnm import capi
 
     rng = np.random.default_rng(7 + B)
@@ -161,16 +164,16 @@ def test_fused_inference_matches_forward(capi_ctx, B):
     scal = rng.normal(0, 1, size=(B, 4)).astype(np.float32)
     sd, cd, scd, pd = dev(sdf), dev(centers), dev(scal), dev(p)
     fused = torch.empty(B, device="cuda")
-    capi_ctx.sur_infer_d(sd, (56, 48, 40), cd, scd, B, 8.0, pd, fused)
+    capi_ctx.sur_infer_d(sd, (56, 48, 40), cd, scd, B, scale, pd, fused)
     patches = torch.empty(B * 4096, device="cuda")
-    capi_ctx.sur_patches_d(sd, (56, 48, 40), cd, B, 8.0, patches)
+    capi_ctx.sur_patches_d(sd, (56, 48, 40), cd, B, scale, patches)
     ref = torch.empty(B, device="cuda")
     capi_ctx.sur_forward_d(B, patches, scd, pd, ref)
     f, r = fused.cpu().numpy(), ref.cpu().numpy()
-    scale = max(1.0, float(np.abs(r).max()))
+    ys = max(1.0, float(np.abs(r).max()))
     print(f"[fused B={B}] max |fused - forward| = {np.abs(f - r).max():.3e}, "
           f"bitwise equal {np.array_equal(f, r)}", flush=True)
-    assert np.abs(f - r).max() <= 1e-5 * scale
+    assert np.abs(f - r).max() <= 5e-5 * ys
     k = 24
     y_ref, _ = so.forward(p.astype(np.float64), patches[: k * 4096].cpu().numpy().reshape(
         k, 16, 16, 16), scal[:k], rounding=True)
