@@ -116,6 +116,66 @@ __device__ void block_wave_sort(uint32_t* k, uint32_t* v, uint32_t* k2, uint32_t
   }
 }
 
+// q = a / d, r = a % d for 0 <= a, 0 < d: a float quotient estimate, then exact corrections.
+__device__ __forceinline__ void fdivmod(int a, int d, float inv_d, int& q, int& r) {
+  q = (int)((float)a * inv_d);
+  r = a - q * d;
+  while (r < 0) {
+    --q;
+    r += d;
+  }
+  while (r >= d) {
+    ++q;
+    r -= d;
+  }
+}
+
+// runs_union_chunk (k_common.hpp) with the same union pairs, every partner label loaded before
+// the first union: the forward rows at x, and at a run's first / last column their x-1 / x+1
+// (used only where the forward row has no marker at x). The unions' finds then overlap the
+// loads of the next pairs instead of waiting behind them. A label read before another union
+// rewrote it is an ancestor of what it would read later, so the components are the same.
+__device__ __forceinline__ void runs_union_chunk_hoist(int* L, int nx, int ny, int nz, int x,
+                                                       int y, int z, int lane) {
+  const bool in = (x < nx) && (y < ny);
+  const int i = in ? (z * ny + y) * nx + x : 0;
+  const int my = in ? L[i] : -1;
+  const bool mk = my != -1;
+  const uint64_t m = __ballot(mk);
+  if (!__any(mk)) return;  // wave-uniform
+  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+  const bool run_end = mk && (lane == 63 || !((m >> (lane + 1)) & 1ull));
+  const int xm = in ? wm1(x, nx) : 0, xp = in ? wp1(x, nx) : 0;
+  const int yp = in ? wp1(y, ny) : 0, ym = in ? wm1(y, ny) : 0, zp = wp1(z, nz);
+  const int rows[4] = {(z * ny + yp) * nx, (zp * ny + ym) * nx, (zp * ny + y) * nx,
+                       (zp * ny + yp) * nx};
+  int c[4], pc[4], ca[4], cb[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) c[r] = in ? L[rows[r] + x] : -1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    ca[r] = run_start ? L[rows[r] + xm] : -1;
+    cb[r] = run_end ? L[rows[r] + xp] : -1;
+  }
+  // +x relation across the chunk start; at x = 0 this is the periodic wrap from x = nx-1
+  const int nl = (mk && lane == 0) ? L[(z * ny + y) * nx + xm] : -1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) pc[r] = __shfl_up(c[r], 1, 64);
+  if (nl != -1) uf_unite(L, my, nl);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (mk && c[r] != -1) {
+      // one union per touching pair: skipped when column x-1 gave the same pair
+      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && pc[r] == c[r];
+      if (!dup) uf_unite(L, my, c[r]);
+    }
+    if (mk && c[r] == -1) {  // diagonal-only contacts at the run's ends
+      if (ca[r] != -1) uf_unite(L, my, ca[r]);
+      if (cb[r] != -1) uf_unite(L, my, cb[r]);
+    }
+  }
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ vols,
                                                   const uint32_t* __restrict__ order, int nvol,
@@ -188,34 +248,52 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
     // ---- P2: run unions ----
     for (int ch = wave; ch < nchunks; ch += NW) {
       const int xc = ch % cx, row = ch / cx;
-      runs_union_chunk(W, nx, ny, nz, (xc << 6) + lane, row % ny, row / ny, lane);
+      runs_union_chunk_hoist(W, nx, ny, nz, (xc << 6) + lane, row % ny, row / ny, lane);
     }
     __syncthreads();
 
     // ---- P3: flatten; roots -> solid keys ----
-    for (int base = 0; base < n; base += BS) {
-      const int i = base + tid;
-      bool root = false;
-      if (i < n) {
-        int l = W[i];
-        if (l != -1) {
-          int p = W[l];
-          while (p != l) {
-            l = p;
-            p = W[l];
+    // four voxels per thread, their root chases advanced side by side
+    for (int base = 0; base < n; base += BS * 4) {
+      int l[4], p[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = base + c * BS + tid;
+        l[c] = i < n ? W[i] : -1;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) p[c] = l[c] != -1 ? W[l[c]] : -1;
+      while (true) {
+        bool more = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (l[c] != -1 && p[c] != l[c]) {
+            l[c] = p[c];
+            more = true;
           }
-          W[i] = l;
-          root = l == i;
+        }
+        if (!more) break;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[c] = l[c] != -1 ? W[l[c]] : -1;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int i = base + c * BS + tid;
+        bool root = false;
+        if (i < n && l[c] != -1) {
+          W[i] = l[c];
+          root = l[c] == i;
           if (root) mn[i] = i;  // the component's smallest marker; flood may lower it
         }
+        const unsigned sidx = wave_append_lds(&ctr[2], root, lane);
+        if (root) skeys[sidx] = (uint32_t)i;
       }
-      const unsigned sidx = wave_append_lds(&ctr[2], root, lane);
-      if (root) skeys[sidx] = (uint32_t)i;
     }
     __syncthreads();
 
     // ---- P4: Jacobi flood over the band ----
     {
+      const float inv_nx = 1.0f / (float)nx, inv_ny = 1.0f / (float)ny;
       unsigned cnt = ctr[1];
       uint32_t* cur = band;
       uint32_t* nb2 = band2;
@@ -228,9 +306,9 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
           uint32_t i = 0;
           if (j < cnt) {
             i = cur[j];
-            const int x = (int)(i % (uint32_t)nx);
-            const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
-            const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+            int row, x, z, y;
+            fdivmod((int)i, nx, inv_nx, row, x);
+            fdivmod(row, ny, inv_ny, z, y);
             const int xs[3] = {wm1(x, nx), x, wp1(x, nx)};
             const int ys[3] = {wm1(y, ny), y, wp1(y, ny)};
             const int zs[3] = {wm1(z, nz), z, wp1(z, nz)};
@@ -272,22 +350,49 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
 
     // ---- P5: roots into W (pore voxels), pore first-appearance minima (voxels before their
     // peak only; solid minima were completed by the flood) ----
-    for (int base = 0; base < n; base += BS) {
-      const int i = base + tid;
-      int pk = -1;
-      if (i < n && sdf[i] > 0.0f) {
-        int w = i, st = 0;
-        for (; st < kMaxSteps; ++st) {
-          const int t = nxt[w];
-          if (t == w) break;
-          w = t;
-        }
-        if (st == kMaxSteps && nxt[w] != w) ctr[4] = 1;
-        W[i] = w;
-        pk = w;
+    // four voxels per thread, their capped chases advanced side by side
+    constexpr int kCh = 4;
+    for (int base = 0; base < n; base += BS * kCh) {
+      int w[kCh], st[kCh];
+      bool go[kCh], pore[kCh];
+#pragma unroll
+      for (int c = 0; c < kCh; ++c) {
+        const int i = base + c * BS + tid;
+        pore[c] = i < n && sdf[i] > 0.0f;
+        go[c] = pore[c];
+        w[c] = i;
+        st[c] = 0;
       }
-      const int ppk = __shfl_up(pk, 1, 64);
-      if (pk >= 0 && i < pk && (lane == 0 || ppk != pk)) atomicMin(mn + pk, i);
+      while (true) {
+        int t[kCh];
+#pragma unroll
+        for (int c = 0; c < kCh; ++c) t[c] = go[c] ? nxt[w[c]] : 0;
+        bool more = false;
+#pragma unroll
+        for (int c = 0; c < kCh; ++c) {
+          if (!go[c]) continue;
+          if (t[c] == w[c]) {
+            go[c] = false;
+          } else {
+            w[c] = t[c];
+            if (++st[c] == kMaxSteps) go[c] = false;
+          }
+          more |= go[c];
+        }
+        if (!more) break;
+      }
+#pragma unroll
+      for (int c = 0; c < kCh; ++c) {
+        const int i = base + c * BS + tid;
+        int pk = -1;
+        if (pore[c]) {
+          if (st[c] == kMaxSteps && nxt[w[c]] != w[c]) ctr[4] = 1;
+          W[i] = w[c];
+          pk = w[c];
+        }
+        const int ppk = __shfl_up(pk, 1, 64);
+        if (pk >= 0 && i < pk && (lane == 0 || ppk != pk)) atomicMin(mn + pk, i);
+      }
     }
     __syncthreads();
     unsigned npk = ctr[0];

@@ -300,10 +300,11 @@ def packing_params(seed):
             "porosity": float(rng.uniform(0.33, 0.45))}
 
 
-def build_dataset(ctx, seeds, size, rtol=1e-9, log=None):
+def build_dataset(ctx, seeds, size, rtol=1e-9, log=None, max_throats=0):
     """Solve and extract every packing; returns device patches [N,4096] and host arrays
     (raw scalars [N,4], log g [N], packing index [N]) of the throats with Q*dp > 0, plus a
-    per-packing summary."""
+    per-packing summary. max_throats > 0 keeps a random sample of that many of each packing's
+    valid throats (seeded by the packing's seed)."""
     import time
 
     torch = _torch()
@@ -318,6 +319,11 @@ def build_dataset(ctx, seeds, size, rtol=1e-9, log=None):
         s, net, t_st = flow_sample(ctx, sdf, rtol=rtol)
         centers, raw = throat_inputs(net, sdf.shape)
         logg, ok = conductance_targets(net)
+        if max_throats > 0 and ok.sum() > max_throats:
+            keep = np.random.default_rng(int(seed)).choice(np.flatnonzero(ok), max_throats,
+                                                           replace=False)
+            ok = np.zeros_like(ok)
+            ok[np.sort(keep)] = True
         P.append(gather_patches(ctx, s, sdf.shape, centers[ok]))
         R.append(raw[ok])
         T.append(logg[ok])
@@ -327,6 +333,70 @@ def build_dataset(ctx, seeds, size, rtol=1e-9, log=None):
                      "stokes_s": round(t_st, 3), "total_s": round(time.time() - t0, 3)})
         if log:
             log(info[-1])
+        del s
+    return (torch.cat(P) if P else torch.empty(0, 4096, device="cuda"),
+            np.concatenate(R) if R else np.zeros((0, 4), np.float32),
+            np.concatenate(T) if T else np.zeros(0, np.float32),
+            np.concatenate(K) if K else np.zeros(0, np.int32), info)
+
+
+def solve_samples(ctx, seeds, size, out_dir, max_throats=0, rtol=1e-9, log=None):
+    """Solve and extract each packing and write its valid throats' inputs and targets to
+    out_dir/pack_<seed>_<size>.npz (centers, raw scalars, log g; a random sample of max_throats
+    when > 0), so that solving can be split over several runs and the patches regathered later
+    from the regenerated SDF (dataset_from_samples)."""
+    import json
+    import os
+    import time
+
+    from . import synth
+
+    os.makedirs(out_dir, exist_ok=True)
+    for seed in seeds:
+        prm = packing_params(seed)
+        t0 = time.time()
+        sdf, phi, _ = synth.polydisperse_sdf((size, size, size), prm["median"], prm["sigma"],
+                                             prm["porosity"], seed=int(seed))
+        _, net, t_st = flow_sample(ctx, sdf, rtol=rtol)
+        centers, raw = throat_inputs(net, sdf.shape)
+        logg, ok = conductance_targets(net)
+        n_ok = int(ok.sum())
+        if max_throats > 0 and n_ok > max_throats:
+            keep = np.random.default_rng(int(seed)).choice(np.flatnonzero(ok), max_throats,
+                                                           replace=False)
+            ok = np.zeros_like(ok)
+            ok[np.sort(keep)] = True
+        info = {"seed": int(seed), **prm, "porosity_achieved": round(phi, 4),
+                "throats": int(len(ok)), "valid": n_ok, "kept": int(ok.sum()),
+                "stokes_s": round(t_st, 3), "total_s": round(time.time() - t0, 3)}
+        np.savez(os.path.join(out_dir, f"pack_{int(seed)}_{size}.npz"), centers=centers[ok],
+                 raw=raw[ok], logg=logg[ok], info=json.dumps(info))
+        if log:
+            log(info)
+
+
+def dataset_from_samples(ctx, seeds, size, in_dir):
+    """build_dataset's result from files written by solve_samples: the SDF of each packing is
+    regenerated from its seed (the generator is deterministic) and the patches gathered at the
+    saved centres."""
+    import json
+    import os
+
+    torch = _torch()
+    from . import synth
+
+    P, R, T, K, info = [], [], [], [], []
+    for k, seed in enumerate(seeds):
+        d = np.load(os.path.join(in_dir, f"pack_{int(seed)}_{size}.npz"))
+        prm = packing_params(seed)
+        sdf, _, _ = synth.polydisperse_sdf((size, size, size), prm["median"], prm["sigma"],
+                                           prm["porosity"], seed=int(seed))
+        s = torch.from_numpy(np.ascontiguousarray(sdf, dtype=np.float32)).cuda()
+        P.append(gather_patches(ctx, s, sdf.shape, d["centers"]))
+        R.append(d["raw"])
+        T.append(d["logg"])
+        K.append(np.full(len(d["logg"]), k, np.int32))
+        info.append(json.loads(str(d["info"])))
         del s
     return (torch.cat(P) if P else torch.empty(0, 4096, device="cuda"),
             np.concatenate(R) if R else np.zeros((0, 4), np.float32),

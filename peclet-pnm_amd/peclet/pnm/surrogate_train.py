@@ -1,7 +1,11 @@
 """Train the throat-conductance surrogate on GPU-generated data and evaluate it.
 
     python -m peclet.pnm.surrogate_train --train 56 --test 8 --size 128 --steps 4000 \\
-        --out <dir> [--weights <file.npz>] [--direction-spread N]
+        --out <dir> [--weights <file.npz>] [--direction-spread N] [--max-throats K]
+
+Large runs can solve the packings in parts first (--samples DIR --solve-train A:B /
+--solve-test A:B write each packing's throat inputs and targets), then train from DIR
+(--samples DIR): the patches are regathered from the regenerated SDFs.
 
 Every packing is a polydisperse grain pack (packing_params(seed)); its fields come from the
 project Stokes solver and its throats and g = Q/dp from extract_network_flow. Train and test
@@ -32,6 +36,16 @@ def main(argv=None):
     ap.add_argument("--direction-spread", type=int, default=0, metavar="N",
                     help="also solve N test packings with the flow along x, y and z and report "
                          "how much each throat's g changes with the direction")
+    ap.add_argument("--max-throats", type=int, default=0, metavar="K",
+                    help="keep a random sample of K valid throats per training packing (0: all)")
+    ap.add_argument("--samples", default=None, metavar="DIR",
+                    help="per-packing throat samples (solve_samples files): with --solve-train /"
+                         " --solve-test, solve those packings into DIR and stop; otherwise train"
+                         " from the samples in DIR instead of solving")
+    ap.add_argument("--solve-train", default=None, metavar="A:B",
+                    help="training packings A..B-1 to solve into --samples")
+    ap.add_argument("--solve-test", default=None, metavar="A:B",
+                    help="test packings A..B-1 to solve into --samples (all their throats)")
     ap.add_argument("--out", required=True)
     ap.add_argument("--weights", default=None)
     args = ap.parse_args(argv)
@@ -48,9 +62,32 @@ def main(argv=None):
     def log(d):
         print(json.dumps(d), flush=True)
 
+    def span(text):
+        a, b = (int(v) for v in text.split(":"))
+        return range(a, b)
+
+    if args.solve_train or args.solve_test:
+        if not args.samples:
+            raise SystemExit("--solve-train / --solve-test need --samples DIR")
+        if args.solve_train:
+            S.solve_samples(ctx, [9000 + k for k in span(args.solve_train)], args.size,
+                            args.samples, max_throats=args.max_throats, log=log)
+        if args.solve_test:
+            S.solve_samples(ctx, [9500 + k for k in span(args.solve_test)], args.size,
+                            args.samples, log=log)
+        ctx.close()
+        return 0
+
     t0 = time.time()
-    tr = S.build_dataset(ctx, [9000 + k for k in range(args.train)], args.size, log=log)
-    te = S.build_dataset(ctx, [9500 + k for k in range(args.test)], args.size, log=log)
+    if args.samples:
+        tr = S.dataset_from_samples(ctx, [9000 + k for k in range(args.train)], args.size,
+                                    args.samples)
+        te = S.dataset_from_samples(ctx, [9500 + k for k in range(args.test)], args.size,
+                                    args.samples)
+    else:
+        tr = S.build_dataset(ctx, [9000 + k for k in range(args.train)], args.size, log=log,
+                             max_throats=args.max_throats)
+        te = S.build_dataset(ctx, [9500 + k for k in range(args.test)], args.size, log=log)
     t_data = time.time() - t0
     Ptr, Rtr, Ttr, _, info_tr = tr
     Pte, Rte, Tte, _, info_te = te
@@ -81,6 +118,7 @@ def main(argv=None):
     fwd_flops = conv_flops + 2 * (36 * 64 + 64)
     report = {
         "train_packings": args.train, "test_packings": args.test, "size": args.size,
+        "max_throats_per_train_packing": args.max_throats,
         "train_throats": int(len(Ttr)), "test_throats": int(len(Tte)),
         "steps": args.steps, "batch": args.batch, "lr": args.lr, "huber_delta": args.delta,
         "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 2),
