@@ -78,26 +78,41 @@ __global__ void k_split_reduce(const float* __restrict__ Cp, int splits, int64_t
 
 // im2col with zero padding: col[p][k] (and colT[k][p]) for p = (b, z, y, x) of a D^3 grid,
 // k = tap*CI + ci (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)), zero for k >= 27*CI (K padding).
-__global__ void k_im2col(const float* __restrict__ act, int D, int CI, int Kp, int64_t P,
-                         uint16_t* __restrict__ col, uint16_t* __restrict__ colT) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P * Kp) return;
-  const int64_t p = i / Kp;
-  const int k = (int)(i - p * Kp);
-  float v = 0.f;
-  if (k < 27 * CI) {
-    const int tap = k / CI, ci = k - tap * CI;
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const int D3 = D * D * D;
-    const int64_t b = p / D3;
-    const int r = (int)(p - b * D3);
-    const int x = r % D + dx, y = (r / D) % D + dy, z = r / (D * D) + dz;
-    if (x >= 0 && x < D && y >= 0 && y < D && z >= 0 && z < D)
-      v = act[(b * D3 + (int64_t)(z * D + y) * D + x) * CI + ci];
+// col[p][k] (and colT[k][p] when wanted) for a tile of 64 positions x 64 columns per block:
+// the tile is built in LDS, then written row by row in both layouts, so both stores are
+// contiguous across the threads (P is a multiple of 64).
+__global__ __launch_bounds__(256) void k_im2col(const float* __restrict__ act, int D, int CI,
+                                                int Kp, int64_t P, uint16_t* __restrict__ col,
+                                                uint16_t* __restrict__ colT) {
+  __shared__ uint16_t T[64][66];
+  const int64_t p0 = (int64_t)blockIdx.x * 64;
+  const int k0 = blockIdx.y * 64;
+  const int D3 = D * D * D;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int pl = e >> 6, kl = e & 63;
+    const int64_t p = p0 + pl;
+    const int k = k0 + kl;
+    float v = 0.f;
+    if (k < 27 * CI) {
+      const int tap = k / CI, ci = k - tap * CI;
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+      const int64_t b = p / D3;
+      const int r = (int)(p - b * D3);
+      const int x = r % D + dx, y = (r / D) % D + dy, z = r / (D * D) + dz;
+      if (x >= 0 && x < D && y >= 0 && y < D && z >= 0 && z < D)
+        v = act[(b * D3 + (int64_t)(z * D + y) * D + x) * CI + ci];
+    }
+    const uint16_t h = f2bf(v);
+    T[pl][kl] = h;
+    if (k < Kp) col[p * Kp + k] = h;
   }
-  const uint16_t h = f2bf(v);
-  col[i] = h;
-  if (colT) colT[(int64_t)k * P + p] = h;
+  if (!colT) return;  // uniform
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int kl = e >> 6, pl = e & 63;
+    const int k = k0 + kl;
+    if (k < Kp) colT[(int64_t)k * P + p0 + pl] = T[pl][kl];
+  }
 }
 
 // dX[b][q][ci] = sum_tap dcol[b][q - off(tap)][tap*CI + ci] over in-range positions.
@@ -291,16 +306,37 @@ __global__ void k_head_bwd_w(const float* __restrict__ dy, const float* __restri
   }
 }
 
-// Bias gradient: db[c] = sum_p dz[p][c] (one block per channel, fixed order).
-__global__ void k_bias_grad(const float* __restrict__ dz, int64_t P, int C, float* __restrict__ db) {
+// Bias gradient db[c] = sum_p dz[p][c] in a fixed order, in two steps: block j sums the rows
+// p of its contiguous range (each thread one channel column of a row slice, coalesced rows),
+// then one block per channel adds the block partials in order.
+constexpr int kBiasBlocks = 512;
+__global__ __launch_bounds__(256) void k_bias_part(const float* __restrict__ dz, int64_t P, int C,
+                                                   float* __restrict__ part) {
+  __shared__ float sh[256];
+  const int rows = 256 / C;  // C divides 256
+  const int c = threadIdx.x % C, r = threadIdx.x / C;
+  const int64_t per = (P + kBiasBlocks - 1) / kBiasBlocks;
+  const int64_t a = (int64_t)blockIdx.x * per, b = min(P, a + per);
+  float acc = 0.f;
+  for (int64_t p = a + r; p < b; p += rows) acc += dz[p * C + c];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float s = 0.f;
+    for (int k = 0; k < rows; ++k) s += sh[k * C + threadIdx.x];
+    part[(int64_t)blockIdx.x * C + threadIdx.x] = s;
+  }
+}
+__global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ part, int C,
+                                                   float* __restrict__ db) {
   __shared__ float sh[256];
   const int c = blockIdx.x;
   float acc = 0.f;
-  for (int64_t p = threadIdx.x; p < P; p += blockDim.x) acc += dz[p * C + c];
+  for (int j = threadIdx.x; j < kBiasBlocks; j += blockDim.x) acc += part[(int64_t)j * C + c];
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+  for (int s2 = blockDim.x >> 1; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
     __syncthreads();
   }
   if (threadIdx.x == 0) db[c] = sh[0];
@@ -455,8 +491,8 @@ void surrogate_forward(Ctx* c, int B, const float* patches, const float* scalars
     S.col[l] = c->sur.get_n<uint16_t>((size_t)(P * cs.Kp));
     S.colT[l] = c->sur.get_n<uint16_t>((size_t)(P * cs.Kp));
     S.a[l] = c->sur.get_n<float>((size_t)(P * cs.CO));
-    PNM_LAUNCH(c, "k_im2col", k_im2col, dim3(cdiv(P * cs.Kp, 256)), dim3(256), 0, in, cs.D,
-               cs.CI, cs.Kp, P, S.col[l], S.colT[l]);
+    PNM_LAUNCH(c, "k_im2col", k_im2col, dim3((unsigned)(P / 64), cdiv(cs.Kp, 64)), dim3(256), 0,
+               in, cs.D, cs.CI, cs.Kp, P, S.col[l], S.colT[l]);
     uint16_t* wb = c->arena.get_n<uint16_t>((size_t)(cs.CO * cs.Kp));
     uint16_t* wT = c->arena.get_n<uint16_t>((size_t)(cs.CO * cs.Kp));
     PNM_LAUNCH(c, "k_weight_bf16", k_weight_bf16, dim3(cdiv(cs.CO * cs.Kp, 256)), dim3(256), 0,
@@ -502,8 +538,13 @@ void surrogate_backward(Ctx* c, int B, const float* dy, const float* params, flo
     else
       PNM_LAUNCH(c, "k_pool_bwd_relu", k_pool_bwd_relu, dim3(cdiv(P * cs.CO, 256)), dim3(256), 0,
                  dnext, S.a[l], B, cs.D, cs.CO, dz, dzb, dzT);
-    PNM_LAUNCH(c, "k_bias_grad", k_bias_grad, dim3(cs.CO), dim3(256), 0, dz, P, cs.CO,
-               grads + L.b[l]);
+    {
+      float* part = c->arena.get_n<float>((size_t)kBiasBlocks * cs.CO);
+      PNM_LAUNCH(c, "k_bias_part", k_bias_part, dim3(kBiasBlocks), dim3(256), 0, dz, P, cs.CO,
+                 part);
+      PNM_LAUNCH(c, "k_bias_grad", k_bias_grad, dim3(cs.CO), dim3(256), 0, part, cs.CO,
+                 grads + L.b[l]);
+    }
     // dW[co][k] = sum_p dzT[co][p] colT[k][p]
     gemm_nt(c, dzT, S.colT[l], grads + L.w[l], cs.CO, cs.Kp, (int)P, nullptr, false);
     if (l > 0) {

@@ -185,7 +185,7 @@ class ThroatSurrogate:
         return loss
 
     def fit(self, patches, scalars_raw, logg, steps=2000, batch=256, lr=2e-3, seed=0,
-            log_every=0, delta=0.25):
+            log_every=0, delta=0.25, log=None):
         """Train on device patches [N,4096], host raw scalars [N,4] and host log g [N] with a
         Huber loss in log g (threshold delta; a few throats with near-zero Q or dp have very
         large |log g| errors, and the error measure is a median)."""
@@ -205,6 +205,8 @@ class ThroatSurrogate:
                                    t.index_select(0, idx).contiguous(), lr_k, delta)
             if log_every and (k % log_every == 0 or k == steps - 1):
                 hist.append((k, float(loss.item())))
+                if log:
+                    log({"step": k, "loss": hist[-1][1]})
         return hist
 
     def state(self):
@@ -375,7 +377,7 @@ def solve_samples(ctx, seeds, size, out_dir, max_throats=0, rtol=1e-9, log=None)
             log(info)
 
 
-def dataset_from_samples(ctx, seeds, size, in_dir):
+def dataset_from_samples(ctx, seeds, size, in_dir, log=None):
     """build_dataset's result from files written by solve_samples: the SDF of each packing is
     regenerated from its seed (the generator is deterministic) and the patches gathered at the
     saved centres."""
@@ -397,6 +399,8 @@ def dataset_from_samples(ctx, seeds, size, in_dir):
         T.append(d["logg"])
         K.append(np.full(len(d["logg"]), k, np.int32))
         info.append(json.loads(str(d["info"])))
+        if log:
+            log({"loaded": int(seed), "throats": int(len(d["logg"]))})
         del s
     return (torch.cat(P) if P else torch.empty(0, 4096, device="cuda"),
             np.concatenate(R) if R else np.zeros((0, 4), np.float32),

@@ -81,9 +81,9 @@ def main(argv=None):
     t0 = time.time()
     if args.samples:
         tr = S.dataset_from_samples(ctx, [9000 + k for k in range(args.train)], args.size,
-                                    args.samples)
+                                    args.samples, log=log)
         te = S.dataset_from_samples(ctx, [9500 + k for k in range(args.test)], args.size,
-                                    args.samples)
+                                    args.samples, log=log)
     else:
         tr = S.build_dataset(ctx, [9000 + k for k in range(args.train)], args.size, log=log,
                              max_throats=args.max_throats)
@@ -105,7 +105,8 @@ def main(argv=None):
     torch.cuda.synchronize()
     t1 = time.time()
     hist = model.fit(Ptr, Rtr, Ttr, steps=args.steps, batch=args.batch, lr=args.lr,
-                     seed=args.seed, log_every=max(1, args.steps // 20), delta=args.delta)
+                     seed=args.seed, log_every=max(1, args.steps // 20), delta=args.delta,
+                     log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t1
     pred_te = model.predict(Pte, model.standardize(Rte)).cpu().numpy()
