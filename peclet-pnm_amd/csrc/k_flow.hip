@@ -150,7 +150,8 @@ __global__ __launch_bounds__(kFaceBlock) void k_face_init(
     const double* __restrict__ ox, const double* __restrict__ oy, const double* __restrict__ oz,
     int64_t* __restrict__ fpar, int64_t* __restrict__ fpair, int64_t* __restrict__ par,
     uint32_t* __restrict__ st_core, uint32_t* __restrict__ st_film, uint32_t* __restrict__ st_itf,
-    uint32_t* __restrict__ cnt3, uint64_t* __restrict__ st_res, uint32_t* __restrict__ cnt_res) {
+    uint32_t* __restrict__ cnt3, uint64_t* __restrict__ st_res, uint32_t* __restrict__ cnt_res,
+    uint8_t* __restrict__ itfm) {
   __shared__ unsigned wc[4][4];
   const int64_t c64 = (int64_t)blockIdx.x * kFaceBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -188,6 +189,7 @@ __global__ __launch_bounds__(kFaceBlock) void k_face_init(
       }
     }
   }
+  if (in) itfm[c] = (uint8_t)(core_m | film_m);
   const unsigned n_core = __popc(core_m), n_film = __popc(film_m), n_itf = __popc(core_m | film_m);
   unsigned n_res = 0;
 #pragma unroll
@@ -263,7 +265,7 @@ __device__ __forceinline__ void uf_unite64(int64_t* P, int64_t a, int64_t b) {
 // and pass the doubled-offset adjacency |2dc + e_d2 - e_d|^2 <= 8 (:711-723, :776-791).
 template <class F>
 __device__ __forceinline__ void for_adjacent(int64_t f, Geo g, const int64_t* __restrict__ fpair,
-                                             F&& fn) {
+                                             const uint8_t* __restrict__ itfm, F&& fn) {
   const int c = (int)(f / 3), d = (int)(f % 3);
   int x, y, z;
   cell_xyz(c, g, x, y, z);
@@ -271,11 +273,24 @@ __device__ __forceinline__ void for_adjacent(int64_t f, Geo g, const int64_t* __
   const int xs[3] = {wm1(x, g.nx), x, wp1(x, g.nx)};
   const int ys[3] = {wm1(y, g.ny), y, wp1(y, g.ny)};
   const int zs[3] = {wm1(z, g.nz), z, wp1(z, g.nz)};
+  // the interface faces of the 27 cells (one mask byte per cell): only those can share the pair
+  unsigned m[27];
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx)
+        m[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] =
+            itfm[(zs[dz + 1] * g.ny + ys[dy + 1]) * g.nx + xs[dx + 1]];
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
+        const unsigned mc = m[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+        if (!mc) continue;
         const int c2 = (zs[dz + 1] * g.ny + ys[dy + 1]) * g.nx + xs[dx + 1];
         for (int d2 = 0; d2 < 3; ++d2) {
+          if (!((mc >> d2) & 1u)) continue;
           const int64_t f2 = 3 * (int64_t)c2 + d2;
           if (f2 == f) continue;
           const int Dx = 2 * dx + (d2 == 0) - (d == 0);
@@ -292,11 +307,12 @@ __device__ __forceinline__ void for_adjacent(int64_t f, Geo g, const int64_t* __
 __global__ __launch_bounds__(256) void k_face_union(const uint32_t* __restrict__ list,
                                                     int64_t cnt, Geo g,
                                                     const int64_t* __restrict__ fpair,
+                                                    const uint8_t* __restrict__ itfm,
                                                     int64_t* par) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cnt) return;
   const int64_t f = list[j];
-  for_adjacent(f, g, fpair, [&](int64_t f2) {
+  for_adjacent(f, g, fpair, itfm, [&](int64_t f2) {
     if (f2 > f && ld_agent64(par + f2) >= 0) uf_unite64(par, f, f2);
   });
 }
@@ -322,13 +338,14 @@ __global__ void k_face_flatten(const uint32_t* __restrict__ list, int64_t cnt, i
 __global__ __launch_bounds__(256) void k_film_attach(const uint32_t* __restrict__ list,
                                                      int64_t cnt, Geo g,
                                                      const int64_t* __restrict__ fpair,
+                                                     const uint8_t* __restrict__ itfm,
                                                      int64_t* fpar, unsigned long long* changed) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cnt) return;
   const int64_t f = list[j];
   const int64_t cur = fpar[f];
   int64_t best = cur;
-  for_adjacent(f, g, fpair, [&](int64_t f2) {
+  for_adjacent(f, g, fpair, itfm, [&](int64_t f2) {
     const int64_t l2 = fpar[f2];
     if (l2 >= 0 && l2 < kSent && l2 < best) best = l2;
   });
@@ -363,19 +380,38 @@ __global__ void k_reset_par(const uint32_t* __restrict__ list, int64_t cnt,
 }
 
 // face_roots (:826-833): interface faces that are their own patch root.
-__global__ void k_face_roots(const uint32_t* __restrict__ list, int64_t cnt,
-                             const int64_t* __restrict__ fpar, uint64_t* __restrict__ roots,
-                             unsigned long long* ctr) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+// Patch roots (faces with fpar[f] == f) of the core list followed by the film list, staged per
+// 256-entry block in list order (blocks 0 .. nbc-1 take the core list, the rest the film list);
+// compacting the blocks in order gives the roots in list order without a contended counter.
+__global__ __launch_bounds__(256) void k_face_roots(const uint32_t* __restrict__ core,
+                                                    int64_t ncore, const uint32_t* __restrict__ film,
+                                                    int64_t nfilm, int64_t nbc,
+                                                    const int64_t* __restrict__ fpar,
+                                                    uint64_t* __restrict__ stage,
+                                                    uint32_t* __restrict__ counts) {
+  __shared__ unsigned wc[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool is_core = (int64_t)blockIdx.x < nbc;
+  const int64_t j = (is_core ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - nbc) * 256 + threadIdx.x;
+  const int64_t cnt = is_core ? ncore : nfilm;
   bool is_root = false;
   int64_t f = 0;
   if (j < cnt) {
-    f = list[j];
+    f = is_core ? core[j] : film[j];
     is_root = fpar[f] == f;
   }
-  const uint64_t s = wave_append(ctr, is_root, lane);
-  if (is_root) roots[s] = (uint64_t)f;
+  const uint64_t m = __ballot(is_root);
+  if (lane == 0) wc[wave] = (unsigned)__popcll(m);
+  __syncthreads();
+  unsigned base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) base += wc[w];
+    tot += wc[w];
+  }
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (is_root) stage[(int64_t)blockIdx.x * 256 + base + (unsigned)__popcll(m & below)] = (uint64_t)f;
+  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
 }
 
 // key for the (pair, root face) order: compact pair (lo << B | hi)
@@ -639,10 +675,10 @@ static T* host_array(Ctx* c, const T* dev, int64_t count) {
 }
 
 int64_t face_ccl(Ctx* c, const uint32_t* list, int64_t cnt, Geo g, const int64_t* fpair,
-                 int64_t* par, int64_t* fpar) {
+                 const uint8_t* itfm, int64_t* par, int64_t* fpar) {
   if (cnt <= 0) return 0;
   PNM_LAUNCH(c, "k_face_union", k_face_union, dim3(cdiv(cnt, 256)), dim3(256), 0, list, cnt, g,
-             fpair, par);
+             fpair, itfm, par);
   PNM_LAUNCH(c, "k_face_flatten", k_face_flatten, dim3(cdiv(cnt, 256)), dim3(256), 0, list, cnt,
              par, fpar);
   return cnt;
@@ -736,14 +772,15 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   uint64_t* st_res = kgeo ? nullptr : c->arena.get_n<uint64_t>((size_t)(nbf * kResCap));
   uint32_t* cnt3 = c->arena.get_n<uint32_t>((size_t)(3 * nbf));
   uint32_t* cnt_res = c->arena.get_n<uint32_t>((size_t)nbf);
+  uint8_t* itfm = c->arena.get_n<uint8_t>((size_t)n);  // per cell: which +d faces are interfaces
   if (kgeo)
     PNM_LAUNCH(c, "k_face_init", k_face_init<true>, dim3((unsigned)nbf), dim3(kFaceBlock), 0,
                sdf, lab, g, n, nullptr, nullptr, nullptr, fpar, fpair, par, st_core, st_film,
-               st_itf, cnt3, st_res, cnt_res);
+               st_itf, cnt3, st_res, cnt_res, itfm);
   else
     PNM_LAUNCH(c, "k_face_init", k_face_init<false>, dim3((unsigned)nbf), dim3(kFaceBlock), 0,
                sdf, lab, g, n, ox, oy, oz, fpar, fpair, par, st_core, st_film, st_itf, cnt3,
-               st_res, cnt_res);
+               st_res, cnt_res, itfm);
   uint32_t* core_list = c->arena.get_n<uint32_t>((size_t)nf);
   uint32_t* film_list = c->arena.get_n<uint32_t>((size_t)nf);
   uint32_t* itf_list = c->arena.get_n<uint32_t>((size_t)nf);
@@ -761,7 +798,7 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
   const int64_t nitf = (int64_t)h[6];
 
   // core tier
-  face_ccl(c, core_list, ncore, g, fpair, par, fpar);
+  face_ccl(c, core_list, ncore, g, fpair, itfm, par, fpar);
   // film attachment to a fixpoint
   // (two sweeps per read-back: a sweep at the fixpoint changes nothing, and every sweep that
   // changes something lowers at least one film label, so nfilm + 2 sweeps always suffice)
@@ -771,7 +808,7 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
       fill_u64(c, ctr + 3, 0, 1);
       for (int k = 0; k < 2; ++k)
         PNM_LAUNCH(c, "k_film_attach", k_film_attach, dim3(cdiv(nfilm, 256)), dim3(256), 0,
-                   film_list, nfilm, g, fpair, fpar, uctr + 3);
+                   film_list, nfilm, g, fpair, itfm, fpar, uctr + 3);
       uint64_t ch[2];
       readback(c, ctr + 3, 1, ch);
       if (ch[0] == 0) break;
@@ -791,18 +828,23 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
                film_list, nfilm, fpar, par, left, uctr + 4);
     readback(c, ctr + 4, 1, h);
     nleft = (int64_t)h[0];
-    face_ccl(c, left, nleft, g, fpair, par, fpar);
+    face_ccl(c, left, nleft, g, fpair, itfm, par, fpar);
   }
 
   // patch roots -> throats ordered by (pair, root face)
   uint64_t* root_fid = c->arena.get_n<uint64_t>((size_t)std::max<int64_t>(ncore + nfilm, 1));
-  fill_u64(c, ctr + 5, 0, 1);
-  if (ncore > 0)
-    PNM_LAUNCH(c, "k_face_roots", k_face_roots, dim3(cdiv(ncore, 256)), dim3(256), 0, core_list,
-               ncore, fpar, root_fid, uctr + 5);
-  if (nfilm > 0)
-    PNM_LAUNCH(c, "k_face_roots", k_face_roots, dim3(cdiv(nfilm, 256)), dim3(256), 0, film_list,
-               nfilm, fpar, root_fid, uctr + 5);
+  {
+    const int64_t nbc = cdiv(ncore, 256), nbm = cdiv(nfilm, 256), nbr = nbc + nbm;
+    uint64_t* rstage = c->arena.get_n<uint64_t>((size_t)std::max<int64_t>(nbr * 256, 1));
+    uint32_t* rcnt = c->arena.get_n<uint32_t>((size_t)std::max<int64_t>(nbr, 1));
+    if (nbr > 0) {
+      PNM_LAUNCH(c, "k_face_roots", k_face_roots, dim3((unsigned)nbr), dim3(256), 0, core_list,
+                 ncore, film_list, nfilm, nbc, fpar, rstage, rcnt);
+      compact_staged64(c, rstage, rcnt, nbr, 256, root_fid, ctr + 5);
+    } else {
+      fill_u64(c, ctr + 5, 0, 1);
+    }
+  }
   readback(c, ctr + 5, 1, h);
   const int64_t nt = (int64_t)h[0];
   int32_t* throats = c->arena.get_n<int32_t>((size_t)std::max<int64_t>(2 * nt, 2));

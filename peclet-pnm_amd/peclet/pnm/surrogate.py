@@ -227,9 +227,11 @@ def median_rel_error(logg_pred, logg_true):
                                            - np.asarray(logg_true, np.float64)))))
 
 
-def flow_sample(ctx, sdf, mu=1.0, fx=1.0, rtol=1e-9, max_iter=40000, geometry_only=False):
+def flow_sample(ctx, sdf, mu=1.0, fx=1.0, rtol=1e-9, max_iter=40000, geometry_only=False,
+                return_fields=False):
     """Stokes solve (unless geometry_only) + extract_network_flow on a host SDF (zyx).
-    Returns (device sdf, result dict, seconds of the Stokes solve)."""
+    Returns (device sdf, result dict, seconds of the Stokes solve), and the seven device fields
+    (u, v, w, p, ox, oy, oz) after them when return_fields."""
     import time
 
     torch = _torch()
@@ -248,6 +250,8 @@ def flow_sample(ctx, sdf, mu=1.0, fx=1.0, rtol=1e-9, max_iter=40000, geometry_on
     ox, oy, oz = f[4], f[5], f[6]
     net = ctx.extract_network_flow_d(s, dims, (0, 0, 0), (1, 1, 1), *f[:4], ox=ox, oy=oy, oz=oz,
                                      grad_p_xyz=(-fx, 0.0, 0.0))
+    if return_fields:
+        return s, net, t_st, f
     return s, net, t_st
 
 
