@@ -107,19 +107,46 @@ __global__ void k_pore_pressure(const pnm_pore_t* __restrict__ pores, int64_t np
   out[k] = acc;
 }
 
-// nf::flow_basins (:621-652): flowLab = seg at the capped ascent root of every cell.
+// nf::flow_basins (:621-652): flowLab = seg at the capped ascent root of every cell. Each
+// thread chases four cells (i, i + B, i + 2B, i + 3B for a block of B) side by side, so their
+// dependent loads overlap.
+constexpr int kFlowLabChains = 4;
 __global__ __launch_bounds__(256) void k_flow_lab(int64_t n, const int32_t* __restrict__ next,
                                                   const int32_t* __restrict__ seg,
                                                   int32_t* __restrict__ lab) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int w = (int)i;
-  for (int s = 0; s < kMaxSteps; ++s) {
-    const int nx = next[w];
-    if (nx == w) break;
-    w = nx;
+  const int64_t base = (int64_t)blockIdx.x * blockDim.x * kFlowLabChains + threadIdx.x;
+  int w[kFlowLabChains], st[kFlowLabChains];
+  bool go[kFlowLabChains];
+#pragma unroll
+  for (int c = 0; c < kFlowLabChains; ++c) {
+    const int64_t i = base + (int64_t)c * blockDim.x;
+    go[c] = i < n;
+    w[c] = go[c] ? (int)i : 0;
+    st[c] = 0;
   }
-  lab[i] = seg[w];
+  while (true) {
+    int t[kFlowLabChains];
+#pragma unroll
+    for (int c = 0; c < kFlowLabChains; ++c) t[c] = go[c] ? next[w[c]] : 0;
+    bool more = false;
+#pragma unroll
+    for (int c = 0; c < kFlowLabChains; ++c) {
+      if (!go[c]) continue;
+      if (t[c] == w[c] || st[c] == kMaxSteps) {
+        go[c] = false;
+      } else {
+        w[c] = t[c];
+        go[c] = ++st[c] < kMaxSteps;
+      }
+      more |= go[c];
+    }
+    if (!more) break;
+  }
+#pragma unroll
+  for (int c = 0; c < kFlowLabChains; ++c) {
+    const int64_t i = base + (int64_t)c * blockDim.x;
+    if (i < n) lab[i] = seg[w[c]];
+  }
 }
 
 // ---------------- faces ----------------
@@ -174,9 +201,13 @@ __global__ __launch_bounds__(kFaceBlock) void k_face_init(
                               : (ox ? (d == 0 ? ox[nb] : (d == 1 ? oy[nb] : oz[nb])) : 1.0);
       const bool itf = (a != b && a > 0 && b > 0 && opn > 0.0);
       const bool is_core = itf && sc > 0.0f && snb > 0.0f;
-      par[f] = is_core ? f : -1;
-      fpar[f] = itf ? (is_core ? f : kSent) : -1;
-      fpair[f] = itf ? (((int64_t)(a < b ? a : b) << 32) | (int64_t)(a < b ? b : a)) : -1;
+      // only interface faces get entries: every reader of par, fpar and fpair takes faces from
+      // the interface lists or behind the cell masks (itfm) written below
+      if (itf) {
+        par[f] = is_core ? f : -1;
+        fpar[f] = is_core ? f : kSent;
+        fpair[f] = ((int64_t)(a < b ? a : b) << 32) | (int64_t)(a < b ? b : a);
+      }
       core_m |= (unsigned)is_core << d;
       film_m |= (unsigned)(itf && !is_core) << d;
       if (!kGeo && a != b) {
@@ -754,8 +785,8 @@ void flow_device(Ctx* c, const float* sdf, Vol v, const float org[3], const floa
     int32_t* next_all = c->arena.get_n<int32_t>((size_t)n);
     int32_t* fl = c->arena.get_n<int32_t>((size_t)n);
     launch_next(c, sdf, v, next_all, kWalkAll);
-    PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256)), dim3(256), 0, n, next_all, seg,
-               fl);
+    PNM_LAUNCH(c, "k_flow_lab", k_flow_lab, dim3(cdiv(n, 256 * kFlowLabChains)), dim3(256), 0, n,
+               next_all, seg, fl);
     lab = fl;
   }
 
