@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void k_ccl_flatten(int* L, int64_t n, int32_t*
   bool root = false;
   if (i < n) {
     int l = L[i];
-    if (l != -1) {
+    if (l >= 0) {  // a marker (every other voxel holds a negative code)
       int p = L[l];
       while (p != l) {
         l = p;
@@ -79,15 +79,15 @@ constexpr int FR_SEG = 64 * TILE_Z;  // staging slots per wave
 constexpr int FLOOD_LIST = 3072;     // LDS list of band candidates per tile (3 blocks/CU)
 // region word: >= 0 label, -1 band candidate, -2 other unlabelled, <= -3 labelled this sweep
 
-__global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restrict__ sdf,
-                                                            const int* __restrict__ L,
+__global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restrict__ L,
                                                             int* __restrict__ Lout, int nx, int ny,
                                                             int nz, int32_t* __restrict__ min_solid,
                                                             uint32_t* __restrict__ key_stage,
                                                             uint32_t* __restrict__ key_cnt,
                                                             uint32_t* __restrict__ band_stage,
                                                             uint32_t* __restrict__ band_cnt,
-                                                            const int32_t* __restrict__ xface) {
+                                                            const int32_t* __restrict__ xface,
+                                                            bool write_all) {
   __shared__ int R[FR_N];
   __shared__ uint16_t cand_list[FLOOD_LIST];
   __shared__ unsigned s_wcnt[TILE_Y];
@@ -101,14 +101,15 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   constexpr int H = FLOOD_SWEEPS;
   // load: labels (a marker's component is L[L[i]]: its tile-local root points at the component
-  // root) and band candidates. Rows of the region are spread over the waves, a lane holding
+  // root) and band candidates (the tile pass wrote -1 for a voxel that is no marker and not
+  // sdf >= 0, -2 for the other non-markers). Rows of the region are spread over the waves, a
+  // lane holding
   // column x0 + lane (rows aligned to the chunk: two 128-byte lines per row and array). The
   // halo columns x0-1 and x0+64 are the neighbouring chunks' edge columns, whose tile labels
   // the tile pass wrote compactly (x-face pairs): one more load chain on the first threads,
   // issued with the first batch, with no value needed (halo entries are never swept, so an
   // unlabelled one is just "not a label"). A partial last chunk reads its right halo column
-  // directly. Per batch: the raw labels and values, then the component roots, then one LDS
-  // store. Loads are issued in branch-free batches of RB rows into registers (rows past the
+  // directly. Per batch: the raw labels, then the component roots, then one LDS store. Loads are issued in branch-free batches of RB rows into registers (rows past the
   // region re-read a valid row). Each region row's wrapped (z*ny + y) and flat offset come from
   // tables the block fills first, kept in the candidate list's LDS (not used before the sweep);
   // flat offsets fit 32 bits (volumes hold fewer than 2^31 voxels).
@@ -129,9 +130,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   }
   __syncthreads();
   auto row_base = [&](int row) -> int64_t { return (int64_t)rbase[row]; };
-  auto region_word = [](int l, int root, float v) {
-    return l >= 0 ? root : (!(v >= 0.0f) ? -1 : -2);
-  };
+  auto region_word = [](int l, int root) { return l >= 0 ? root : l; };
   const int swave = __builtin_amdgcn_readfirstlane(wave);
   const int nhalo = rows * 2;  // <= 2 * FR_Y * FR_Z < NT
   const int hrow = tid < nhalo ? tid >> 1 : 0, hside = tid & 1;  // 0: x0-1, 1: x0+64
@@ -151,13 +150,10 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   for (int j0 = 0; j0 < MAXR; j0 += RB) {
     const bool halo = j0 == 0;
     int la[RB];
-    float va[RB];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
       const int row0 = swave + (j0 + j) * TILE_Y;
-      const int64_t g = row_base(row0 < rows ? row0 : swave) + gx0;
-      la[j] = L[g];
-      va[j] = sdf[g];
+      la[j] = L[row_base(row0 < rows ? row0 : swave) + gx0];
     }
     const int lh = halo ? hsrc[hidx] : -1;
     int ra[RB];
@@ -167,7 +163,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
       const int row = swave + (j0 + j) * TILE_Y;
-      if (row < rows) R[row * FR_X + lane + 1] = region_word(la[j], ra[j], va[j]);
+      if (row < rows) R[row * FR_X + lane + 1] = region_word(la[j], ra[j]);
     }
     if (halo && tid < nhalo) R[hrow * FR_X + (hside ? 65 : 0)] = lh >= 0 ? rh : -2;
   }
@@ -258,7 +254,9 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
     const int v = col_in ? R[e] : -2;
     bool root = false;
     if (col_in) {
-      Lout[i] = v >= 0 ? v : -1;
+      // -2: sdf >= 0 and unlabelled, a voxel whose seg source word the tile pass or the root
+      // resolver writes (unless write_all)
+      if (write_all || v != -2) Lout[i] = v >= 0 ? v : -1;
       if (v >= 0 && ((cand0 >> zl) & 1u) && i < v) atomicMin(min_solid + v, i);
       root = v == i;
     }
@@ -276,20 +274,43 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const float* __restr
   }
 }
 
-void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
-                       int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
-                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt,
-                       const int32_t* xface) {
+void launch_flood_tile(Ctx* c, Vol v, const int32_t* labels_ccl, int32_t* labels,
+                       int32_t* min_solid, uint32_t* key_stage, uint32_t* key_cnt,
+                       uint32_t* band_stage, uint32_t* band_cnt, const int32_t* xface,
+                       bool write_all) {
   PNM_LAUNCH(c, "k_flood_tile", k_flood_tile,
              dim3(cdiv(v.nx, 64), cdiv(v.ny, TILE_Y), cdiv(v.nz, TILE_Z)), dim3(64, TILE_Y), 0,
-             sdf, labels_ccl, labels, v.nx, v.ny, v.nz, min_solid, key_stage, key_cnt, band_stage,
-             band_cnt, xface);
+             labels_ccl, labels, v.nx, v.ny, v.nz, min_solid, key_stage, key_cnt, band_stage,
+             band_cnt, xface, write_all);
+}
+
+// Marker labels with every negative code as -1 (the reference's labels after its CCL).
+__global__ void k_label_codes(const int32_t* __restrict__ L, int64_t n, int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = max(L[i], -1);
+}
+
+void launch_label_codes(Ctx* c, Vol v, const int32_t* labels, int32_t* out) {
+  PNM_LAUNCH(c, "k_label_codes", k_label_codes, dim3(cdiv(v.n, 256)), dim3(256), 0, labels, v.n,
+             out);
+}
+
+__global__ void k_merge_roots(const float* __restrict__ sdf, int64_t n,
+                              const int32_t* __restrict__ roots, int32_t* __restrict__ segsrc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && sdf[i] > 0.0f) segsrc[i] = (int32_t)((uint32_t)roots[i] | kPoreFlag);
+}
+
+void launch_merge_roots(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* segsrc) {
+  PNM_LAUNCH(c, "k_merge_roots", k_merge_roots, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
+             roots, segsrc);
 }
 
 // ---------------- flood (pore_extraction.hpp:237-269) ----------------
 // One Jacobi sweep over the current band list (count read on the device): the new label is the
 // minimum labelled neighbour of the previous state; voxels still unlabelled are staged per block
-// for the next band list.
+// for the next band list. Neighbour words are seg source words: a label, -1, or a flagged root
+// (kPoreFlag), the last two above every label as unsigned numbers.
 __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__ L, int nx,
                                                      int ny, int nz, UDiv dx, UDiv dy,
                                                      const uint32_t* __restrict__ band,
@@ -325,12 +346,12 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
           if (a == 1 && b == 1 && cc == 1) continue;
           nl[k++] = L[(int)i + oz[a] + oy[b] + ox[cc]];
         }
-    // a label is a flat index (below 2^31); -1 (unlabelled) is above every label as unsigned
+    // a label is a flat index (below 2^31); -1 (unlabelled) and flagged roots are not
     uint32_t best = ~0u;
 #pragma unroll
     for (int q = 0; q < 26; ++q) best = min(best, (uint32_t)nl[q]);
-    newlab[j] = (int)best;
-    keep = (best == ~0u);
+    keep = best >= kPoreFlag;
+    newlab[j] = keep ? -1 : (int)best;
   }
   unsigned tot;
   const unsigned pos = block_stage_pos<4>(keep, wc, wave, lane, tot);
@@ -365,57 +386,18 @@ void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uin
              newlab, count, min_solid);
 }
 
-// ---------------- roots + pore first appearance ----------------
-// The capped walk (pore_extraction.hpp:279-309) fused with relabel_min for pores (:333-334):
-// the basin's minimum starts at its peak, so only voxels before the peak issue an atomicMin, and
-// of a run of equal roots in a wave only the first lane does.
-__global__ __launch_bounds__(256) void k_roots_min(const float* __restrict__ sdf, int64_t n,
-                                                   const int32_t* __restrict__ next,
-                                                   int32_t* __restrict__ roots,
-                                                   int32_t* __restrict__ min_pore,
-                                                   unsigned long long* flags, bool do_min) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int pk = -1;
-  if (i < n) {
-    if (sdf[i] > 0.0f) {
-      int w = (int)i;
-      int s = 0;
-      for (; s < kMaxSteps; ++s) {
-        const int nx = next[w];
-        if (nx == w) break;
-        w = nx;
-      }
-      roots[i] = w;
-      if (s == kMaxSteps && next[w] != w) atomicOr(flags, 1ull);
-      pk = w;
-    } else {
-      roots[i] = (int)i;
-    }
-  }
-  if (!do_min) return;
-  const int ppk = __shfl_up(pk, 1, 64);
-  if (pk >= 0 && (int)i < pk && (lane == 0 || ppk != pk)) atomicMin(min_pore + pk, (int)i);
-}
-
-void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
-                      int32_t* min_pore, uint64_t* flag, bool do_min) {
-  PNM_LAUNCH(c, "k_roots_min", k_roots_min, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, next,
-             roots, min_pore, reinterpret_cast<unsigned long long*>(flag), do_min);
-}
-
 // Capped-walk fallback: the root set is no longer the peak set, so redo the pore minima from a
 // fully initialised array and collect the keys by a full scan.
 __global__ void k_min_pore_full(const float* __restrict__ sdf, int64_t n,
-                                const int32_t* __restrict__ roots, int32_t* min_pore) {
+                                const int32_t* __restrict__ segsrc, int32_t* min_pore) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && sdf[i] > 0.0f) atomicMin(min_pore + roots[i], (int)i);
+  if (i < n && sdf[i] > 0.0f) atomicMin(min_pore + ((uint32_t)segsrc[i] & ~kPoreFlag), (int)i);
 }
 
-void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* min_pore) {
+void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* segsrc, int32_t* min_pore) {
   fill_u32(c, reinterpret_cast<uint32_t*>(min_pore), (uint32_t)INT_MAX, v.n);
   PNM_LAUNCH(c, "k_min_pore_full", k_min_pore_full, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n,
-             roots, min_pore);
+             segsrc, min_pore);
 }
 
 __global__ void k_collect_keys(const float* __restrict__ sdf, int64_t n,
@@ -595,30 +577,27 @@ void assign_ids_bitmap(Ctx* c, const float* sdf, Vol v, const uint32_t* pkeys, i
              reinterpret_cast<unsigned long long*>(counts_dev));
 }
 
-// relabel_seg (pore_extraction.hpp:378-381).
-__global__ __launch_bounds__(256) void k_seg(const float* __restrict__ sdf, int64_t n,
-                                             const int32_t* __restrict__ roots,
-                                             const int32_t* __restrict__ labels,
+// relabel_seg (pore_extraction.hpp:378-381) from the seg source words: a flagged root takes its
+// pore id, a label its solid id, -1 gives 0.
+__global__ __launch_bounds__(256) void k_seg(int64_t n, const int32_t* __restrict__ segsrc,
                                              const int32_t* __restrict__ pore_id,
                                              const int32_t* __restrict__ solid_id,
                                              int32_t* __restrict__ seg) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float cv = sdf[i];
-  int32_t s;
-  if (cv > 0.0f) {
-    s = pore_id[roots[i]];
-  } else {
-    const int32_t l = labels[i];
-    s = (l == -1) ? 0 : solid_id[l];
-  }
+  const int32_t k = segsrc[i];
+  int32_t s = 0;
+  if (k < -1)
+    s = pore_id[(uint32_t)k & ~kPoreFlag];
+  else if (k >= 0)
+    s = solid_id[k];
   seg[i] = s;
 }
 
-void launch_seg(Ctx* c, const float* sdf, Vol v, const int32_t* roots, const int32_t* labels,
-                const int32_t* pore_id, const int32_t* solid_id, int32_t* seg) {
-  PNM_LAUNCH(c, "k_seg", k_seg, dim3(cdiv(v.n, 256)), dim3(256), 0, sdf, v.n, roots,
-                     labels, pore_id, solid_id, seg);
+void launch_seg(Ctx* c, Vol v, const int32_t* segsrc, const int32_t* pore_id,
+                const int32_t* solid_id, int32_t* seg) {
+  PNM_LAUNCH(c, "k_seg", k_seg, dim3(cdiv(v.n, 256)), dim3(256), 0, v.n, segsrc, pore_id,
+             solid_id, seg);
 }
 
 }  // namespace pnm

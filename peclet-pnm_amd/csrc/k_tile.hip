@@ -237,7 +237,8 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   load_slice(z0 + 2, A0, A1);
   uint64_t m_cur = slice_max(z0, B0, B1, c_cur, d_cur);
   load_slice(z0 + 3, B0, B1);
-  uint32_t pore_mask = 0, peak_mask = 0;  // this thread's sdf>0 slices and peak slices
+  // this thread's sdf>0 slices, peak slices, sdf == 0 slices and sdf >= 0 slices
+  uint32_t pore_mask = 0, peak_mask = 0, zero_mask = 0, nonneg_mask = 0;
   // one slice of the march: needs slice k = zl + 2 (values va, vb)
   auto march = [&](int zl, float va, float vb) {
     const int z = z0 + zl;
@@ -262,6 +263,8 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
         }
       }
     }
+    if (col_in && cv == 0.0f) zero_mask |= 1u << zl;
+    if (col_in && cv >= 0.0f) nonneg_mask |= 1u << zl;
     const bool mk = col_in && (cv < thr);
     const uint64_t mm = __ballot(mk);
     const uint64_t starts = mm & ~(mm << 1);
@@ -359,8 +362,10 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const int l = (zl * TILE_Y + wave) * 64 + lane;
       const int i = srow + lane + mul(zl, plane_n);  // = gidx(l)
       if (col_in) {
-        o.labels[i] = r[zl] == -1 ? -1 : gidx(r[zl]);
+        // a voxel that is no marker: -1 if it is a flood candidate (!(sdf >= 0)), else -2
+        o.labels[i] = r[zl] != -1 ? gidx(r[zl]) : (((nonneg_mask >> zl) & 1u) ? -2 : -1);
         if (r[zl] == l) o.min_solid[i] = i;
+        if ((zero_mask >> zl) & 1u) o.segsrc[i] = -1;  // never a pore, never flooded
       }
       const bool lroot = col_in && r[zl] == l;
       const uint64_t rm = __ballot(lroot);
@@ -547,7 +552,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       }
     }
     if (fin) {
-      o.roots[i] = target;
+      o.roots[i] = (int)((uint32_t)target | o.root_flag);
       if (!capped) atomicMin(&LAB[fin_u], l);
       capped_any |= capped;
     }
@@ -756,10 +761,11 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
     }
     R.my = ok && in ? L[i] : -1;
   };
+  // label words: a marker's label (>= 0) or a negative code (-1, -2) for every other voxel
   auto pairs = [&](const Row& R) {
     if (!R.ok) return;
     const int my = R.my;
-    const bool mk = my != -1;
+    const bool mk = my >= 0;
     const uint64_t m = __ballot(mk);
     if (m == 0) return;
     const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
@@ -768,7 +774,7 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
     for (int r = 0; r < 4; ++r) {
       if (R.same[r]) continue;  // wave-uniform; edges of same-tile rows: the x-face pass
       const int c = in ? R.fwd[r] : -1;
-      const bool cm = c != -1;
+      const bool cm = c >= 0;
       const int prev_c = __shfl_up(c, 1, 64);
       const int next_c = __shfl_down(c, 1, 64);
       if (mk && cm) {
@@ -778,13 +784,13 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
       if (mk && !cm) {
         if (run_start) {
           const int a = lane == 0 ? R.fe[r] : prev_c;
-          if (a != -1) seam_pair(tab, L, my, a);
+          if (a >= 0) seam_pair(tab, L, my, a);
         }
         if (run_end) {
           const int b = x != xlast ? next_c
                                    : (lane != 0 ? R.fe[r]
                                                 : L[(R.fz[r] * ny + R.fy[r]) * nx + wp1(x, nx)]);
-          if (b != -1) seam_pair(tab, L, my, b);
+          if (b >= 0) seam_pair(tab, L, my, b);
         }
       }
     }
@@ -910,6 +916,7 @@ __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__
                                                        int64_t nseg, int seg_cap,
                                                        const uint64_t* __restrict__ walk,
                                                        int32_t* __restrict__ roots,
+                                                       uint32_t root_flag,
                                                        int32_t* __restrict__ min_pore,
                                                        unsigned long long* cap_flag) {
   constexpr int K = RESOLVE_WALKS;
@@ -1006,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (root[k] >= 0) roots[vox[k]] = root[k];
+      if (root[k] >= 0) roots[vox[k]] = (int)((uint32_t)root[k] | root_flag);
       // of two neighbouring lanes with one root, the one with the larger voxel index skips
       const int prev = __shfl_up(root[k], 1, 64), prev_i = __shfl_up(vox[k], 1, 64);
       if (root[k] >= 0 && vox[k] < root[k] && (lane == 0 || prev != root[k] || prev_i > vox[k]))
@@ -1047,10 +1054,11 @@ void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
 
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint64_t* stage,
                           const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
-                          int32_t* roots, int32_t* min_pore, uint64_t* cap_flag) {
+                          int32_t* roots, uint32_t root_flag, int32_t* min_pore,
+                          uint64_t* cap_flag) {
   if (nseg <= 0) return;
   PNM_LAUNCH(c, "k_roots_resolve", k_roots_resolve, dim3(cdiv(nseg, 4)), dim3(256), 0, sdf, v.nx,
-             v.ny, v.nz, v.n, stage, cnt, nseg, seg_cap, walk, roots, min_pore,
+             v.ny, v.nz, v.n, stage, cnt, nseg, seg_cap, walk, roots, root_flag, min_pore,
              reinterpret_cast<unsigned long long*>(cap_flag));
 }
 

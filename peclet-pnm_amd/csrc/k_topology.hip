@@ -32,13 +32,11 @@ __device__ __forceinline__ uint32_t slot_hash(uint64_t k) {
 // previous lane's key for the same direction skips (rows along a boundary repeat the pair), the
 // rest go through an LDS hash set; the set is staged per block (no global counter). Keys that
 // find no slot within the probe limit are appended to a small overflow list.
-// Source of the labels: a finished seg array, or (SegSrc::ids) the segmentation's last step
-// fused in — relabel_seg (:378-381) computed for the brick and its +x/+y/+z halo into LDS, the
-// brick's part written out as seg.
+// Source of the labels: a finished seg array, or (SegIds) the segmentation's last step fused in
+// — relabel_seg (:378-381) computed from the seg source words (kPoreFlag) for the brick and its
+// +x/+y/+z halo into LDS, the brick's part written out as seg.
 struct SegIds {
-  const float* sdf;
-  const int32_t* roots;
-  const int32_t* labels;
+  const int32_t* segsrc;
   const int32_t* pore_id;
   const int32_t* solid_id;
   int32_t* seg_out;
@@ -77,20 +75,15 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
     // is one more chain on threads 0..84, issued with the first batch. Positions past the volume
     // hold the wrapped voxel: the +x/+y/+z neighbour of the volume's last voxel is read there.
     // Brick voxels inside the volume also write seg.
-    // Each value's sign picks the label source (root of an sdf>0 voxel, flagged in the top bit,
-    // or the solid label), the label gives the id. Loads are issued in branch-free batches of RB
-    // rows into registers (rows past the region re-read a valid row). Row offsets come from the
-    // table (flat offsets fit 32 bits: volumes hold fewer than 2^31 voxels).
+    // Each seg source word gives the id (pore id of a flagged root, solid id of a label, 0 for
+    // -1). Loads are issued in branch-free batches of RB rows into registers (rows past the
+    // region re-read a valid row). Row offsets come from the table (flat offsets fit 32 bits:
+    // volumes hold fewer than 2^31 voxels).
     constexpr int RB = 11, MAXR = (NROW + 3) / 4;
     const int gx = wrap_any(x0 + lane, nx), gxh = wrap_any(x0 + 64, nx);
     auto row_base = [&](int row) -> int64_t { return (int64_t)(RBASE[row] & 0x7fffffffu); };
-    auto label_of = [&](float v, int64_t g) {
-      const int32_t* src = v > 0.0f ? ids.roots : ids.labels;
-      const int r = src[g];
-      return v > 0.0f ? (int)((uint32_t)r | 0x80000000u) : r;
-    };
     auto id_of = [&](int k) {
-      const int a = k < -1 ? ids.pore_id[(uint32_t)k & 0x7fffffffu]
+      const int a = k < -1 ? ids.pore_id[(uint32_t)k & ~kPoreFlag]
                            : ids.solid_id[k >= 0 ? k : 0];
       return k == -1 ? 0 : a;
     };
@@ -100,19 +93,15 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
     for (int j0 = 0; j0 < MAXR; j0 += RB) {
       const bool halo = j0 == 0;
       uint32_t rb[RB];  // the rows' table words (offset, brick flag)
-      float v[RB];
+      int k[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         const int row0 = swave + (j0 + j) * 4;
         rb[j] = RBASE[row0 < NROW ? row0 : swave];
-        v[j] = ids.sdf[(rb[j] & 0x7fffffffu) + (uint32_t)gx];
+        k[j] = ids.segsrc[(rb[j] & 0x7fffffffu) + (uint32_t)gx];
       }
-      const int64_t gh = row_base(hrow) + gxh;
-      const float vh = ids.sdf[gh];  // both batches load it (branch-free; cached the second time)
-      int k[RB];
-#pragma unroll
-      for (int j = 0; j < RB; ++j) k[j] = label_of(v[j], (rb[j] & 0x7fffffffu) + (uint32_t)gx);
-      const int kh = label_of(vh, gh);
+      // both batches load it (branch-free; cached the second time)
+      const int kh = ids.segsrc[row_base(hrow) + gxh];
 #pragma unroll
       for (int j = 0; j < RB; ++j) k[j] = id_of(k[j]);
       const int sh = id_of(kh);
@@ -293,11 +282,10 @@ int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi
                        needed);
 }
 
-int64_t seg_topology(Ctx* c, const float* sdf, const int32_t* roots, const int32_t* labels,
-                     const int32_t* pore_id, const int32_t* solid_id, int32_t* seg, Vol v,
-                     int32_t lo_label, int32_t hi_label, int32_t* pairs_d, int64_t cap,
-                     int64_t* needed) {
-  const SegIds ids{sdf, roots, labels, pore_id, solid_id, seg};
+int64_t seg_topology(Ctx* c, const int32_t* segsrc, const int32_t* pore_id,
+                     const int32_t* solid_id, int32_t* seg, Vol v, int32_t lo_label,
+                     int32_t hi_label, int32_t* pairs_d, int64_t cap, int64_t* needed) {
+  const SegIds ids{segsrc, pore_id, solid_id, seg};
   return topology_impl(c, nullptr, &ids, v, lo_label, hi_label, true, pairs_d, cap, needed);
 }
 

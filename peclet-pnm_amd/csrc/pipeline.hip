@@ -59,8 +59,7 @@ struct SegInternal {
   uint32_t* peaks;  // peak list (extract_pores predicate) in ascending voxel order
   int64_t npeaks;
   // inputs of relabel_seg, for a caller that fuses it (seg_topology)
-  const int32_t* roots;
-  const int32_t* labels;
+  const int32_t* segsrc;
   const int32_t* pore_id;
   const int32_t* solid_id;
 };
@@ -77,8 +76,16 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   const float thr = -1.5f * mn;  // pore_extraction.hpp:163-164
   const int64_t nseg = tile_segments(v), scap = tile_segment_cap();
   const int64_t nb1 = cdiv(n, 256);
-  int32_t* labels = labels_out ? labels_out : c->arena.get_n<int32_t>((size_t)n);
-  int32_t* roots = roots_out ? roots_out : c->arena.get_n<int32_t>((size_t)n);
+  // One seg source word per voxel (kPoreFlag): the tile pass and the root resolver write the
+  // flagged ascent roots of sdf>0 voxels into it, the flood the labels of the others. Separate
+  // arrays instead, merged after the flood, when markers can be sdf>0 voxels (thr > 0: their
+  // label and their root are both needed), when the caller wants the unflagged roots or the
+  // labels of every voxel after the flood.
+  const bool split = thr > 0.0f || labels_only || roots_out != nullptr;
+  int32_t* segsrc = labels_out ? labels_out : c->arena.get_n<int32_t>((size_t)n);
+  int32_t* roots =
+      !split ? segsrc : (roots_out ? roots_out : c->arena.get_n<int32_t>((size_t)n));
+  const uint32_t root_flag = split ? 0u : kPoreFlag;
   uint64_t* walk = c->arena.get_n<uint64_t>((size_t)n);
   int32_t* min_pore = c->arena.get_n<int32_t>((size_t)n);
   int32_t* min_solid = c->arena.get_n<int32_t>((size_t)n);
@@ -109,6 +116,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   TileOut o{};
   o.labels = lab_ccl;
   o.roots = roots;
+  o.root_flag = root_flag;
+  o.segsrc = segsrc;
   o.walk = walk;
   o.min_pore = min_pore;
   o.min_solid = min_solid;
@@ -125,25 +134,25 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   launch_tile(c, sdf, v, thr, o);
   bitmap_list(c, peak_map, nwords, nwx, v.nx, peaks, ctr + 0);  // peaks in voxel order
   // the walks that left their tile, read where the tile pass staged them. The resolver shares
-  // no buffer with the CCL and flood below (it writes roots, min_pore and the cap-flag word
-  // ctr[3]) and no look-back scan state (a scan on each stream at once would race on it), so
-  // it runs beside them on the second stream, joined before the counters are read back (so its
-  // outputs are complete when the call returns, as every output is).
+  // no data with the CCL and flood below (it writes the root words of sdf>0 voxels, which the
+  // flood neither reads as labels nor writes, min_pore and the cap-flag word ctr[3]) and no
+  // look-back scan state (a scan on each stream at once would race on it), so it runs beside
+  // them on the second stream, joined before the counters are read back (so its outputs are
+  // complete when the call returns, as every output is).
   if (!labels_only) {
     OnSide side(c);
-    launch_roots_resolve(c, sdf, v, stage_o, cnt_c, nseg, (int)scap, walk, roots, min_pore,
-                         ctr + 3);
+    launch_roots_resolve(c, sdf, v, stage_o, cnt_c, nseg, (int)scap, walk, roots, root_flag,
+                         min_pore, ctr + 3);
   }
   launch_ccl_seams(c, v, lab_ccl, xface);
   launch_lroot_flatten(c, v, lab_ccl, stage_r, cnt_r);
   if (ccl_out) {  // stage-level output: the flattened CCL labels before the flood
     launch_ccl_flatten(c, v, lab_ccl, min_solid, stage_b, cnt_b);
-    PNM_HIP(hipMemcpyAsync(ccl_out, lab_ccl, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
-                           c->stream));
+    launch_label_codes(c, v, lab_ccl, ccl_out);
   }
   // flatten + the first flood sweeps per tile; the band left over goes to the global sweeps
-  launch_flood_tile(c, sdf, v, lab_ccl, labels, min_solid, stage_b, cnt_pair2, stage_c,
-                    cnt_pair2 + nseg, xface);
+  launch_flood_tile(c, v, lab_ccl, segsrc, min_solid, stage_b, cnt_pair2, stage_c,
+                    cnt_pair2 + nseg, xface, split);
   {
     const uint32_t* st[2] = {stage_b, stage_c};
     uint32_t* outs[2] = {skeys, band};
@@ -171,7 +180,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
     while (true) {
       c->arena.rewind(sweep_mark);
       for (int k = 0; k < 2; ++k, ++s) {
-        flood_sweep(c, v, labels, cur, ctr + 8 + (s & 3), grid, nxt, ctr + 8 + ((s + 1) & 3),
+        flood_sweep(c, v, segsrc, cur, ctr + 8 + (s & 3), grid, nxt, ctr + 8 + ((s + 1) & 3),
                     newlab, stage_b, cnt_b, min_solid);
         std::swap(cur, nxt);
       }
@@ -187,18 +196,19 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
 
   SegInternal out{};
   if (labels_only) return out;
+  if (split) launch_merge_roots(c, sdf, v, roots, segsrc);
   if (!have_flag) readback(c, ctr + 3, 1, h);
   if (h[0] & 2u) fail(PNM_EHIP, "internal error: ascent walk record missing");
   const uint32_t* pkeys = peaks;
   int64_t npk = npeaks;
   if (h[0] & 1u) {  // a walk stopped at the 512-step cap on a non-fixed point
-    launch_min_full(c, sdf, v, roots, min_pore);
+    launch_min_full(c, sdf, v, segsrc, min_pore);
     uint32_t* full = c->arena.get_n<uint32_t>((size_t)n);
     npk = collect_keys_full(c, sdf, v, min_pore, full, ctr + 6);
     pkeys = full;
   }
   assign_ids_bitmap(c, sdf, v, pkeys, npk, min_pore, skeys, nskeys, min_solid, ctr + 4);
-  if (seg) launch_seg(c, sdf, v, roots, labels, min_pore, min_solid, seg);
+  if (seg) launch_seg(c, v, segsrc, min_pore, min_solid, seg);
   if (roots_out) launch_descent_roots(c, sdf, v, roots_out);  // rootsOut of sdf <= 0 voxels
   if (read_counts) {
     readback(c, ctr + 4, 2, h);
@@ -209,8 +219,7 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   out.solid_count = nskeys;
   out.peaks = peaks;
   out.npeaks = npeaks;
-  out.roots = roots;
-  out.labels = labels;
+  out.segsrc = segsrc;
   out.pore_id = min_pore;
   out.solid_id = min_solid;
   return out;
@@ -242,8 +251,8 @@ int64_t network_device(Ctx* c, const float* sdf, Vol v, const float org[3], cons
   const int64_t np = finish_pores(c, sdf, v, org, spc, s.peaks, s.npeaks, true, pores_d,
                                   cap_pores, total_pores);
   int64_t needed = 0;
-  const int64_t nt = seg_topology(c, sdf, s.roots, s.labels, s.pore_id, s.solid_id, seg, v,
-                                  -s.solid_count, s.pore_bound, pairs_d, cap_pairs, &needed);
+  const int64_t nt = seg_topology(c, s.segsrc, s.pore_id, s.solid_id, seg, v, -s.solid_count,
+                                  s.pore_bound, pairs_d, cap_pairs, &needed);
   *npairs = needed;
   *pairs_overflow = nt < 0;
   return np;

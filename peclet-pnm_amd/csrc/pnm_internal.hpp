@@ -226,10 +226,24 @@ void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const
 
 // CCL of markers: x-run unions + flatten; flatten stages the component roots (solid keys) per
 // 256-voxel block and sets min_solid[root] = root.
+// Seg source word of a voxel, the one int32 relabel_seg (:378-381) needs per voxel: for an sdf>0
+// voxel its ascent root with the top bit set (root | kPoreFlag; a root is a flat index below
+// 2^31 - 1, so the word is never -1), for every other voxel its marker label after the flood (a
+// flat index >= 0) or -1. As unsigned numbers root words and -1 both lie above every label, so
+// the unsigned minimum of neighbour words is the smallest label among them, if there is one.
+constexpr uint32_t kPoreFlag = 0x80000000u;
+
 // Segmentation tile pass (k_tile.hip).
 struct TileOut {
-  int32_t* labels;        // marker label = flat index of its tile-local CCL root, or -1 [n]
-  int32_t* roots;         // roots of sdf>0 voxels whose walk ends inside their tile    [n]
+  // marker label = flat index of its tile-local CCL root; for a voxel that is no marker -1 if
+  // !(sdf >= 0) (a flood candidate), else -2                                             [n]
+  int32_t* labels;
+  // Ascent roots of sdf>0 voxels whose walk ends inside their tile, as root | root_flag [n].
+  // With root_flag = kPoreFlag this is the seg source array (see kPoreFlag), which the flood
+  // fills in for the other voxels; the tile pass writes -1 into `segsrc` at sdf == 0 voxels.
+  int32_t* roots;
+  uint32_t root_flag;
+  int32_t* segsrc;
   uint64_t* walk;         // walk records: sdf>0 voxels on tile boundaries or left open [n]
   int32_t* min_pore;      // first-appearance minimum over the tile, at in-tile roots   [n]
   int32_t* min_solid;     // seeded with its own index at every tile-local marker root  [n]
@@ -254,16 +268,23 @@ void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
 // counts in cnt): one wave per segment.
 void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint64_t* stage,
                           const uint32_t* cnt, int64_t nseg, int seg_cap, const uint64_t* walk,
-                          int32_t* roots, int32_t* min_pore, uint64_t* cap_flag);
+                          int32_t* roots, uint32_t root_flag, int32_t* min_pore,
+                          uint64_t* cap_flag);
 void launch_ccl_flatten(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                         uint32_t* key_cnt);
 // Flatten + the first flood sweep(s) per tile: reads the CCL labels (tile-local roots whose
 // own entries already hold their component root), writes the flooded labels to a second array
 // (segments as tile_segments / tile_segment_cap). xface: the tile pass's x-face labels.
-void launch_flood_tile(Ctx* c, const float* sdf, Vol v, const int32_t* labels_ccl,
-                       int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
-                       uint32_t* key_cnt, uint32_t* band_stage, uint32_t* band_cnt,
-                       const int32_t* xface);
+// write_all = false: voxels that are neither labelled nor flood candidates (sdf >= 0) are not
+// written (the seg source array holds their ascent roots, or -1 at sdf == 0, already).
+void launch_flood_tile(Ctx* c, Vol v, const int32_t* labels_ccl, int32_t* labels,
+                       int32_t* min_solid, uint32_t* key_stage, uint32_t* key_cnt,
+                       uint32_t* band_stage, uint32_t* band_cnt, const int32_t* xface,
+                       bool write_all);
+// out[i] = max(labels[i], -1): the tile pass's CCL labels with the reference's -1.
+void launch_label_codes(Ctx* c, Vol v, const int32_t* labels, int32_t* out);
+// segsrc[i] = roots[i] | kPoreFlag at every sdf>0 voxel (the others keep their label).
+void launch_merge_roots(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* segsrc);
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                 uint32_t* key_cnt);
 // One Jacobi flood sweep over the band list whose length is the device word *count (grid sized
@@ -271,12 +292,8 @@ void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* ke
 void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uint64_t* count,
                  int64_t grid_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
                  uint32_t* stage, uint32_t* stage_cnt, int32_t* min_solid);
-// Roots by chasing `next` (cap 512) for sdf>0 voxels + pore first-appearance atomicMin; *flag is
-// set when a walk stopped at the cap on a non-fixed point.
-void launch_roots_min(Ctx* c, const float* sdf, Vol v, const int32_t* next, int32_t* roots,
-                      int32_t* min_pore, uint64_t* flag, bool do_min);
-// Capped-walk fallback: full reinitialisation of the pore minima.
-void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* min_pore);
+// Capped-walk fallback: full reinitialisation of the pore minima from the seg source array.
+void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* segsrc, int32_t* min_pore);
 int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_pore,
                           uint32_t* key_list, uint64_t* counter);
 // Ranks keys by their minimum voxel; writes ids into id_arr[key] (pores: rank+1; solids:
@@ -288,8 +305,8 @@ void assign_ids_bitmap(Ctx* c, const float* sdf, Vol v, const uint32_t* pkeys, i
                        uint64_t* counts_dev);
 void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, int32_t* id_arr,
                 bool solid, uint64_t* count_dev);
-void launch_seg(Ctx* c, const float* sdf, Vol v, const int32_t* roots, const int32_t* labels,
-                const int32_t* pore_id, const int32_t* solid_id, int32_t* seg);
+void launch_seg(Ctx* c, Vol v, const int32_t* segsrc, const int32_t* pore_id,
+                const int32_t* solid_id, int32_t* seg);
 // Gradient-walk successors and capped chases, computed stand-alone. Modes: kWalkPores = the
 // reference's step 4 (ascent from sdf>0, descent from the rest), kWalkAll = ascent from every
 // voxel (flow basins), kWalkSolids = descent from !(sdf>0) voxels only (chase writes only those).
@@ -306,10 +323,9 @@ int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi
                  bool labels_known, int32_t* pairs_d, int64_t cap, int64_t* needed);
 // relabel_seg fused into the topology candidates: writes seg and returns the pair count as
 // topology() does.
-int64_t seg_topology(Ctx* c, const float* sdf, const int32_t* roots, const int32_t* labels,
-                     const int32_t* pore_id, const int32_t* solid_id, int32_t* seg, Vol v,
-                     int32_t lo_label, int32_t hi_label, int32_t* pairs_d, int64_t cap,
-                     int64_t* needed);
+int64_t seg_topology(Ctx* c, const int32_t* segsrc, const int32_t* pore_id,
+                     const int32_t* solid_id, int32_t* seg, Vol v, int32_t lo_label,
+                     int32_t hi_label, int32_t* pairs_d, int64_t cap, int64_t* needed);
 
 // ---------- pipelines (pipeline.hip) ----------
 struct SegResult {
