@@ -153,11 +153,22 @@ __global__ void k_centroids(const float* __restrict__ sdf, int nx, int ny, int n
   const int xs[3] = {wm1(ix, nx), ix, wp1(ix, nx)};
   const int ys[3] = {wm1(iy, ny), iy, wp1(iy, ny)};
   const int zs[3] = {wm1(iz, nz), iz, wp1(iz, nz)};
-  float sw = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+  // the 27 values are loaded before any is used, so that all loads are in flight together
+  float vals[27];
+#pragma unroll
   for (int a = 0; a < 3; ++a)
+#pragma unroll
     for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) vals[(a * 3 + b) * 3 + cc] = sdf[(zs[a] * ny + ys[b]) * nx + xs[cc]];
+  float sw = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
-        const float val = sdf[(zs[a] * ny + ys[b]) * nx + xs[cc]];
+        const float val = vals[(a * 3 + b) * 3 + cc];
         float w = val > 0.0f ? val : 0.0f;
         w = w * w;
         sw += w;

@@ -426,92 +426,23 @@ int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_po
   return (int64_t)h;
 }
 
-// ---------------- ids (relabel_first/scan/assign, pore_extraction.hpp:340-374) ----------------
-// sortkey = the key's first-appearance voxel; pore keys that are not sdf>0 (NaN peaks of the
-// extract_pores predicate) sort last with 0xFFFFFFFF and get no id.
-__global__ void k_gather_first(const float* __restrict__ sdf, const uint32_t* __restrict__ keys,
-                               int64_t nkeys, const int32_t* __restrict__ mins, bool solid,
-                               uint32_t* __restrict__ sk, uint32_t* __restrict__ sv) {
+// ---------------- ids by first appearance (relabel_first/scans/assign, :340-374) ----------------
+// Pores: every key sets the bit of its first-appearance voxel in an n-bit map, a rank directory
+// of the map (map_rank) turns a bit into its rank, and a key's id is its rank + 1; pore keys that
+// are not sdf>0 (NaN peaks of the extract_pores predicate) get no id. Solids: a key's first
+// voxel with sdf <= 0 gives id -(rank + 1) among those voxels, any other solid key id 0. A volume
+// has few solid components, so up to SOLID_RANK_MAX of them are ranked by direct comparison of
+// their first voxels in one block; more go through a map of their own like the pores.
+constexpr int SOLID_RANK_MAX = 4096;
+
+__global__ void k_mark_first(const float* __restrict__ sdf, const uint32_t* __restrict__ keys,
+                             int64_t nk, const int32_t* __restrict__ mins, bool solid,
+                             uint32_t* __restrict__ map) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nkeys) return;
+  if (j >= nk) return;
   const uint32_t r = keys[j];
-  const bool valid = solid || sdf[r] > 0.0f;
-  sk[j] = valid ? (uint32_t)mins[r] : 0xFFFFFFFFu;
-  sv[j] = r;
-}
-
-// Solid ranks only count labels whose first voxel has sdf <= 0 (the relabel_first predicate,
-// :343-345; a NaN first voxel leaves the label's id at 0).
-__global__ void k_solid_flags(const float* __restrict__ sdf, const uint32_t* __restrict__ sk,
-                              int64_t nkeys, uint32_t* __restrict__ flag) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nkeys) return;
-  flag[j] = (sdf[sk[j]] <= 0.0f) ? 1u : 0u;
-}
-
-__global__ void k_assign(const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv,
-                         const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank,
-                         int64_t nkeys, bool solid, int32_t* __restrict__ ids,
-                         unsigned long long* __restrict__ count) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nkeys) return;
-  const uint32_t r = sv[j];
-  if (solid) {
-    const bool v = flag[j] != 0;
-    ids[r] = v ? -(int32_t)(rank[j] + 1) : 0;
-    if (j == nkeys - 1) *count = rank[j] + (v ? 1u : 0u);
-  } else {
-    const bool v = sk[j] != 0xFFFFFFFFu;
-    if (v) ids[r] = (int32_t)(j + 1);
-    if (v && (j == nkeys - 1 || sk[j + 1] == 0xFFFFFFFFu)) *count = (unsigned long long)(j + 1);
-  }
-}
-
-// Writes the number of ids assigned into *count_dev (no synchronisation).
-void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, int32_t* id_arr,
-                bool solid, uint64_t* count_dev) {
-  fill_u64(c, count_dev, 0, 1);
-  if (nkeys <= 0) return;
-  uint32_t* sk = c->arena.get_n<uint32_t>((size_t)nkeys);
-  uint32_t* sv = c->arena.get_n<uint32_t>((size_t)nkeys);
-  uint32_t* flag = c->arena.get_n<uint32_t>((size_t)nkeys);
-  uint32_t* rank = c->arena.get_n<uint32_t>((size_t)nkeys);
-  const unsigned nb = cdiv(nkeys, 256);
-  PNM_LAUNCH(c, "k_gather_first", k_gather_first, dim3(nb), dim3(256), 0, sdf, keys, nkeys,
-             (const int32_t*)id_arr, solid, sk, sv);
-  radix_sort_u32(c, sk, sv, nkeys, 0, 32);
-  if (solid) {
-    PNM_LAUNCH(c, "k_solid_flags", k_solid_flags, dim3(nb), dim3(256), 0, sdf, sk, nkeys, flag);
-    scan_exclusive_u32(c, flag, rank, nkeys);
-  }
-  PNM_LAUNCH(c, "k_assign", k_assign, dim3(nb), dim3(256), 0, sk, sv, flag, rank, nkeys, solid,
-             id_arr, reinterpret_cast<unsigned long long*>(count_dev));
-}
-
-// ---------------- ids by first-appearance bitmaps ----------------
-// relabel_first + the two exclusive scans + assign (:340-374) without sorting the keys: every
-// key sets the bit of its first-appearance voxel in an n-bit map (pores and solids in separate
-// maps), an exclusive scan of the per-word popcounts turns a bit into its rank, and a key's id
-// is its rank + 1 (pores) or -(rank + 1) (solids whose first voxel has sdf <= 0; other solid
-// keys get id 0, and pore keys that are not sdf>0 — NaN peaks of the extract_pores predicate —
-// get no id).
-__global__ void k_mark_first(const float* __restrict__ sdf, const uint32_t* __restrict__ pkeys,
-                             int64_t npk, const int32_t* __restrict__ pmin,
-                             const uint32_t* __restrict__ skeys, int64_t nsk,
-                             const int32_t* __restrict__ smin, uint32_t* __restrict__ pmap,
-                             uint32_t* __restrict__ smap) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < npk) {
-    const uint32_t r = pkeys[j];
-    if (sdf[r] > 0.0f) {
-      const uint32_t f = (uint32_t)pmin[r];
-      atomicOr(pmap + (f >> 5), 1u << (f & 31));
-    }
-  } else if (j - npk < nsk) {
-    const uint32_t r = skeys[j - npk];
-    const uint32_t f = (uint32_t)smin[r];
-    if (sdf[f] <= 0.0f) atomicOr(smap + (f >> 5), 1u << (f & 31));
-  }
+  const uint32_t f = (uint32_t)mins[r];
+  if (solid ? sdf[f] <= 0.0f : sdf[r] > 0.0f) atomicOr(map + (f >> 5), 1u << (f & 31));
 }
 
 // Rank of bit f in a bitmap with a rank directory (map_rank): the chunk's prefix, the words of
@@ -524,57 +455,78 @@ __device__ __forceinline__ uint32_t map_rank_of(const uint32_t* __restrict__ map
   return r + (uint32_t)__popc(map[w] & ((1u << (f & 31)) - 1u));
 }
 
-// Ranks both key sets; the solid map starts at word wpad (a chunk boundary), so its ranks are
-// offset by the number of pore bits, crank[wpad / MAP_CHUNK_WORDS].
-__global__ void k_assign_ranks(const float* __restrict__ sdf, const uint32_t* __restrict__ pkeys,
-                               int64_t npk, const uint32_t* __restrict__ skeys, int64_t nsk,
-                               int32_t* __restrict__ pid, int32_t* __restrict__ sid,
+// ids[key] still holds the key's first-appearance voxel and becomes its id.
+__global__ void k_assign_ranks(const float* __restrict__ sdf, const uint32_t* __restrict__ keys,
+                               int64_t nk, int32_t* __restrict__ ids, bool solid,
                                const uint32_t* __restrict__ map,
-                               const uint32_t* __restrict__ crank, int64_t wpad,
-                               unsigned long long* __restrict__ counts) {
+                               const uint32_t* __restrict__ crank) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t npore = crank[wpad / MAP_CHUNK_WORDS];
-  if (j == 0) {  // counts[1] holds the bit total of both maps
-    const unsigned long long all = counts[1];
-    counts[0] = npore;
-    counts[1] = all - npore;
-  }
-  if (j < npk) {
-    const uint32_t r = pkeys[j];
-    if (sdf[r] > 0.0f) {
-      const uint32_t f = (uint32_t)pid[r];  // still the first-appearance voxel
-      pid[r] = (int32_t)(map_rank_of(map, crank, f) + 1);
-    }
-  } else if (j - npk < nsk) {
-    const uint32_t r = skeys[j - npk];
-    const uint32_t f = (uint32_t)sid[r];
-    if (sdf[f] <= 0.0f) {
-      const uint32_t rk = map_rank_of(map, crank, (wpad << 5) + f) - npore;
-      sid[r] = -(int32_t)(rk + 1);
-    } else {
-      sid[r] = 0;
-    }
+  if (j >= nk) return;
+  const uint32_t r = keys[j];
+  const uint32_t f = (uint32_t)ids[r];
+  if (!solid) {
+    if (sdf[r] > 0.0f) ids[r] = (int32_t)(map_rank_of(map, crank, f) + 1);
+  } else {
+    ids[r] = sdf[f] <= 0.0f ? -(int32_t)(map_rank_of(map, crank, f) + 1) : 0;
   }
 }
 
-// Both key sets at once; counts_dev[0] = number of pore ids, counts_dev[1] = solid ids.
+// All solid keys in one block: every first voxel is read before any id is written.
+__global__ __launch_bounds__(1024) void k_solid_ranks(const float* __restrict__ sdf,
+                                                      const uint32_t* __restrict__ keys,
+                                                      int64_t nk, int32_t* __restrict__ ids,
+                                                      unsigned long long* __restrict__ count) {
+  __shared__ uint32_t F[SOLID_RANK_MAX];  // first voxel of a key with sdf <= 0, else ~0
+  __shared__ unsigned nvalid;
+  if (threadIdx.x == 0) nvalid = 0;
+  __syncthreads();
+  const int n = (int)nk;
+  unsigned mine = 0;
+  for (int j = threadIdx.x; j < n; j += 1024) {
+    const uint32_t f = (uint32_t)ids[keys[j]];
+    const bool ok = sdf[f] <= 0.0f;
+    F[j] = ok ? f : ~0u;
+    mine += ok ? 1u : 0u;
+  }
+  atomicAdd(&nvalid, mine);
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += 1024) {
+    const uint32_t f = F[j];
+    int32_t id = 0;
+    if (f != ~0u) {  // first voxels are distinct (a voxel carries one label)
+      uint32_t r = 0;
+      for (int q = 0; q < n; ++q) r += F[q] < f ? 1u : 0u;
+      id = -(int32_t)(r + 1);
+    }
+    ids[keys[j]] = id;
+  }
+  if (threadIdx.x == 0) *count = nvalid;
+}
+
+// Both key sets; counts_dev[0] = number of pore ids, counts_dev[1] = solid ids (no
+// synchronisation).
 void assign_ids_bitmap(Ctx* c, const float* sdf, Vol v, const uint32_t* pkeys, int64_t npk,
                        int32_t* pore_ids, const uint32_t* skeys, int64_t nsk, int32_t* solid_ids,
                        uint64_t* counts_dev) {
   const int64_t words = (v.n + 31) / 32;
-  const int64_t wpad = (words + MAP_CHUNK_WORDS - 1) / MAP_CHUNK_WORDS * MAP_CHUNK_WORDS;
-  uint32_t* map = c->arena.get_n<uint32_t>((size_t)(2 * wpad));  // pore map | solid map
-  uint32_t* crank = c->arena.get_n<uint32_t>((size_t)(2 * wpad / MAP_CHUNK_WORDS));
-  fill_u32(c, map, 0u, 2 * wpad);
-  const int64_t nk = npk + nsk;
-  if (nk > 0)
-    PNM_LAUNCH(c, "k_mark_first", k_mark_first, dim3(cdiv(nk, 256)), dim3(256), 0, sdf, pkeys,
-               npk, (const int32_t*)pore_ids, skeys, nsk, (const int32_t*)solid_ids, map,
-               map + wpad);
-  map_rank(c, map, 2 * wpad, crank, counts_dev + 1);
-  PNM_LAUNCH(c, "k_assign_ranks", k_assign_ranks, dim3(cdiv(std::max<int64_t>(nk, 1), 256)),
-             dim3(256), 0, sdf, pkeys, npk, skeys, nsk, pore_ids, solid_ids, map, crank, wpad,
-             reinterpret_cast<unsigned long long*>(counts_dev));
+  auto by_map = [&](const uint32_t* keys, int64_t nk, int32_t* ids, bool solid, uint64_t* cnt) {
+    uint32_t* map = c->arena.get_n<uint32_t>((size_t)words);
+    uint32_t* crank = c->arena.get_n<uint32_t>((size_t)cdiv(words, MAP_CHUNK_WORDS));
+    fill_u32(c, map, 0u, words);
+    if (nk > 0)
+      PNM_LAUNCH(c, "k_mark_first", k_mark_first, dim3(cdiv(nk, 256)), dim3(256), 0, sdf, keys,
+                 nk, (const int32_t*)ids, solid, map);
+    map_rank(c, map, words, crank, cnt);
+    if (nk > 0)
+      PNM_LAUNCH(c, "k_assign_ranks", k_assign_ranks, dim3(cdiv(nk, 256)), dim3(256), 0, sdf,
+                 keys, nk, ids, solid, map, crank);
+  };
+  by_map(pkeys, npk, pore_ids, false, counts_dev);
+  if (nsk > SOLID_RANK_MAX)
+    by_map(skeys, nsk, solid_ids, true, counts_dev + 1);
+  else
+    PNM_LAUNCH(c, "k_solid_ranks", k_solid_ranks, dim3(1), dim3(1024), 0, sdf, skeys, nsk,
+               solid_ids, reinterpret_cast<unsigned long long*>(counts_dev + 1));
 }
 
 // relabel_seg (pore_extraction.hpp:378-381) from the seg source words: a flagged root takes its

@@ -281,7 +281,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bitmap_list(
 
 // Rank directory of a uint32 bitmap: chunk_rank[c] = number of set bits in words [0, 32c).
 // The rank of bit f is then chunk_rank[f >> 10] + the popcounts of the words of its chunk
-// before word f >> 5 + the bits below f in that word.
+// before word f >> 5 + the bits below f in that word. Each thread counts MAP_RANK_CPT
+// consecutive chunks, so that few blocks are chained by the look-back.
+constexpr int MAP_RANK_CPT = 4;
 __global__ __launch_bounds__(SCAN_THREADS) void k_map_rank(
     const uint32_t* __restrict__ map, int64_t words, uint32_t* __restrict__ chunk_rank,
     uint64_t* __restrict__ total_out, uint32_t* __restrict__ ctl, uint64_t* __restrict__ state,
@@ -291,13 +293,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_map_rank(
   __shared__ uint32_t s_bid, s_epoch, s_excl;
   uint32_t bid, epoch;
   lb_begin(ctl, bid, epoch, &s_bid, &s_epoch);
-  const int64_t chunk = (int64_t)bid * SCAN_THREADS + threadIdx.x;
-  const int64_t base = chunk * ITEMS;
-  uint32_t w[ITEMS];
-  load_run<ITEMS>(map, base, words, w);
+  const int64_t chunk0 = ((int64_t)bid * SCAN_THREADS + threadIdx.x) * MAP_RANK_CPT;
+  uint32_t cc[MAP_RANK_CPT];
   uint32_t s = 0;
 #pragma unroll
-  for (int k = 0; k < ITEMS; ++k) s += (uint32_t)__popc(w[k]);
+  for (int q = 0; q < MAP_RANK_CPT; ++q) {
+    uint32_t w[ITEMS];
+    load_run<ITEMS>(map, (chunk0 + q) * ITEMS, words, w);
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) c += (uint32_t)__popc(w[k]);
+    cc[q] = c;
+    s += c;
+  }
   uint32_t tot;
   const uint32_t pre = block_excl_scan(s, lds, &tot);
   if (threadIdx.x < 64) {
@@ -305,7 +313,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_map_rank(
     if (threadIdx.x == 0) s_excl = excl;
   }
   __syncthreads();
-  if (base < words) chunk_rank[chunk] = s_excl + pre;
+  uint32_t run = s_excl + pre;
+#pragma unroll
+  for (int q = 0; q < MAP_RANK_CPT; ++q) {
+    if ((chunk0 + q) * ITEMS < words) chunk_rank[chunk0 + q] = run;
+    run += cc[q];
+  }
   if (threadIdx.x == 0) {
     if (total_out && bid == nblocks - 1) *total_out = (uint64_t)s_excl + tot;
     lb_end(ctl, epoch, nblocks);
@@ -391,7 +404,7 @@ void map_rank(Ctx* c, const uint32_t* map, int64_t words, uint32_t* chunk_rank,
     if (total_dev) fill_u64(c, total_dev, 0, 1);
     return;
   }
-  const int64_t nb = cdiv(words, (int64_t)SCAN_THREADS * MAP_CHUNK_WORDS);
+  const int64_t nb = cdiv(words, (int64_t)SCAN_THREADS * MAP_CHUNK_WORDS * MAP_RANK_CPT);
   lb_reserve(c, nb);
   PNM_LAUNCH(c, "k_map_rank", k_map_rank, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, map, words,
              chunk_rank, total_dev, c->lb_ctl, c->lb_state, (unsigned)nb, c->err_dev);

@@ -296,15 +296,12 @@ void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uin
 void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* segsrc, int32_t* min_pore);
 int64_t collect_keys_full(Ctx* c, const float* sdf, Vol v, const int32_t* min_pore,
                           uint32_t* key_list, uint64_t* counter);
-// Ranks keys by their minimum voxel; writes ids into id_arr[key] (pores: rank+1; solids:
-// -(rank+1) when sdf[first] <= 0 else 0) and the number of ids into *count_dev.
-// Pore and solid ids of both key sets by first-appearance bitmaps (no key sort); writes
-// counts_dev[0] = pore ids, counts_dev[1] = solid ids (no synchronisation).
+// Pore and solid ids of both key sets ranked by first appearance (id arrays hold each key's
+// first-appearance voxel on entry); writes counts_dev[0] = pore ids, counts_dev[1] = solid ids
+// (no synchronisation).
 void assign_ids_bitmap(Ctx* c, const float* sdf, Vol v, const uint32_t* pkeys, int64_t npk,
                        int32_t* pore_ids, const uint32_t* skeys, int64_t nsk, int32_t* solid_ids,
                        uint64_t* counts_dev);
-void assign_ids(Ctx* c, const float* sdf, const uint32_t* keys, int64_t nkeys, int32_t* id_arr,
-                bool solid, uint64_t* count_dev);
 void launch_seg(Ctx* c, Vol v, const int32_t* segsrc, const int32_t* pore_id,
                 const int32_t* solid_id, int32_t* seg);
 // Gradient-walk successors and capped chases, computed stand-alone. Modes: kWalkPores = the
