@@ -710,13 +710,18 @@ __device__ __forceinline__ void seam_flush(unsigned long long* tab, int* L) {
 }
 
 // Full run-union rule on the crossing forward rows of row chunks, one row per wave at a time.
-// mode 0: rows at a y boundary (boundary row by), slices bz * 16 .. + 15;
-// mode 1: slices at a z boundary (bz), rows of y tile by not at a y boundary.
+// mode 0: rows at a y boundary (boundary row by), slices bz * SEAM_ZR .. + SEAM_ZR - 1;
+// mode 1: slices at a z boundary (bz), rows of the SEAM_YT y tiles from by * SEAM_YT that are
+// not at a y boundary.
+// A block takes many rows, so that its pair set merges the repeats of more rows and the launch
+// has fewer blocks.
 // A row's contacts with a crossing forward row: each voxel with the voxel of the same column,
 // and a run's first and last voxel with the columns just outside the run (interior columns of
 // a run reach those through their own same-column contact). Those outside columns are held by
 // the neighbouring lanes, so only the chunk's edge columns are loaded, together with the rows:
 // one round of loads per row.
+constexpr int SEAM_ZR = 64;  // mode 0: slices per block
+constexpr int SEAM_YT = 4;   // mode 1: y tiles per block
 __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L, int nx, int ny,
                                                 int nz, int mode, int nsel, int bx, int by, int bz,
                                                 int lane, int wave) {
@@ -728,7 +733,7 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
   // column the one after it (a one-column chunk loads the latter when it needs it)
   const bool edge = in && (lane == 0 || x == xlast);
   const int xe = lane == 0 ? wm1(x, nx) : wp1(x, nx);
-  const int nrow = mode == 0 ? 16 : TILE_Y;
+  const int nrow = mode == 0 ? SEAM_ZR : SEAM_YT * TILE_Y;
   struct Row {
     int fy[4], fz[4], fwd[4], fe[4], my;
     bool same[4], ok;
@@ -740,10 +745,10 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
     bool ok = k < nrow;
     if (mode == 0) {
       y = y_boundary_row(by < nsel ? by : 0, ny);
-      z = bz * 16 + k;
+      z = bz * SEAM_ZR + k;
       ok = ok && by < nsel && z < nz;
     } else {
-      y = by * TILE_Y + k;
+      y = by * (SEAM_YT * TILE_Y) + k;
       ok = ok && y < ny && !y_boundary(y, ny);
       z = z_boundary_slice(bz, nz);
     }
@@ -856,12 +861,12 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restric
   __syncthreads();
   const int64_t b = blockIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  if (b < n0) {  // grid (cx, ysel, ceil(nz / 16))
+  if (b < n0) {  // grid (cx, ysel, ceil(nz / SEAM_ZR))
     const int bx = (int)(b % cx), rest = (int)(b / cx);
     seam_rows_block(tab, L, nx, ny, nz, 0, ysel, bx, rest % ysel, rest / ysel, lane, wave);
-  } else if (b < n0 + n1) {  // grid (cx, ceil(ny / TILE_Y), zsel)
+  } else if (b < n0 + n1) {  // grid (cx, ceil(ny / (SEAM_YT * TILE_Y)), zsel)
     const int64_t bb = b - n0;
-    const int nyt = (ny + TILE_Y - 1) / TILE_Y;
+    const int nyt = (ny + SEAM_YT * TILE_Y - 1) / (SEAM_YT * TILE_Y);
     const int bx = (int)(bb % cx), rest = (int)(bb / cx);
     seam_rows_block(tab, L, nx, ny, nz, 1, zsel, bx, rest % nyt, rest / nyt, lane, wave);
   } else {
@@ -918,18 +923,23 @@ __global__ __launch_bounds__(256) void k_roots_resolve(const float* __restrict__
                                                        int32_t* __restrict__ roots,
                                                        uint32_t root_flag,
                                                        int32_t* __restrict__ min_pore,
-                                                       unsigned long long* cap_flag) {
+                                                       unsigned long long* cap_flag,
+                                                       UDiv dgx, UDiv dgy) {
   constexpr int K = RESOLVE_WALKS;
   const int lane = threadIdx.x & 63;
-  const int64_t seg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (seg >= nseg) return;  // wave-uniform; the kernel has no block barrier
+  // wave-uniform segment index (fewer than 2^31 segments), so that its decoding runs on the
+  // scalar unit
+  const uint32_t seg = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if ((int64_t)seg >= nseg) return;  // wave-uniform; the kernel has no block barrier
   const int count = (int)cnt[seg];
-  const uint64_t* __restrict__ list = stage + seg * seg_cap;
+  const uint64_t* __restrict__ list = stage + (int64_t)seg * seg_cap;
   // the segment's tile row: seg = ((bz * gy + by) * gx + bx) * TILE_Y + row
-  const int gx = (nx + 63) / 64, gy = (ny + TILE_Y - 1) / TILE_Y;
-  const int64_t tr = seg / TILE_Y;
-  const int row = (int)(seg - tr * TILE_Y);
-  const int bx = (int)(tr % gx), by = (int)((tr / gx) % gy), bz = (int)(tr / ((int64_t)gx * gy));
+  const uint32_t tr = seg / TILE_Y;
+  const int row = (int)(seg % TILE_Y);
+  uint32_t ux, uy;
+  const uint32_t rest = udiv(tr, dgx, ux);
+  const uint32_t uz = udiv(rest, dgy, uy);
+  const int bx = (int)ux, by = (int)uy, bz = (int)uz;
   const int plane_n = nx * ny;
   const int rowbase = ((bz * TILE_Z) * ny + by * TILE_Y + row) * nx + bx * 64;
   unsigned long long flags = 0;
@@ -1045,8 +1055,8 @@ void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
 void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
   const int cx = (int)cdiv(v.nx, 64);
   const int ysel = 2 * (int)cdiv(v.ny, TILE_Y), zsel = (int)cdiv(v.nz, TILE_Z);
-  const int64_t n0 = (int64_t)cx * ysel * cdiv(v.nz, 16);
-  const int64_t n1 = (int64_t)cx * cdiv(v.ny, TILE_Y) * zsel;
+  const int64_t n0 = (int64_t)cx * ysel * cdiv(v.nz, SEAM_ZR);
+  const int64_t n1 = (int64_t)cx * cdiv(v.ny, SEAM_YT * TILE_Y) * zsel;
   const int64_t n2 = cdiv((int64_t)cx * v.ny * v.nz, 256);
   PNM_LAUNCH(c, "k_ccl_seams", k_ccl_seams, dim3((unsigned)(n0 + n1 + n2)), dim3(256), 0, labels,
              reinterpret_cast<const int2*>(xface), v.nx, v.ny, v.nz, cx, ysel, zsel, n0, n1);
@@ -1059,7 +1069,8 @@ void launch_roots_resolve(Ctx* c, const float* sdf, Vol v, const uint64_t* stage
   if (nseg <= 0) return;
   PNM_LAUNCH(c, "k_roots_resolve", k_roots_resolve, dim3(cdiv(nseg, 4)), dim3(256), 0, sdf, v.nx,
              v.ny, v.nz, v.n, stage, cnt, nseg, seg_cap, walk, roots, root_flag, min_pore,
-             reinterpret_cast<unsigned long long*>(cap_flag));
+             reinterpret_cast<unsigned long long*>(cap_flag),
+             make_udiv((uint32_t)cdiv(v.nx, 64)), make_udiv((uint32_t)cdiv(v.ny, TILE_Y)));
 }
 
 }  // namespace pnm
