@@ -117,7 +117,7 @@ __device__ __forceinline__ int ascent_best(const float* __restrict__ sdf, int x,
 // NaN neighbour never wins. A NaN centre needs its own test (nothing beats it).
 __device__ __forceinline__ uint64_t lex_key(float v, uint32_t idx) {
   uint32_t u = __float_as_uint(v + 0.0f);  // -0 -> +0
-  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  u ^= (uint32_t)((int32_t)u >> 31) | 0x80000000u;  // negative: all bits flipped; else the sign
   if (v != v) u = 0u;
   return ((uint64_t)u << 32) | idx;
 }

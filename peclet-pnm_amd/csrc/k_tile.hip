@@ -190,37 +190,29 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     K[tid] = lex_key(a, zbase + (uint32_t)off0);
     if (has1) K[tid + 64 * TILE_Y] = lex_key(b, zbase + (uint32_t)off1);
     __syncthreads();
-    for (int e = tid; e < T_LY * 64; e += 64 * TILE_Y) {
-      const int ly = e >> 6, lx = e & 63;
-      uint64_t m = K[ly * T_LX + lx];
-      uint8_t d = 0;
-      const uint64_t k1 = K[ly * T_LX + lx + 1], k2 = K[ly * T_LX + lx + 2];
-      if (k1 > m) {
-        m = k1;
-        d = 1;
-      }
-      if (k2 > m) {
-        m = k2;
-        d = 2;
-      }
-      XM[e] = m;
-      XD[ly][lx] = d;
-    }
+    // rows wave and (waves 0, 1) TILE_Y + wave: the second element is a wave-uniform branch
+    static_assert(T_LY * 64 <= 2 * 64 * TILE_Y, "two x-triples per thread at most");
+    auto xtriple = [&](int ly) {
+      const uint64_t k0 = K[ly * T_LX + lane], k1 = K[ly * T_LX + lane + 1],
+                     k2 = K[ly * T_LX + lane + 2];
+      const bool g1 = k1 > k0;
+      const uint64_t m1 = g1 ? k1 : k0;
+      const bool g2 = k2 > m1;
+      XM[ly * 64 + lane] = g2 ? k2 : m1;
+      XD[ly][lane] = (uint8_t)(g2 ? 2 : (g1 ? 1 : 0));
+    };
+    xtriple(wave);
+    if (wave < T_LY - TILE_Y) xtriple(TILE_Y + wave);
     centre = K[(wave + 1) * T_LX + lane + 1];
     __syncthreads();
-    uint64_t m = XM[wave * 64 + lane];
-    int dy = 0;
-    const uint64_t m1 = XM[(wave + 1) * 64 + lane], m2 = XM[(wave + 2) * 64 + lane];
-    if (m1 > m) {
-      m = m1;
-      dy = 1;
-    }
-    if (m2 > m) {
-      m = m2;
-      dy = 2;
-    }
+    const uint64_t m0 = XM[wave * 64 + lane], m1 = XM[(wave + 1) * 64 + lane],
+                   m2 = XM[(wave + 2) * 64 + lane];
+    const bool g1 = m1 > m0;
+    const uint64_t n1 = g1 ? m1 : m0;
+    const bool g2 = m2 > n1;
+    const int dy = g2 ? 2 : (g1 ? 1 : 0);
     code = dy * 3 + XD[wave + dy][lane];
-    return m;
+    return g2 ? m2 : n1;
   };
 
   // ---- march: slices z0-1 .. z0+nzt, three slices of loads in flight ----
@@ -237,8 +229,8 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   load_slice(z0 + 2, A0, A1);
   uint64_t m_cur = slice_max(z0, B0, B1, c_cur, d_cur);
   load_slice(z0 + 3, B0, B1);
-  // this thread's sdf>0 slices, peak slices, sdf == 0 slices and sdf >= 0 slices
-  uint32_t pore_mask = 0, peak_mask = 0, zero_mask = 0, nonneg_mask = 0;
+  // this thread's sdf>0 slices, peak slices and sdf == 0 slices
+  uint32_t pore_mask = 0, peak_mask = 0, zero_mask = 0;
   // one slice of the march: needs slice k = zl + 2 (values va, vb)
   auto march = [&](int zl, float va, float vb) {
     const int z = z0 + zl;
@@ -264,7 +256,6 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       }
     }
     if (col_in && cv == 0.0f) zero_mask |= 1u << zl;
-    if (col_in && cv >= 0.0f) nonneg_mask |= 1u << zl;
     const bool mk = col_in && (cv < thr);
     const uint64_t mm = __ballot(mk);
     const uint64_t starts = mm & ~(mm << 1);
@@ -363,7 +354,8 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const int i = srow + lane + mul(zl, plane_n);  // = gidx(l)
       if (col_in) {
         // a voxel that is no marker: -1 if it is a flood candidate (!(sdf >= 0)), else -2
-        o.labels[i] = r[zl] != -1 ? gidx(r[zl]) : (((nonneg_mask >> zl) & 1u) ? -2 : -1);
+        // sdf >= 0: an sdf>0 slice or an sdf == 0 slice
+        o.labels[i] = r[zl] != -1 ? gidx(r[zl]) : ((((pore_mask | zero_mask) >> zl) & 1u) ? -2 : -1);
         if (r[zl] == l) o.min_solid[i] = i;
         if ((zero_mask >> zl) & 1u) o.segsrc[i] = -1;  // never a pore, never flooded
       }
