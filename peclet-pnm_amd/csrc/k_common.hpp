@@ -48,6 +48,16 @@ __device__ __forceinline__ int wrap_near(int v, int N) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Neighbouring lanes of a wave by a DPP wave shift (one VALU move, no LDS crossbar):
+// lane_prev gives lane - 1's value (lane 0 gets `first`), lane_next lane + 1's (lane 63 gets
+// `last`).
+__device__ __forceinline__ uint32_t lane_prev(uint32_t v, uint32_t first) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t lane_next(uint32_t v, uint32_t last) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)v, 0x130, 0xf, 0xf, false);
+}
+
 // Wave-aggregated append: one atomic per wave. Every lane of the wave must call it.
 // Returns the slot of a lane with pred set (undefined otherwise).
 __device__ __forceinline__ uint64_t wave_append(unsigned long long* counter, bool pred,

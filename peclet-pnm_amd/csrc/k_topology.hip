@@ -171,8 +171,10 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
       const uint64_t kv = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
                           (uint64_t)((uint32_t)l2 - (uint32_t)lo);
       const uint64_t key = act && my != nl ? kv : kEmpty;
-      const uint64_t prev = __shfl_up(key, 1, 64);
-      const bool need = key != kEmpty && key != last[k] && !(lane > 0 && prev == key);
+      // the previous lane's key (kEmpty at lane 0)
+      const uint64_t prev = ((uint64_t)lane_prev((uint32_t)(key >> 32), ~0u) << 32) |
+                            lane_prev((uint32_t)key, ~0u);
+      const bool need = key != kEmpty && key != last[k] && prev != key;
       last[k] = key;
       if (!__any(need)) continue;  // wave-uniform
       const uint32_t h = slot_hash(key);
