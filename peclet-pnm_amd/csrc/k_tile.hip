@@ -363,7 +363,9 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const uint64_t rm = __ballot(lroot);
       if (lroot) lstage[nr + __popcll(rm & below)] = (uint32_t)i;
       nr += (unsigned)__popcll(rm);
-      const int a0 = __shfl(r[zl], 0), aL = __shfl(r[zl], xlast - x0);
+      // the chunk's first and last column (uniform lanes: read into scalar registers)
+      const int a0 = __builtin_amdgcn_readlane(r[zl], 0);
+      const int aL = __builtin_amdgcn_readlane(r[zl], xlast - x0);
       if (lane == 0 && y < ny) {
         int2 v;
         v.x = a0 == -1 ? -1 : gidx(a0);

@@ -14,6 +14,7 @@ namespace pnm {
 constexpr int TOPO_ZB = 16;         // z-slices per block (a 64x4x16 brick)
 constexpr int TOPO_TABLE = 512;     // LDS hash slots (4 KiB): a brick has ~50 distinct pairs
 constexpr int TOPO_PROBES = 32;     // linear-probe limit before a key bypasses the table
+constexpr int TOPO_FACES = 256;     // per-wave list of differing faces (fused kernel)
 constexpr uint64_t kEmpty = ~0ull;
 
 // Slot of a pair key in the brick's table (9 bits), from full-rate 24-bit multiplies only (a
@@ -56,6 +57,8 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   __shared__ uint32_t RBASE[FUSED ? TB_Y * TB_Z : 1];
   __shared__ unsigned wc[4];
   __shared__ unsigned s_base;
+  // per wave: the brick's differing faces not yet looked up (S index | direction << 14)
+  __shared__ uint16_t PF[FUSED ? 4 * TOPO_FACES : 1];
   const int tid = threadIdx.y * 64 + threadIdx.x;
   const int lane = threadIdx.x, wave = threadIdx.y;
   for (int s = tid; s < TOPO_TABLE; s += 256) table[s] = kEmpty;
@@ -124,11 +127,9 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   const int y = y0 + threadIdx.y;
   const bool in = x < nx && y < ny;
   const int xp = in ? wp1(x, nx) : 0, yp = in ? wp1(y, ny) : 0;
-  // a face pair repeats along x (the previous lane) and along z (this lane's previous slice):
-  // both repeats skip the hash set. The keys are computed branch-free; a direction in which no
-  // lane of the wave has a new key is skipped as a whole; the first probe (the key found, or
-  // inserted into an empty slot) is straight-line code, and only lanes whose first slot held
-  // another key continue probing.
+  // Given seg (topology alone): a face pair repeats along x (the previous lane) and along z
+  // (this lane's previous slice): both repeats skip the hash set. The keys are computed
+  // branch-free; a direction in which no lane of the wave has a new key is skipped as a whole.
   uint64_t last[3] = {kEmpty, kEmpty, kEmpty};
   auto insert_slow = [&](uint64_t key, uint32_t h) {
 #pragma unroll 1
@@ -145,6 +146,73 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
     const unsigned long long sl = atomicAdd(ocount, 1ull);
     overflow[sl] = key;
   };
+  // The first probe of a key (the key found, or inserted into an empty slot) is straight-line
+  // code; only lanes whose first slot held another key continue probing.
+  auto insert = [&](uint64_t key, bool need) {
+    const uint32_t h = slot_hash(key);
+    bool placed = true;
+    if (need) {
+      const unsigned long long cur = table[h];
+      placed = cur == key;
+      if (cur == kEmpty) {
+        const unsigned long long prv = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
+        placed = prv == kEmpty || prv == key;
+      }
+    }
+    if (__any(!placed) && !placed) insert_slow(key, h);
+  };
+  if constexpr (FUSED) {
+    // Only about one face in ten separates two labels. Each wave lists its differing faces by
+    // ballot (S index and direction, in LDS) and computes the pair keys of the list 64 at a
+    // time, when the list may not hold another slice's faces and at the end: the key work runs
+    // on full waves. A listed face whose key equals the previous entry's (a boundary run along
+    // x) skips the set; repeats along z find their key at the first probe.
+    uint16_t* const pf = PF + wave * TOPO_FACES;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned np = 0;
+    auto flush = [&]() {
+      __builtin_amdgcn_wave_barrier();
+      for (unsigned j0 = 0; j0 < np; j0 += 64) {
+        const unsigned j = j0 + lane;
+        uint64_t key = kEmpty;
+        if (j < np) {
+          const uint32_t f = pf[j];
+          const int e = (int)(f & 0x3fffu), k = (int)(f >> 14);
+          const int a = S[e], b = S[e + (k == 0 ? 1 : (k == 1 ? TB_X : TB_X * TB_Y))];
+          const int l1 = a < b ? a : b, l2 = a < b ? b : a;
+          key = ((uint64_t)((uint32_t)l1 - (uint32_t)lo) << bits) |
+                (uint64_t)((uint32_t)l2 - (uint32_t)lo);
+        }
+        const uint64_t prev = ((uint64_t)lane_prev((uint32_t)(key >> 32), ~0u) << 32) |
+                              lane_prev((uint32_t)key, ~0u);
+        insert(key, key != kEmpty && prev != key);
+      }
+      __builtin_amdgcn_wave_barrier();
+      np = 0;
+    };
+    static_assert(((TOPO_ZB - 1) * TB_Y + 3) * TB_X + 63 < (1 << 14), "S index in 14 bits");
+    static_assert(TOPO_FACES >= 4 * 64, "room for a slice's faces after the flush test");
+    for (int zz = 0; zz < TOPO_ZB; ++zz) {
+      const bool act = in && z0 + zz < nz;
+      const int e = (zz * TB_Y + wave) * TB_X + lane;
+      int my = 0, nb[3] = {0, 0, 0};
+      if (act) {
+        my = S[e];
+        nb[0] = S[e + 1];
+        nb[1] = S[e + TB_X];
+        nb[2] = S[e + TB_X * TB_Y];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const bool d = act && my != nb[k];
+        const uint64_t dm = __ballot(d);
+        if (d) pf[np + (unsigned)__popcll(dm & below)] = (uint16_t)(e | (k << 14));
+        np += (unsigned)__popcll(dm);
+      }
+      if (np > (unsigned)(TOPO_FACES - 3 * 64)) flush();  // wave-uniform
+    }
+    flush();
+  } else {
   for (int zz = 0; zz < TOPO_ZB; ++zz) {
     const int z = z0 + zz;
     const bool act = in && z < nz;
@@ -177,18 +245,9 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
       const bool need = key != kEmpty && key != last[k] && prev != key;
       last[k] = key;
       if (!__any(need)) continue;  // wave-uniform
-      const uint32_t h = slot_hash(key);
-      bool placed = true;
-      if (need) {
-        const unsigned long long cur = table[h];
-        placed = cur == key;
-        if (cur == kEmpty) {
-          const unsigned long long prv = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
-          placed = prv == kEmpty || prv == key;
-        }
-      }
-      if (__any(!placed) && !placed) insert_slow(key, h);
+      insert(key, need);
     }
+  }
   }
   __syncthreads();
   const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
