@@ -30,10 +30,12 @@ namespace pnm {
 constexpr int T_LX = 66, T_LY = TILE_Y + 2;
 constexpr int T_VOX = 64 * TILE_Y * TILE_Z;
 constexpr int T_SEG = 64 * TILE_Z;  // staging slots per wave
-// DIR byte: bits 0-4 the ascent direction of an sdf>0 voxel (0..26, 13 = itself; kNoDir for
-// other voxels), bit 5 the extract_pores peak predicate.
-constexpr uint8_t kNoDir = 31;
-constexpr uint8_t kDirMask = 31;
+// DIR byte: the ascent direction of an sdf>0 voxel as (dz+1) << 4 | (dy+1) << 2 | (dx+1)
+// (kSelf: the voxel itself), kNoDir for other voxels; the offsets come back by bit-field
+// extraction.
+constexpr uint8_t kNoDir = 63;
+constexpr uint8_t kDirMask = 63;
+constexpr int kSelf = (1 << 4) | (1 << 2) | 1;
 
 __device__ __forceinline__ int lds_find(int* L, int x) {
   int p = L[x];
@@ -107,11 +109,11 @@ __device__ __forceinline__ uint64_t open_entry(uint64_t rec, int zl, int xl) {
 __device__ __forceinline__ int open_slice(uint64_t e) { return (int)((e >> 56) & 15u); }
 __device__ __forceinline__ int open_column(uint64_t e) { return (int)((e >> 50) & 63u); }
 
-// direction code d = (dz+1)*9 + (dy+1)*3 + (dx+1)
+// direction code d = (dz+1) << 4 | (dy+1) << 2 | (dx+1)
 __device__ __forceinline__ void dir_delta(int d, int& dx, int& dy, int& dz) {
-  dz = d / 9 - 1;
-  dy = (d / 3) % 3 - 1;
-  dx = d % 3 - 1;
+  dz = (d >> 4) - 1;
+  dy = ((d >> 2) & 3) - 1;
+  dx = (d & 3) - 1;
 }
 
 // Three 512-thread blocks per CU need 6 waves per SIMD, i.e. at most 80 VGPRs.
@@ -180,10 +182,10 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     b = plane[off1];
   };
   // 3x3 (x, y) key maximum of this thread's column at slice zg (values a, b already loaded),
-  // with the winner's (dy, dx) as dy*3+dx, and the centre key. Separable: the x-triple maximum
-  // of each of the slice's 10 rows is computed once (1.25 per thread) and shared through LDS,
-  // then each thread takes the maximum of three of them: 4.5 compare-selects per voxel instead
-  // of 8, at the price of a second barrier. The first position wins among equal keys (a voxel
+  // with the winner's (dy, dx) as dy << 2 | dx, and the centre key. Separable: the x-triple
+  // maximum of each of the slice's 10 rows is computed once (1.25 per thread) and shared
+  // through LDS, then each thread takes the maximum of three of them: 4.5 compare-selects per
+  // voxel instead of 8, at the price of a second barrier. The first position wins among equal keys (a voxel
   // met twice through a periodic wrap), in (dy, dx) row-major order.
   auto slice_max = [&](int zg, float a, float b, uint64_t& centre, int& code) -> uint64_t {
     const uint32_t zbase = (uint32_t)wrap_near(zg, nz) * (uint32_t)plane_n;
@@ -211,7 +213,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     const uint64_t n1 = g1 ? m1 : m0;
     const bool g2 = m2 > n1;
     const int dy = g2 ? 2 : (g1 ? 1 : 0);
-    code = dy * 3 + XD[wave + dy][lane];
+    code = (dy << 2) | XD[wave + dy][lane];
     return g2 ? m2 : n1;
   };
 
@@ -250,7 +252,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
         if (m_next > m) { m = m_next; dz = 2; cd = d_next; }
         is_peak = (int)(uint32_t)m == i;
         if (cv > 0.0f) {
-          dir = is_peak ? (uint8_t)13 : (uint8_t)(dz * 9 + cd);
+          dir = is_peak ? (uint8_t)kSelf : (uint8_t)((dz << 4) | cd);
           pore_mask |= 1u << zl;
         }
       }
@@ -389,7 +391,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     const int d = DIR[l] & kDirMask;
     uint32_t w;
     bool active = false;
-    if (d == 13) {
+    if (d == kSelf) {
       w = cs_pack(l, 0, CS_FIX);
     } else {
       int dx, dy, dz;
@@ -491,7 +493,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     const int i = srow + xl + mul(zl, plane_n);
     int target, steps, fin_u = -1;
     bool fin = false, capped = false;
-    const bool root_here = (DIR[l] & kDirMask) == 13;
+    const bool root_here = (DIR[l] & kDirMask) == kSelf;
     const uint32_t w = root_here ? cs_pack(l, 0, CS_FIX) : (uint32_t)LAB[l];
     const int u = cs_u(w);
     steps = cs_s(w);
@@ -518,7 +520,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       steps = 0;
       while (true) {
         const int d = DIR[cl] & kDirMask;
-        if (d == 13) {
+        if (d == kSelf) {
           fin = true;
           fin_u = cl;
           target = gidx(cl);
@@ -538,7 +540,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
         cl = (az * TILE_Y + ay) * 64 + ax;
         if (steps == kMaxSteps) {
           fin = true;
-          capped = (DIR[cl] & kDirMask) != 13;
+          capped = (DIR[cl] & kDirMask) != kSelf;
           if (!capped) fin_u = cl;
           target = gidx(cl);
           break;
