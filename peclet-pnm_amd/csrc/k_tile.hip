@@ -716,11 +716,14 @@ __device__ __forceinline__ void seam_flush(unsigned long long* tab, int* L) {
 // a run reach those through their own same-column contact). Those outside columns are held by
 // the neighbouring lanes, so only the chunk's edge columns are loaded, together with the rows:
 // one round of loads per row.
-constexpr int SEAM_ZR = 64;  // mode 0: slices per block
-constexpr int SEAM_YT = 4;   // mode 1: y tiles per block
-__device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L, int nx, int ny,
-                                                int nz, int mode, int nsel, int bx, int by, int bz,
-                                                int lane, int wave) {
+// The contacts found are listed per wave (by ballot, in LDS) and go through the pair set 64 at
+// a time, so that the set's probing runs on full waves and has one call site.
+constexpr int SEAM_ZR = 64;     // mode 0: slices per block
+constexpr int SEAM_YT = 4;      // mode 1: y tiles per block
+constexpr int SEAM_LIST = 256;  // per-wave list of contacts
+__device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int2* list, int* L,
+                                                int nx, int ny, int nz, int mode, int nsel,
+                                                int bx, int by, int bz, int lane, int wave) {
   const int x0 = bx * 64;
   const int x = x0 + lane;
   const bool in = x < nx;
@@ -762,6 +765,24 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
     }
     R.my = ok && in ? L[i] : -1;
   };
+  int2* const pl = list + wave * SEAM_LIST;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  unsigned np = 0;
+  auto add = [&](bool v, int a, int b) {
+    const uint64_t vm = __ballot(v);
+    if (v) pl[np + (unsigned)__popcll(vm & below)] = make_int2(a, b);
+    np += (unsigned)__popcll(vm);
+  };
+  auto flush = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    for (unsigned j = lane; j < np; j += 64) {
+      const int2 q = pl[j];
+      seam_pair(tab, L, q.x, q.y);
+    }
+    __builtin_amdgcn_wave_barrier();
+    np = 0;
+  };
+  static_assert(SEAM_LIST >= 4 * 64, "room for a forward row's contacts after the flush test");
   // label words: a marker's label (>= 0) or a negative code (-1, -2) for every other voxel
   auto pairs = [&](const Row& R) {
     if (!R.ok) return;
@@ -778,22 +799,16 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
       const bool cm = c >= 0;
       const int prev_c = __shfl_up(c, 1, 64);
       const int next_c = __shfl_down(c, 1, 64);
-      if (mk && cm) {
-        const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
-        if (!dup) seam_pair(tab, L, my, c);
-      }
-      if (mk && !cm) {
-        if (run_start) {
-          const int a = lane == 0 ? R.fe[r] : prev_c;
-          if (a >= 0) seam_pair(tab, L, my, a);
-        }
-        if (run_end) {
-          const int b = x != xlast ? next_c
-                                   : (lane != 0 ? R.fe[r]
-                                                : L[(R.fz[r] * ny + R.fy[r]) * nx + wp1(x, nx)]);
-          if (b >= 0) seam_pair(tab, L, my, b);
-        }
-      }
+      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+      add(mk && cm && !dup, my, c);
+      int a = -1, b = -1;
+      if (mk && !cm && run_start) a = lane == 0 ? R.fe[r] : prev_c;
+      if (mk && !cm && run_end)
+        b = x != xlast ? next_c
+                       : (lane != 0 ? R.fe[r] : L[(R.fz[r] * ny + R.fy[r]) * nx + wp1(x, nx)]);
+      add(a >= 0, my, a);
+      add(b >= 0, my, b);
+      if (np > (unsigned)(SEAM_LIST - 3 * 64)) flush();  // wave-uniform
     }
   };
   // two rows of the wave at a time: the second row's loads are in flight while the first's
@@ -805,6 +820,7 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int* L,
     pairs(A);
     pairs(B);
   }
+  flush();
 }
 
 // One thread per row chunk, from the x-face labels k_tile wrote (entry (chunk, z, y): labels of
@@ -852,6 +868,7 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restric
                                                    int ny, int nz, int cx, int ysel, int zsel,
                                                    int64_t n0, int64_t n1) {
   __shared__ unsigned long long tab[SEAM_TABLE];
+  __shared__ int2 list[4 * SEAM_LIST];
   const int tid = threadIdx.x;
   for (int s = tid; s < SEAM_TABLE; s += 256) tab[s] = kNoPair;
   __syncthreads();
@@ -859,12 +876,13 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restric
   const int lane = tid & 63, wave = tid >> 6;
   if (b < n0) {  // grid (cx, ysel, ceil(nz / SEAM_ZR))
     const int bx = (int)(b % cx), rest = (int)(b / cx);
-    seam_rows_block(tab, L, nx, ny, nz, 0, ysel, bx, rest % ysel, rest / ysel, lane, wave);
+    seam_rows_block(tab, list, L, nx, ny, nz, 0, ysel, bx, rest % ysel, rest / ysel, lane,
+                    wave);
   } else if (b < n0 + n1) {  // grid (cx, ceil(ny / (SEAM_YT * TILE_Y)), zsel)
     const int64_t bb = b - n0;
     const int nyt = (ny + SEAM_YT * TILE_Y - 1) / (SEAM_YT * TILE_Y);
     const int bx = (int)(bb % cx), rest = (int)(bb / cx);
-    seam_rows_block(tab, L, nx, ny, nz, 1, zsel, bx, rest % nyt, rest / nyt, lane, wave);
+    seam_rows_block(tab, list, L, nx, ny, nz, 1, zsel, bx, rest % nyt, rest / nyt, lane, wave);
   } else {
     seam_edges_block(tab, L, xf, ny, nz, cx, b - n0 - n1, tid);
   }
