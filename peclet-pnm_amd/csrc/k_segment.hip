@@ -7,6 +7,8 @@
 //   roots    : chase the ascent successor array up to 512 steps (exactly the capped walk).
 //   renumber : per-key first-appearance minimum by atomicMin, then a rank by radix sort of the
 //              (few) keys instead of two full-volume scans.
+#include <algorithm>
+
 #include "k_common.hpp"
 
 namespace pnm {
@@ -307,56 +309,67 @@ void launch_merge_roots(Ctx* c, const float* sdf, Vol v, const int32_t* roots, i
 }
 
 // ---------------- flood (pore_extraction.hpp:237-269) ----------------
-// One Jacobi sweep over the current band list (count read on the device): the new label is the
-// minimum labelled neighbour of the previous state; voxels still unlabelled are staged per block
-// for the next band list. Neighbour words are seg source words: a label, -1, or a flagged root
-// (kPoreFlag), the last two above every label as unsigned numbers.
+// One Jacobi sweep over the current band list, whose length the kernels read on the device (the
+// host does not size the grids, so no read back precedes a sweep): a fixed grid of FS_BLOCKS
+// blocks, block b taking the b-th contiguous share of the list. The new label is the minimum
+// labelled neighbour of the previous state; voxels still unlabelled are staged in the block's
+// segment (seg_cap slots, at least a share) for the next band list. Neighbour words are seg
+// source words: a label, -1, or a flagged root (kPoreFlag), the last two above every label as
+// unsigned numbers.
 __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__ L, int nx,
                                                      int ny, int nz, UDiv dx, UDiv dy,
                                                      const uint32_t* __restrict__ band,
                                                      const uint64_t* __restrict__ count,
                                                      int32_t* __restrict__ newlab,
                                                      uint32_t* __restrict__ stage,
-                                                     uint32_t* __restrict__ cnt) {
+                                                     uint32_t* __restrict__ cnt, int64_t seg_cap) {
   __shared__ unsigned wc[4];
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t nband = (int64_t)*count;
-  bool keep = false;
-  uint32_t i = 0;
-  if (j < nband) {
-    i = band[j];
-    uint32_t x, y;
-    const uint32_t r = udiv(i, dx, x);  // r = z * ny + y
-    const uint32_t z = udiv(r, dy, y);
-    // neighbour offsets along each axis: -1, 0, +1 steps, wrapped at the faces
-    const int plane = nx * ny;
-    const int ox[3] = {x > 0 ? -1 : nx - 1, 0, (int)x + 1 < nx ? 1 : 1 - nx};
-    const int oy[3] = {y > 0 ? -nx : (ny - 1) * nx, 0, (int)y + 1 < ny ? nx : (1 - ny) * nx};
-    const int oz[3] = {z > 0 ? -plane : (nz - 1) * plane, 0,
-                       (int)z + 1 < nz ? plane : (1 - nz) * plane};
-    int nl[26];
-    int k = 0;
+  const int64_t share = (nband + gridDim.x - 1) / gridDim.x;
+  const int64_t j_begin = (int64_t)blockIdx.x * share;
+  const int64_t j_end = min(nband, j_begin + share);
+  unsigned base = 0;
+  for (int64_t j0 = j_begin; j0 < j_end; j0 += 256) {  // block-uniform
+    const int64_t j = j0 + threadIdx.x;
+    bool keep = false;
+    uint32_t i = 0;
+    if (j < j_end) {
+      i = band[j];
+      uint32_t x, y;
+      const uint32_t r = udiv(i, dx, x);  // r = z * ny + y
+      const uint32_t z = udiv(r, dy, y);
+      // neighbour offsets along each axis: -1, 0, +1 steps, wrapped at the faces
+      const int plane = nx * ny;
+      const int ox[3] = {x > 0 ? -1 : nx - 1, 0, (int)x + 1 < nx ? 1 : 1 - nx};
+      const int oy[3] = {y > 0 ? -nx : (ny - 1) * nx, 0, (int)y + 1 < ny ? nx : (1 - ny) * nx};
+      const int oz[3] = {z > 0 ? -plane : (nz - 1) * plane, 0,
+                         (int)z + 1 < nz ? plane : (1 - nz) * plane};
+      int nl[26];
+      int k = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b)
+        for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          if (a == 1 && b == 1 && cc == 1) continue;
-          nl[k++] = L[(int)i + oz[a] + oy[b] + ox[cc]];
-        }
-    // a label is a flat index (below 2^31); -1 (unlabelled) and flagged roots are not
-    uint32_t best = ~0u;
+          for (int cc = 0; cc < 3; ++cc) {
+            if (a == 1 && b == 1 && cc == 1) continue;
+            nl[k++] = L[(int)i + oz[a] + oy[b] + ox[cc]];
+          }
+      // a label is a flat index (below 2^31); -1 (unlabelled) and flagged roots are not
+      uint32_t best = ~0u;
 #pragma unroll
-    for (int q = 0; q < 26; ++q) best = min(best, (uint32_t)nl[q]);
-    keep = best >= kPoreFlag;
-    newlab[j] = keep ? -1 : (int)best;
+      for (int q = 0; q < 26; ++q) best = min(best, (uint32_t)nl[q]);
+      keep = best >= kPoreFlag;
+      newlab[j] = keep ? -1 : (int)best;
+    }
+    unsigned tot;
+    const unsigned pos = block_stage_pos<4>(keep, wc, wave, lane, tot);
+    if (keep) stage[(int64_t)blockIdx.x * seg_cap + base + pos] = i;
+    base += tot;
+    __syncthreads();  // wc is rewritten by the next round
   }
-  unsigned tot;
-  const unsigned pos = block_stage_pos<4>(keep, wc, wave, lane, tot);
-  if (keep) stage[(int64_t)blockIdx.x * 256 + pos] = i;
-  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = base;
 }
 
 // Applies a sweep; a newly labelled voxel before its label's first appearance lowers it
@@ -364,25 +377,31 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
 __global__ void k_flood_apply(int32_t* __restrict__ L, const uint32_t* __restrict__ band,
                               const int32_t* __restrict__ newlab,
                               const uint64_t* __restrict__ count, int32_t* min_solid) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= (int64_t)*count) return;
-  const int32_t b = newlab[j];
-  if (b != -1) {
-    const uint32_t i = band[j];
-    L[i] = b;
-    if ((int32_t)i < b) atomicMin(min_solid + b, (int32_t)i);
+  const int64_t nband = (int64_t)*count;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nband;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = newlab[j];
+    if (b != -1) {
+      const uint32_t i = band[j];
+      L[i] = b;
+      if ((int32_t)i < b) atomicMin(min_solid + b, (int32_t)i);
+    }
   }
 }
 
+// Staging segment of a sweep block: a share of the longest band, at least the 64 slots the
+// compaction's gather needs.
+int64_t flood_seg_cap(int64_t max_items) { return std::max<int64_t>(cdiv(max_items, FS_BLOCKS), 64); }
+
 void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uint64_t* count,
-                 int64_t grid_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
+                 int64_t max_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
                  uint32_t* stage, uint32_t* stage_cnt, int32_t* min_solid) {
-  const int64_t nb = cdiv(grid_items, 256);
-  PNM_LAUNCH(c, "k_flood_sweep", k_flood_sweep, dim3((unsigned)nb), dim3(256), 0, labels, v.nx,
+  const int64_t seg_cap = flood_seg_cap(max_items);
+  PNM_LAUNCH(c, "k_flood_sweep", k_flood_sweep, dim3(FS_BLOCKS), dim3(256), 0, labels, v.nx,
              v.ny, v.nz, make_udiv((uint32_t)v.nx), make_udiv((uint32_t)v.ny), band, count,
-             newlab, stage, stage_cnt);
-  compact_staged(c, stage, stage_cnt, nb, 256, band_next, count_next);
-  PNM_LAUNCH(c, "k_flood_apply", k_flood_apply, dim3((unsigned)nb), dim3(256), 0, labels, band,
+             newlab, stage, stage_cnt, seg_cap);
+  compact_staged(c, stage, stage_cnt, FS_BLOCKS, seg_cap, band_next, count_next);
+  PNM_LAUNCH(c, "k_flood_apply", k_flood_apply, dim3(FS_BLOCKS), dim3(256), 0, labels, band,
              newlab, count, min_solid);
 }
 

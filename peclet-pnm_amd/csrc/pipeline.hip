@@ -91,7 +91,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   int32_t* min_solid = c->arena.get_n<int32_t>((size_t)n);
   // staging: the tile pass fills nseg segments of scap slots; flatten and the flood use nb1
   // blocks of 256
-  const int64_t stage_words = std::max(nseg * scap, nb1 * 256);
+  const int64_t stage_words =
+      std::max(std::max(nseg * scap, nb1 * 256), (int64_t)FS_BLOCKS * flood_seg_cap(n));
   uint32_t* stage_b = c->arena.get_n<uint32_t>((size_t)stage_words);
   uint32_t* stage_c = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
   uint64_t* stage_o = c->arena.get_n<uint64_t>((size_t)(nseg * scap));  // open walks
@@ -99,7 +100,8 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
   // for one multi-list compaction
   uint32_t* cnt_c = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* cnt_pair2 = c->arena.get_n<uint32_t>((size_t)(2 * nseg));
-  uint32_t* cnt_b = c->arena.get_n<uint32_t>((size_t)std::max(nseg, nb1));
+  uint32_t* cnt_b =
+      c->arena.get_n<uint32_t>((size_t)std::max(std::max(nseg, nb1), (int64_t)FS_BLOCKS));
   uint32_t* stage_r = c->arena.get_n<uint32_t>((size_t)(nseg * scap));
   uint32_t* cnt_r = c->arena.get_n<uint32_t>((size_t)nseg);
   uint32_t* peaks = c->arena.get_n<uint32_t>((size_t)n);
@@ -160,44 +162,39 @@ static SegInternal segment_core(Ctx* c, const float* sdf, Vol v, const float spc
     compact_staged_multi(c, 2, st, cnt_pair2, nseg, scap, outs, tots);
   }
   join_side(c);
-  uint64_t h[9];
-  readback(c, ctr, 9, h);
-  const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2], nband = (int64_t)h[8];
 
   // Remaining Jacobi sweeps until a sweep labels nothing (pore_extraction.hpp:238-269), with
   // no cap on their number. Band lengths live on the device in a ring of four counter words
   // (ctr[8 + s % 4] is the length sweep s reads, ctr[8 + (s + 1) % 4] the one it writes); the
-  // host reads them every two sweeps and sizes the next two sweeps' grids by the last length
-  // (the band only shrinks). The counter block is read whole, so the walk-cap flag (set by the
-  // resolver, launched before the sweeps) comes back with the band lengths.
-  bool have_flag = false;
-  if (nband > 0) {
+  // sweeps size their work from them, and the host reads the counter block every two sweeps to
+  // stop when a sweep labels nothing. The first read also brings the list sizes of the tile
+  // passes and the walk-cap flag (set by the resolver, joined above).
+  uint64_t h[12];
+  {
     uint32_t* cur = band;
     uint32_t* nxt = band2;
-    int64_t grid = nband;
     int64_t s = 0;
+    int64_t bound = n;  // band length bound for the staging segments (the band only shrinks)
     const size_t sweep_mark = c->arena.mark();  // every sweep pair reuses the same scratch
     while (true) {
       c->arena.rewind(sweep_mark);
       for (int k = 0; k < 2; ++k, ++s) {
-        flood_sweep(c, v, segsrc, cur, ctr + 8 + (s & 3), grid, nxt, ctr + 8 + ((s + 1) & 3),
+        flood_sweep(c, v, segsrc, cur, ctr + 8 + (s & 3), bound, nxt, ctr + 8 + ((s + 1) & 3),
                     newlab, stage_b, cnt_b, min_solid);
         std::swap(cur, nxt);
       }
-      uint64_t hh[12];
-      readback(c, ctr, 12, hh);
-      h[0] = hh[3];
-      have_flag = true;
-      const uint64_t before = hh[8 + ((s - 1) & 3)], after = hh[8 + (s & 3)];
+      readback(c, ctr, 12, h);
+      const uint64_t before = h[8 + ((s - 1) & 3)], after = h[8 + (s & 3)];
       if (after == before || after == 0) break;
-      grid = (int64_t)after;
+      bound = (int64_t)after;
     }
   }
+  const int64_t npeaks = (int64_t)h[0], nskeys = (int64_t)h[2];
+  h[0] = h[3];  // the walk-cap flag
 
   SegInternal out{};
   if (labels_only) return out;
   if (split) launch_merge_roots(c, sdf, v, roots, segsrc);
-  if (!have_flag) readback(c, ctr + 3, 1, h);
   if (h[0] & 2u) fail(PNM_EHIP, "internal error: ascent walk record missing");
   const uint32_t* pkeys = peaks;
   int64_t npk = npeaks;

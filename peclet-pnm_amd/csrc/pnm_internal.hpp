@@ -287,10 +287,14 @@ void launch_label_codes(Ctx* c, Vol v, const int32_t* labels, int32_t* out);
 void launch_merge_roots(Ctx* c, const float* sdf, Vol v, const int32_t* roots, int32_t* segsrc);
 void launch_ccl(Ctx* c, Vol v, int32_t* labels, int32_t* min_solid, uint32_t* key_stage,
                 uint32_t* key_cnt);
-// One Jacobi flood sweep over the band list whose length is the device word *count (grid sized
-// for grid_items >= that length); writes the remaining band list and its length.
+// Grid of the global flood sweeps (k_flood_sweep): block b takes the b-th share of the band.
+constexpr unsigned FS_BLOCKS = 1024;
+// One Jacobi flood sweep over the band list whose length is the device word *count (at most
+// max_items; stage holds FS_BLOCKS * flood_seg_cap(max_items) slots, stage_cnt FS_BLOCKS);
+// writes the remaining band list and its length. No host synchronisation.
+int64_t flood_seg_cap(int64_t max_items);
 void flood_sweep(Ctx* c, Vol v, int32_t* labels, const uint32_t* band, const uint64_t* count,
-                 int64_t grid_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
+                 int64_t max_items, uint32_t* band_next, uint64_t* count_next, int32_t* newlab,
                  uint32_t* stage, uint32_t* stage_cnt, int32_t* min_solid);
 // Capped-walk fallback: full reinitialisation of the pore minima from the seg source array.
 void launch_min_full(Ctx* c, const float* sdf, Vol v, const int32_t* segsrc, int32_t* min_pore);
