@@ -1,7 +1,7 @@
 // k_topology.hip — boundary-pair topology (extractTopologyView, pore_extraction.hpp:401-451).
 //
 // The reference appends one (min,max) pair per differing +x/+y/+z face (tens of millions at
-// 512^3) and sorts them on the host. Here each block deduplicates the pairs of a 64x4x16 brick in
+// 512^3) and sorts them on the host. Here each block deduplicates the pairs of a 64x8x16 brick in
 // an LDS hash set and appends only its distinct keys; the short candidate list is then radix
 // sorted and made unique on the device. The result is the same sorted unique list.
 //
@@ -11,8 +11,10 @@
 
 namespace pnm {
 
-constexpr int TOPO_ZB = 16;         // z-slices per block (a 64x4x16 brick)
-constexpr int TOPO_TABLE = 512;     // LDS hash slots (4 KiB): a brick has ~50 distinct pairs
+constexpr int TOPO_BY = 8;          // rows per block, one wave each
+constexpr int TOPO_ZB = 16;         // z-slices per block (a 64x8x16 brick)
+constexpr int TOPO_THREADS = 64 * TOPO_BY;
+constexpr int TOPO_TABLE = 512;     // LDS hash slots (4 KiB): a brick has ~80 distinct pairs
 constexpr int TOPO_PROBES = 32;     // linear-probe limit before a key bypasses the table
 constexpr int TOPO_FACES = 256;     // per-wave list of differing faces (fused kernel)
 constexpr uint64_t kEmpty = ~0ull;
@@ -42,10 +44,10 @@ struct SegIds {
   const int32_t* solid_id;
   int32_t* seg_out;
 };
-constexpr int TB_X = 65, TB_Y = 5, TB_Z = TOPO_ZB + 1;  // brick + forward halo
+constexpr int TB_X = 65, TB_Y = TOPO_BY + 1, TB_Z = TOPO_ZB + 1;  // brick + forward halo
 
 template <bool FUSED>
-__global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ seg, SegIds ids,
+__global__ __launch_bounds__(TOPO_THREADS) void k_topo_cand(const int32_t* __restrict__ seg, SegIds ids,
                                                    int nx, int ny, int nz, int32_t lo, int bits,
                                                    uint64_t* __restrict__ stage,
                                                    uint32_t* __restrict__ scnt,
@@ -55,15 +57,15 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   __shared__ int S[FUSED ? TB_X * (TB_Y * TB_Z + 1) : 1];  // + a spare row (see below)
   // flat offset of each region row; bit 31: a row of the brick inside the volume (writes seg)
   __shared__ uint32_t RBASE[FUSED ? TB_Y * TB_Z : 1];
-  __shared__ unsigned wc[4];
+  __shared__ unsigned wc[TOPO_BY];
   __shared__ unsigned s_base;
   // per wave: the brick's differing faces not yet looked up (S index | direction << 14)
-  __shared__ uint16_t PF[FUSED ? 4 * TOPO_FACES : 1];
+  __shared__ uint16_t PF[FUSED ? TOPO_BY * TOPO_FACES : 1];
   const int tid = threadIdx.y * 64 + threadIdx.x;
   const int lane = threadIdx.x, wave = threadIdx.y;
-  for (int s = tid; s < TOPO_TABLE; s += 256) table[s] = kEmpty;
+  for (int s = tid; s < TOPO_TABLE; s += TOPO_THREADS) table[s] = kEmpty;
   if (tid == 0) s_base = 0;
-  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, z0 = blockIdx.z * TOPO_ZB;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TOPO_BY, z0 = blockIdx.z * TOPO_ZB;
   constexpr int NROW = TB_Y * TB_Z;
   if (FUSED && tid < NROW) {  // wrapped flat offset of each row of the region, once per block
     const int ry = tid % TB_Y, rz = tid / TB_Y;
@@ -74,15 +76,16 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   }
   if (FUSED) __syncthreads();
   if (FUSED) {
-    // Rows of the brick + halo over the 4 waves, lane = column; the halo column 64 of the 85 rows
-    // is one more chain on threads 0..84, issued with the first batch. Positions past the volume
+    // Rows of the brick + halo over the waves, lane = column; the halo column 64 of the NROW rows
+    // is one more chain on threads 0..NROW-1, issued with the first batch. Positions past the volume
     // hold the wrapped voxel: the +x/+y/+z neighbour of the volume's last voxel is read there.
     // Brick voxels inside the volume also write seg.
     // Each seg source word gives the id (pore id of a flagged root, solid id of a label, 0 for
     // -1). Loads are issued in branch-free batches of RB rows into registers (rows past the
     // region re-read a valid row). Row offsets come from the table (flat offsets fit 32 bits:
     // volumes hold fewer than 2^31 voxels).
-    constexpr int RB = 11, MAXR = (NROW + 3) / 4;
+    constexpr int RB = 10, MAXR = (NROW + TOPO_BY - 1) / TOPO_BY;
+    static_assert(NROW <= TOPO_THREADS, "one halo chain per thread");
     const int gx = wrap_any(x0 + lane, nx), gxh = wrap_any(x0 + 64, nx);
     auto row_base = [&](int row) -> int64_t { return (int64_t)(RBASE[row] & 0x7fffffffu); };
     auto id_of = [&](int k) {
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
       int k[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
-        const int row0 = swave + (j0 + j) * 4;
+        const int row0 = swave + (j0 + j) * TOPO_BY;
         rb[j] = RBASE[row0 < NROW ? row0 : swave];
         k[j] = ids.segsrc[(rb[j] & 0x7fffffffu) + (uint32_t)gx];
       }
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
       // a branch lets the compiler sink that row's loads into the branch, one wait per load.
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
-        const int row = swave + (j0 + j) * 4;
+        const int row = swave + (j0 + j) * TOPO_BY;
         S[(row < NROW ? row : NROW) * TB_X + lane] = k[j];
         // a brick row inside the volume is its own unwrapped row: its offset plus gx is the
         // voxel's flat index
@@ -251,11 +254,11 @@ __global__ __launch_bounds__(256) void k_topo_cand(const int32_t* __restrict__ s
   }
   __syncthreads();
   const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  for (int s0 = 0; s0 < TOPO_TABLE; s0 += 256) {
+  for (int s0 = 0; s0 < TOPO_TABLE; s0 += TOPO_THREADS) {
     const uint64_t key = table[s0 + tid];
     const bool has = key != kEmpty;
     unsigned tot;
-    const unsigned pos = block_stage_pos<4>(has, wc, wave, lane, tot);
+    const unsigned pos = block_stage_pos<TOPO_BY>(has, wc, wave, lane, tot);
     if (has) stage[bid * TOPO_TABLE + s_base + pos] = key;
     __syncthreads();
     if (tid == 0) s_base += tot;
@@ -299,7 +302,7 @@ static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol 
     bits = nbits((uint64_t)((int64_t)hi_label - (int64_t)lo_label));
     if (bits < 1) bits = 1;
   }
-  const dim3 grid(cdiv(v.nx, 64), cdiv(v.ny, 4), cdiv(v.nz, TOPO_ZB));
+  const dim3 grid(cdiv(v.nx, 64), cdiv(v.ny, TOPO_BY), cdiv(v.nz, TOPO_ZB));
   const int64_t nblk = (int64_t)grid.x * grid.y * grid.z;
   const int64_t maxc = 3 * v.n;
   uint64_t* stage = c->arena.get_n<uint64_t>((size_t)(nblk * TOPO_TABLE));
@@ -311,11 +314,11 @@ static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol 
   // keys that find no slot in a brick's table go to `ovf`; the staged tables are compacted to
   // the head of `cand` and the overflow appended after them, on the device
   if (ids)
-    PNM_LAUNCH(c, "k_seg_topo", k_topo_cand<true>, grid, dim3(64, 4, 1), 0, seg, *ids, v.nx,
+    PNM_LAUNCH(c, "k_seg_topo", k_topo_cand<true>, grid, dim3(64, TOPO_BY, 1), 0, seg, *ids, v.nx,
                v.ny, v.nz, lo, bits, stage, scnt, ovf,
                reinterpret_cast<unsigned long long*>(cnt + 1));
   else
-    PNM_LAUNCH(c, "k_topo_cand", k_topo_cand<false>, grid, dim3(64, 4, 1), 0, seg, SegIds{},
+    PNM_LAUNCH(c, "k_topo_cand", k_topo_cand<false>, grid, dim3(64, TOPO_BY, 1), 0, seg, SegIds{},
                v.nx, v.ny, v.nz, lo, bits, stage, scnt, ovf,
                reinterpret_cast<unsigned long long*>(cnt + 1));
   compact_staged64(c, stage, scnt, nblk, TOPO_TABLE, cand, cnt);
