@@ -77,7 +77,6 @@ __device__ void block_wave_sort(uint32_t* k, uint32_t* v, uint32_t* k2, uint32_t
     }
     __syncthreads();
     if (wave == 0) {
-      const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
       for (unsigned r0 = 0; r0 < K; r0 += 64) {
         const unsigned j = r0 + lane;
         const bool valid = j < K;
@@ -90,7 +89,7 @@ __device__ void block_wave_sort(uint32_t* k, uint32_t* v, uint32_t* k2, uint32_t
           const uint64_t m = __ballot(bit);
           peers &= bit ? m : ~m;
         }
-        const unsigned rank = (unsigned)__popcll(peers & below);
+        const unsigned rank = lanes_below(peers);
         const unsigned base = valid ? hist[d] : 0u;
         if (valid) {
           k2[base + rank] = key;
@@ -443,8 +442,7 @@ __global__ __launch_bounds__(BS) void k_seg_batch(const BatchVol* __restrict__ v
         const unsigned j = r0 + lane;
         const bool valid = j < nsk && sdf[band[j]] <= 0.0f;
         const uint64_t m = __ballot(valid);
-        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const unsigned rank = carry + (unsigned)__popcll(m & below);
+        const unsigned rank = carry + lanes_below(m);
         if (j < nsk) nxt[band2[j]] = valid ? -(int32_t)(rank + 1) : 0;
         carry += (unsigned)__popcll(m);
       }

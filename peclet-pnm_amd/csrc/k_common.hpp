@@ -48,6 +48,13 @@ __device__ __forceinline__ int wrap_near(int v, int N) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Set bits of the wave mask m in the lanes below this one: v_mbcnt_lo/hi, two instructions (a
+// popcount of m & below-mask takes two ANDs and two bit counts).
+__device__ __forceinline__ unsigned lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Neighbouring lanes of a wave by a DPP wave shift (one VALU move, no LDS crossbar):
 // lane_prev gives lane - 1's value (lane 0 gets `first`), lane_next lane + 1's (lane 63 gets
 // `last`).
@@ -68,8 +75,7 @@ __device__ __forceinline__ uint64_t wave_append(unsigned long long* counter, boo
   unsigned long long base = 0;
   if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
   base = __shfl(base, leader, 64);
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  return base + (uint64_t)__popcll(m & below);
+  return base + (uint64_t)lanes_below(m);
 }
 
 // Coherent (L1-bypassing) load for union-find parents that other waves are lowering.
@@ -199,8 +205,7 @@ __device__ __forceinline__ unsigned block_stage_pos(bool pred, unsigned* wcnt, i
     tot += cc;
   }
   total = tot;
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  return base + (unsigned)__popcll(m & below);
+  return base + lanes_below(m);
 }
 
 // Wave-aggregated append to an LDS counter (batched kernels). Every lane must call it.
@@ -211,8 +216,7 @@ __device__ __forceinline__ unsigned wave_append_lds(unsigned* ctr, bool pred, in
   unsigned base = 0;
   if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(m));
   base = __shfl(base, leader, 64);
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  return base + (unsigned)__popcll(m & below);
+  return base + lanes_below(m);
 }
 
 // merge_markers (pore_extraction.hpp:198-224) over x-runs. Every marker carries the index of

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(64 * PRE_TY) void k_pre(const float* __restrict__ s
       }
     }
     const uint64_t pm = __ballot(is_peak);
-    if (is_peak) pstage[npk + __popcll(pm & below)] = (uint32_t)i;
+    if (is_peak) pstage[npk + lanes_below(pm)] = (uint32_t)i;
     npk += (unsigned)__popcll(pm);
     if (SEG) {
       const bool mk = col_in && (cv < thr);
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64 * PRE_TY) void k_pre(const float* __restrict__ s
       }
       const bool band = col_in && !(cv >= 0.0f) && !mk;
       const uint64_t bm = __ballot(band);
-      if (band) bstage[nbd + __popcll(bm & below)] = (uint32_t)i;
+      if (band) bstage[nbd + lanes_below(bm)] = (uint32_t)i;
       nbd += (unsigned)__popcll(bm);
     }
     m_prev = m_cur;

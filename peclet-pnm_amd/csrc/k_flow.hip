@@ -440,8 +440,7 @@ __global__ __launch_bounds__(256) void k_face_roots(const uint32_t* __restrict__
     if (w < wave) base += wc[w];
     tot += wc[w];
   }
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  if (is_root) stage[(int64_t)blockIdx.x * 256 + base + (unsigned)__popcll(m & below)] = (uint64_t)f;
+  if (is_root) stage[(int64_t)blockIdx.x * 256 + base + lanes_below(m)] = (uint64_t)f;
   if (threadIdx.x == 0) counts[blockIdx.x] = tot;
 }
 

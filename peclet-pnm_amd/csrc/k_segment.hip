@@ -100,7 +100,6 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
   const int nzt = min(TILE_Z, nz - z0);
   const int64_t seg_id =
       (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * TILE_Y + wave;
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   constexpr int H = FLOOD_SWEEPS;
   // load: labels (a marker's component is L[L[i]]: its tile-local root points at the component
   // root) and band candidates (the tile pass wrote -1 for a voxel that is no marker and not
@@ -185,7 +184,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
     const bool cand = zl < nzt && R[e] == -1;
     if (cand) cand0 |= 1u << zl;
     const uint64_t m = __ballot(cand);
-    const unsigned pos = nw + (unsigned)__popcll(m & below);
+    const unsigned pos = nw + lanes_below(m);
     if (cand && pos < (unsigned)WL) cand_list[wave * WL + pos] = (uint16_t)e;
     nw += (unsigned)__popcll(m);
   }
@@ -264,10 +263,10 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
     }
     const bool band = v == -1;
     const uint64_t km = __ballot(root);
-    if (root) key_stage[seg_id * FR_SEG + nk + __popcll(km & below)] = (uint32_t)i;
+    if (root) key_stage[seg_id * FR_SEG + nk + lanes_below(km)] = (uint32_t)i;
     nk += (unsigned)__popcll(km);
     const uint64_t bm = __ballot(band);
-    if (band) band_stage[seg_id * FR_SEG + nb + __popcll(bm & below)] = (uint32_t)i;
+    if (band) band_stage[seg_id * FR_SEG + nb + lanes_below(bm)] = (uint32_t)i;
     nb += (unsigned)__popcll(bm);
   }
   if (lane == 0) {

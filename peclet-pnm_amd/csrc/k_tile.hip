@@ -363,7 +363,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       }
       const bool lroot = col_in && r[zl] == l;
       const uint64_t rm = __ballot(lroot);
-      if (lroot) lstage[nr + __popcll(rm & below)] = (uint32_t)i;
+      if (lroot) lstage[nr + lanes_below(rm)] = (uint32_t)i;
       nr += (unsigned)__popcll(rm);
       // the chunk's first and last column (uniform lanes: read into scalar registers)
       const int a0 = __builtin_amdgcn_readlane(r[zl], 0);
@@ -420,7 +420,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   for (int zl = 0; zl < TILE_Z; ++zl) {
     const bool pb = (pore_mask >> zl) & 1u;
     const uint64_t pm = __ballot(pb);
-    const unsigned pos = na + (unsigned)__popcll(pm & below);
+    const unsigned pos = na + lanes_below(pm);
     if (pb && pos < (unsigned)ALIST_W) alist[pos] = (uint16_t)((zl * TILE_Y + wave) * 64 + lane);
     na += (unsigned)__popcll(pm);
   }
@@ -434,7 +434,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const int l = j < na ? (int)alist[j] : 0;
       const bool active = j < na && walk_init(l, l >> 9, l & 63);
       const uint64_t am = __ballot(active);
-      if (active) alist[nk + (unsigned)__popcll(am & below)] = (uint16_t)l;
+      if (active) alist[nk + lanes_below(am)] = (uint16_t)l;
       nk += (unsigned)__popcll(am);
     }
     na = nk;
@@ -468,7 +468,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
         const int l = j < na ? (int)alist[j] : 0;
         const bool still = j < na && jump(l);
         const uint64_t sm = __ballot(still);
-        if (still) alist[nk + (unsigned)__popcll(sm & below)] = (uint16_t)l;
+        if (still) alist[nk + lanes_below(sm)] = (uint16_t)l;
         nk += (unsigned)__popcll(sm);
       }
       na = nk;
@@ -571,7 +571,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
   for (int zl = 0; zl < TILE_Z; ++zl) {
     const bool pb = (pore_mask >> zl) & 1u;
     const uint64_t pm = __ballot(pb);
-    const unsigned pos = np + (unsigned)__popcll(pm & below);
+    const unsigned pos = np + lanes_below(pm);
     if (pb && pos < (unsigned)PLIST_W) plist[pos] = (uint16_t)((zl * TILE_Y + wave) * 64 + lane);
     np += (unsigned)__popcll(pm);
   }
@@ -583,7 +583,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       const bool open = j < np && walk_out(li, li >> 9, li & 63, rec);
       const uint64_t om = __ballot(open);
       if (open)
-        ostage[nop + __popcll(om & below)] =
+        ostage[nop + lanes_below(om)] =
             open_entry(rec, li >> 9, li & 63);
       nop += (unsigned)__popcll(om);
     }
@@ -597,7 +597,7 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       bool open = false;
       if (has) open = walk_out((zl * TILE_Y + wave) * 64 + lane, zl, lane, rec);
       const uint64_t om = __ballot(open);
-      if (open) ostage[nop + __popcll(om & below)] = open_entry(rec, zl, lane);
+      if (open) ostage[nop + lanes_below(om)] = open_entry(rec, zl, lane);
       nop += (unsigned)__popcll(om);
       if (has) m &= m - 1;
     }
@@ -766,11 +766,10 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int2* l
     R.my = ok && in ? L[i] : -1;
   };
   int2* const pl = list + wave * SEAM_LIST;
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   unsigned np = 0;
   auto add = [&](bool v, int a, int b) {
     const uint64_t vm = __ballot(v);
-    if (v) pl[np + (unsigned)__popcll(vm & below)] = make_int2(a, b);
+    if (v) pl[np + lanes_below(vm)] = make_int2(a, b);
     np += (unsigned)__popcll(vm);
   };
   auto flush = [&]() {

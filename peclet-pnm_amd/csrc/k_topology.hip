@@ -171,7 +171,6 @@ __global__ __launch_bounds__(TOPO_THREADS) void k_topo_cand(const int32_t* __res
     // on full waves. A listed face whose key equals the previous entry's (a boundary run along
     // x) skips the set; repeats along z find their key at the first probe.
     uint16_t* const pf = PF + wave * TOPO_FACES;
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     unsigned np = 0;
     auto flush = [&]() {
       __builtin_amdgcn_wave_barrier();
@@ -209,7 +208,7 @@ __global__ __launch_bounds__(TOPO_THREADS) void k_topo_cand(const int32_t* __res
       for (int k = 0; k < 3; ++k) {
         const bool d = act && my != nb[k];
         const uint64_t dm = __ballot(d);
-        if (d) pf[np + (unsigned)__popcll(dm & below)] = (uint16_t)(e | (k << 14));
+        if (d) pf[np + lanes_below(dm)] = (uint16_t)(e | (k << 14));
         np += (unsigned)__popcll(dm);
       }
       if (np > (unsigned)(TOPO_FACES - 3 * 64)) flush();  // wave-uniform

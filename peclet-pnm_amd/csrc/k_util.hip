@@ -476,7 +476,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
   __shared__ uint32_t wbase[RS_WAVES][NB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int d = t; d < NB; d += RS_THREADS) gbase[d] = ghs[(int64_t)d * nblocks + blockIdx.x];
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t i = base + r * RS_THREADS + t;
@@ -490,7 +489,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    const uint32_t rank = (uint32_t)__popcll(peers & below);
+    const uint32_t rank = lanes_below(peers);
     for (int dd = t; dd < NB; dd += RS_THREADS) {
 #pragma unroll
       for (int q = 0; q < RS_WAVES; ++q) cnt[q][dd] = 0;
