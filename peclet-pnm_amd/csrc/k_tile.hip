@@ -643,13 +643,6 @@ __device__ __forceinline__ void seam_rows(int y, int z, int ny, int nz, int fy[4
   same[2] = tz_p;
   same[3] = ty_p && tz_p;
 }
-__device__ __forceinline__ bool y_boundary(int y, int ny) {
-  return y % TILE_Y == 0 || y % TILE_Y == TILE_Y - 1 || y == ny - 1;
-}
-__device__ __forceinline__ int y_boundary_row(int j, int ny) {  // j < 2 * ceil(ny / TILE_Y)
-  const int y = (j >> 1) * TILE_Y + ((j & 1) ? TILE_Y - 1 : 0);
-  return y < ny ? y : ny - 1;
-}
 __device__ __forceinline__ int z_boundary_slice(int j, int nz) {  // j < ceil(nz / TILE_Z)
   const int z = j * TILE_Z + TILE_Z - 1;
   return z < nz ? z : nz - 1;
@@ -705,74 +698,19 @@ __device__ __forceinline__ void seam_flush(unsigned long long* tab, int* L) {
   if (v1) seam_unite_from(L, a1, b1, pa1, pb1, o1);
 }
 
-// Full run-union rule on the crossing forward rows of row chunks, one row per wave at a time.
-// mode 0: rows at a y boundary (boundary row by), slices bz * SEAM_ZR .. + SEAM_ZR - 1;
-// mode 1: slices at a z boundary (bz), rows of the SEAM_YT y tiles from by * SEAM_YT that are
-// not at a y boundary.
-// A block takes many rows, so that its pair set merges the repeats of more rows and the launch
-// has fewer blocks.
-// A row's contacts with a crossing forward row: each voxel with the voxel of the same column,
-// and a run's first and last voxel with the columns just outside the run (interior columns of
-// a run reach those through their own same-column contact). Those outside columns are held by
-// the neighbouring lanes, so only the chunk's edge columns are loaded, together with the rows:
-// one round of loads per row.
-// The contacts found are listed per wave (by ballot, in LDS) and go through the pair set 64 at
-// a time, so that the set's probing runs on full waves and has one call site.
-constexpr int SEAM_ZR = 64;     // mode 0: slices per block
-constexpr int SEAM_YT = 4;      // mode 1: y tiles per block
+// A wave's contacts, listed by ballot in LDS and put through the pair set 64 at a time, so that
+// the set's probing runs on full waves and has one call site.
 constexpr int SEAM_LIST = 256;  // per-wave list of contacts
-__device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int2* list, int* L,
-                                                int nx, int ny, int nz, int mode, int nsel,
-                                                int bx, int by, int bz, int lane, int wave) {
-  const int x0 = bx * 64;
-  const int x = x0 + lane;
-  const bool in = x < nx;
-  const int xlast = min(x0 + 63, nx - 1);
-  // the column outside the chunk this lane may need: lane 0 the one before the chunk, the last
-  // column the one after it (a one-column chunk loads the latter when it needs it)
-  const bool edge = in && (lane == 0 || x == xlast);
-  const int xe = lane == 0 ? wm1(x, nx) : wp1(x, nx);
-  const int nrow = mode == 0 ? SEAM_ZR : SEAM_YT * TILE_Y;
-  struct Row {
-    int fy[4], fz[4], fwd[4], fe[4], my;
-    bool same[4], ok;
-  };
-  // Row k of the block's share: its crossing forward rows, and the loads (all unconditional; a
-  // row not taken, and rows not needed, re-read L[0] or the row itself).
-  auto load = [&](int k, Row& R) {
-    int y, z;
-    bool ok = k < nrow;
-    if (mode == 0) {
-      y = y_boundary_row(by < nsel ? by : 0, ny);
-      z = bz * SEAM_ZR + k;
-      ok = ok && by < nsel && z < nz;
-    } else {
-      y = by * (SEAM_YT * TILE_Y) + k;
-      ok = ok && y < ny && !y_boundary(y, ny);
-      z = z_boundary_slice(bz, nz);
-    }
-    if (!ok) y = z = 0;
-    seam_rows(y, z, ny, nz, R.fy, R.fz, R.same);
-    ok = ok && !(R.same[0] && R.same[1] && R.same[2] && R.same[3]);
-    R.ok = ok;  // wave-uniform
-    const int i = ok && in ? (z * ny + y) * nx + x : 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int rb = (R.fz[r] * ny + R.fy[r]) * nx;
-      const bool skip = !ok || R.same[r] || !in;
-      R.fwd[r] = L[skip ? i : rb + x];
-      R.fe[r] = L[(skip || !edge) ? i : rb + xe];
-    }
-    R.my = ok && in ? L[i] : -1;
-  };
-  int2* const pl = list + wave * SEAM_LIST;
-  unsigned np = 0;
-  auto add = [&](bool v, int a, int b) {
+struct SeamList {
+  int2* pl;
+  unsigned np;
+  int lane;
+  __device__ __forceinline__ void add(bool v, int a, int b) {
     const uint64_t vm = __ballot(v);
     if (v) pl[np + lanes_below(vm)] = make_int2(a, b);
     np += (unsigned)__popcll(vm);
-  };
-  auto flush = [&]() {
+  }
+  __device__ __forceinline__ void flush(unsigned long long* tab, int* L) {
     __builtin_amdgcn_wave_barrier();
     for (unsigned j = lane; j < np; j += 64) {
       const int2 q = pl[j];
@@ -780,46 +718,155 @@ __device__ __forceinline__ void seam_rows_block(unsigned long long* tab, int2* l
     }
     __builtin_amdgcn_wave_barrier();
     np = 0;
+  }
+};
+static_assert(SEAM_LIST >= 4 * 64, "room for one row pair's contacts after the flush test");
+
+// Contacts of row P with an adjacent row Q (the next or previous row in y, z or both) in one row
+// chunk: each P marker with the Q voxel of its column, and where that is no marker a run's first
+// and last P voxel with the Q voxels just before and after the run (interior voxels of a run
+// reach those through their own same-column contacts; a run has one label in its chunk). my / c:
+// this lane's P / Q label word (a marker's label >= 0, a negative code otherwise); m: P's marker
+// ballot; qe: Q's label just outside the chunk (lane 0: column x0-1, the chunk's last column:
+// xlast+1); qrow: Q's flat row offset (a one-column chunk loads Q's xlast+1 on lane 0 from it).
+__device__ __forceinline__ void seam_contacts(SeamList& sl, unsigned long long* tab, int* L,
+                                              int my, uint64_t m, int c, int qe, int qrow, int x,
+                                              int nx, bool last_col, int lane) {
+  const bool mk = my >= 0;
+  const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
+  const bool run_end = mk && (last_col || !((m >> (lane + 1)) & 1ull));
+  const bool cm = c >= 0;
+  const int prev_c = __shfl_up(c, 1, 64);
+  const int next_c = __shfl_down(c, 1, 64);
+  const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
+  sl.add(mk && cm && !dup, my, c);
+  int a = -1, b = -1;
+  if (mk && !cm && run_start) a = lane == 0 ? qe : prev_c;
+  if (mk && !cm && run_end) b = !last_col ? next_c : (lane != 0 ? qe : L[qrow + wp1(x, nx)]);
+  sl.add(a >= 0, my, a);
+  sl.add(b >= 0, my, b);
+  if (sl.np > (unsigned)(SEAM_LIST - 3 * 64)) sl.flush(tab, L);  // wave-uniform
+}
+
+// y seams. Seam s joins the last row ya of tile row s-1 and the first row yb of tile row s (s = 0:
+// the volume's last row and row 0, across the periodic wrap). With A = (ya, z) and B = (yb, z),
+// the relations A(z)-B(z), A(z)-B(z+1) and B(z)-A(z+1) are all those that cross a y tile
+// boundary. A wave walks SEAM_ZR / 4 consecutive slices of the seam with the next slice's two rows
+// loaded ahead, so that every row is loaded once (with its outside columns) for the three
+// relations it takes part in.
+constexpr int SEAM_ZR = 64;  // slices per block
+__device__ __forceinline__ void seam_y_block(unsigned long long* tab, int2* list, int* L, int nx,
+                                             int ny, int nz, int bx, int s, int bz, int lane,
+                                             int wave) {
+  const int x0 = bx * 64, x = x0 + lane;
+  const bool in = x < nx;
+  const int xlast = min(x0 + 63, nx - 1);
+  const bool last_col = x == xlast;
+  // the column outside the chunk this lane may need: lane 0 the one before the chunk, the last
+  // column the one after it (a one-column chunk loads the latter when it needs it)
+  const bool edge = in && (lane == 0 || last_col);
+  const int xe = lane == 0 ? wm1(x, nx) : wp1(x, nx);
+  const int ya = s == 0 ? ny - 1 : s * TILE_Y - 1, yb = s == 0 ? 0 : s * TILE_Y;
+  constexpr int ZW = SEAM_ZR / 4;
+  const int zs = bz * SEAM_ZR + wave * ZW, ze = min(zs + ZW, nz);
+  SeamList sl{list + wave * SEAM_LIST, 0u, lane};
+  // a row's label word and outside-column label; loads unconditional (a lane without a column,
+  // or without an outside column, re-reads its row's first word)
+  auto load = [&](int y, int z, int& v, int& e) {
+    const int rb = (z * ny + y) * nx;
+    const int lv = L[in ? rb + x : rb], le = L[edge ? rb + xe : rb];
+    v = in ? lv : -1;
+    e = edge ? le : -1;
   };
-  static_assert(SEAM_LIST >= 4 * 64, "room for a forward row's contacts after the flush test");
-  // label words: a marker's label (>= 0) or a negative code (-1, -2) for every other voxel
+  int va = -1, ea = -1, vb = -1, eb = -1;
+  if (zs < ze) {  // wave-uniform
+    load(ya, zs, va, ea);
+    load(yb, zs, vb, eb);
+  }
+  for (int z = zs; z < ze; ++z) {
+    const int z1 = wp1(z, nz);
+    int va1, ea1, vb1, eb1;
+    load(ya, z1, va1, ea1);
+    load(yb, z1, vb1, eb1);
+    const uint64_t ma = __ballot(va >= 0), mb = __ballot(vb >= 0);
+    if (ma) {  // wave-uniform
+      seam_contacts(sl, tab, L, va, ma, vb, eb, (z * ny + yb) * nx, x, nx, last_col, lane);
+      seam_contacts(sl, tab, L, va, ma, vb1, eb1, (z1 * ny + yb) * nx, x, nx, last_col, lane);
+    }
+    if (mb) seam_contacts(sl, tab, L, vb, mb, va1, ea1, (z1 * ny + ya) * nx, x, nx, last_col, lane);
+    va = va1;
+    ea = ea1;
+    vb = vb1;
+    eb = eb1;
+  }
+  sl.flush(tab, L);
+}
+
+// z seams: every row (y, z) of a slice z at a z tile boundary (z % TILE_Z = TILE_Z-1, or the last
+// slice) with its forward rows in the next slice (y-1, z+1), (y, z+1), (y+1, z+1), except those
+// in another y tile row (the y seams take them). A block takes the rows of SEAM_YT y tiles, one
+// row per wave at a time, two rows in flight.
+constexpr int SEAM_YT = 4;  // y tiles per block
+__device__ __forceinline__ void seam_z_block(unsigned long long* tab, int2* list, int* L, int nx,
+                                             int ny, int nz, int bx, int by, int bz, int lane,
+                                             int wave) {
+  const int x0 = bx * 64, x = x0 + lane;
+  const bool in = x < nx;
+  const int xlast = min(x0 + 63, nx - 1);
+  const bool last_col = x == xlast;
+  const bool edge = in && (lane == 0 || last_col);
+  const int xe = lane == 0 ? wm1(x, nx) : wp1(x, nx);
+  const int z = z_boundary_slice(bz, nz), zp = wp1(z, nz);
+  constexpr int NROW = SEAM_YT * TILE_Y;
+  struct Row {
+    int fy[3], fwd[3], fe[3], my;
+    bool take[3], ok;
+  };
+  auto load = [&](int k, Row& R) {
+    int y = by * NROW + k;
+    const bool ok = k < NROW && y < ny;  // wave-uniform
+    if (!ok) y = 0;
+    const bool ty_p = (y + 1 < ny) && ((y + 1) / TILE_Y == y / TILE_Y);
+    const bool ty_m = (y >= 1) && ((y - 1) / TILE_Y == y / TILE_Y);
+    R.fy[0] = wm1(y, ny);
+    R.fy[1] = y;
+    R.fy[2] = wp1(y, ny);
+    R.take[0] = ok && ty_m;
+    R.take[1] = ok;
+    R.take[2] = ok && ty_p;
+    R.ok = ok;
+    const int i = ok && in ? (z * ny + y) * nx + x : 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int rb = (zp * ny + R.fy[r]) * nx;
+      const bool skip = !R.take[r] || !in;
+      R.fwd[r] = L[skip ? i : rb + x];
+      R.fe[r] = L[(skip || !edge) ? i : rb + xe];
+    }
+    R.my = ok && in ? L[i] : -1;
+  };
+  SeamList sl{list + wave * SEAM_LIST, 0u, lane};
   auto pairs = [&](const Row& R) {
     if (!R.ok) return;
-    const int my = R.my;
-    const bool mk = my >= 0;
-    const uint64_t m = __ballot(mk);
+    const uint64_t m = __ballot(R.my >= 0);
     if (m == 0) return;
-    const bool run_start = mk && (lane == 0 || !((m >> (lane - 1)) & 1ull));
-    const bool run_end = mk && (x == xlast || !((m >> (lane + 1)) & 1ull));
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (R.same[r]) continue;  // wave-uniform; edges of same-tile rows: the x-face pass
-      const int c = in ? R.fwd[r] : -1;
-      const bool cm = c >= 0;
-      const int prev_c = __shfl_up(c, 1, 64);
-      const int next_c = __shfl_down(c, 1, 64);
-      const bool dup = lane > 0 && ((m >> (lane - 1)) & 1ull) && prev_c == c;
-      add(mk && cm && !dup, my, c);
-      int a = -1, b = -1;
-      if (mk && !cm && run_start) a = lane == 0 ? R.fe[r] : prev_c;
-      if (mk && !cm && run_end)
-        b = x != xlast ? next_c
-                       : (lane != 0 ? R.fe[r] : L[(R.fz[r] * ny + R.fy[r]) * nx + wp1(x, nx)]);
-      add(a >= 0, my, a);
-      add(b >= 0, my, b);
-      if (np > (unsigned)(SEAM_LIST - 3 * 64)) flush();  // wave-uniform
+    for (int r = 0; r < 3; ++r) {
+      if (!R.take[r]) continue;  // wave-uniform
+      seam_contacts(sl, tab, L, R.my, m, in ? R.fwd[r] : -1, R.fe[r], (zp * ny + R.fy[r]) * nx, x,
+                    nx, last_col, lane);
     }
   };
   // two rows of the wave at a time: the second row's loads are in flight while the first's
   // pairs are collected
-  for (int k = wave; k < nrow; k += 8) {
+  for (int k = wave; k < NROW; k += 8) {
     Row A, B;
     load(k, A);
     load(k + 4, B);
     pairs(A);
     pairs(B);
   }
-  flush();
+  sl.flush(tab, L);
 }
 
 // One thread per row chunk, from the x-face labels k_tile wrote (entry (chunk, z, y): labels of
@@ -861,8 +908,8 @@ __device__ __forceinline__ void seam_edges_block(unsigned long long* tab, int* L
   }
 }
 
-// All cross-tile and wrapping marker relations in one launch: blocks [0, n0) take the rows at
-// a y boundary, [n0, n0 + n1) the slices at a z boundary, the rest the x-face relations.
+// All cross-tile and wrapping marker relations in one launch: blocks [0, n0) take the y seams,
+// [n0, n0 + n1) the z seams, the rest the x-face relations.
 __global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restrict__ xf, int nx,
                                                    int ny, int nz, int cx, int ysel, int zsel,
                                                    int64_t n0, int64_t n1) {
@@ -873,15 +920,14 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restric
   __syncthreads();
   const int64_t b = blockIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  if (b < n0) {  // grid (cx, ysel, ceil(nz / SEAM_ZR))
+  if (b < n0) {  // grid (cx, ysel = ceil(ny / TILE_Y) seams, ceil(nz / SEAM_ZR))
     const int bx = (int)(b % cx), rest = (int)(b / cx);
-    seam_rows_block(tab, list, L, nx, ny, nz, 0, ysel, bx, rest % ysel, rest / ysel, lane,
-                    wave);
+    seam_y_block(tab, list, L, nx, ny, nz, bx, rest % ysel, rest / ysel, lane, wave);
   } else if (b < n0 + n1) {  // grid (cx, ceil(ny / (SEAM_YT * TILE_Y)), zsel)
     const int64_t bb = b - n0;
     const int nyt = (ny + SEAM_YT * TILE_Y - 1) / (SEAM_YT * TILE_Y);
     const int bx = (int)(bb % cx), rest = (int)(bb / cx);
-    seam_rows_block(tab, list, L, nx, ny, nz, 1, zsel, bx, rest % nyt, rest / nyt, lane, wave);
+    seam_z_block(tab, list, L, nx, ny, nz, bx, rest % nyt, rest / nyt, lane, wave);
   } else {
     seam_edges_block(tab, L, xf, ny, nz, cx, b - n0 - n1, tid);
   }
@@ -1067,7 +1113,7 @@ void launch_tile(Ctx* c, const float* sdf, Vol v, float thr, const TileOut& o) {
 
 void launch_ccl_seams(Ctx* c, Vol v, int32_t* labels, const int32_t* xface) {
   const int cx = (int)cdiv(v.nx, 64);
-  const int ysel = 2 * (int)cdiv(v.ny, TILE_Y), zsel = (int)cdiv(v.nz, TILE_Z);
+  const int ysel = (int)cdiv(v.ny, TILE_Y), zsel = (int)cdiv(v.nz, TILE_Z);  // y seams, z seams
   const int64_t n0 = (int64_t)cx * ysel * cdiv(v.nz, SEAM_ZR);
   const int64_t n1 = (int64_t)cx * cdiv(v.ny, SEAM_YT * TILE_Y) * zsel;
   const int64_t n2 = cdiv((int64_t)cx * v.ny * v.nz, 256);
