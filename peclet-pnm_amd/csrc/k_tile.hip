@@ -457,8 +457,13 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
     LAB[l] = (int)cs_pack(cs_u(wv), s2, sv);
     return sv == CS_ACT;
   };
+  // "any walk still active" per round with one barrier: the waves' flags of round r go to
+  // s_more[r % 3], and thread 0 clears the slot of round r + 1, last read in round r - 2, before
+  // the barrier of round r (every wave has passed that of round r - 1, so those reads are done)
+  __shared__ unsigned s_more[3];
+  if (tid < 3) s_more[tid] = 0u;
   __syncthreads();
-  while (true) {
+  for (int round = 0;; ++round) {
     bool any;
     if (listed) {
       unsigned nk = 0;
@@ -480,7 +485,11 @@ void k_tile(const float* __restrict__ sdf, int nx, int ny, int nz, float thr, Ti
       }
       any = act != 0;
     }
-    if (!__syncthreads_or(any)) break;
+    const int slot = round % 3;
+    if (__any(any) && lane == 0) s_more[slot] = 1u;
+    if (tid == 0) s_more[slot == 2 ? 0 : slot + 1] = 0u;
+    __syncthreads();
+    if (s_more[slot] == 0u) break;
   }
 
   // ---- walk outputs; LAB at each fixed point collects the smallest local index whose walk
