@@ -471,13 +471,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     uint32_t* __restrict__ ovals, int64_t n, int shift, const uint32_t* __restrict__ ghs,
     unsigned nblocks) {
   constexpr int NB = 1 << DB;
+  // per-wave digit counts and bases, double-buffered by round parity: a round's prefix zeroes
+  // the other parity's counts (last read by the previous round's prefix, before this round's
+  // first barrier), so each round takes two barriers
   __shared__ uint32_t gbase[NB];
-  __shared__ uint32_t cnt[RS_WAVES][NB];
-  __shared__ uint32_t wbase[RS_WAVES][NB];
+  __shared__ uint32_t cnt[2][RS_WAVES][NB];
+  __shared__ uint32_t wbase[2][RS_WAVES][NB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  for (int d = t; d < NB; d += RS_THREADS) gbase[d] = ghs[(int64_t)d * nblocks + blockIdx.x];
+  for (int d = t; d < NB; d += RS_THREADS) {
+    gbase[d] = ghs[(int64_t)d * nblocks + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q) cnt[0][q][d] = 0;
+  }
+  __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   for (int r = 0; r < RS_ITEMS; ++r) {
+    const int p = r & 1;
     const int64_t i = base + r * RS_THREADS + t;
     const bool valid = i < n;
     const K k = valid ? keys[i] : (K)0;
@@ -490,25 +499,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
       peers &= bit ? m : ~m;
     }
     const uint32_t rank = lanes_below(peers);
-    for (int dd = t; dd < NB; dd += RS_THREADS) {
-#pragma unroll
-      for (int q = 0; q < RS_WAVES; ++q) cnt[q][dd] = 0;
-    }
-    __syncthreads();
-    if (valid && rank == 0) cnt[w][d] = (uint32_t)__popcll(peers);
+    if (valid && rank == 0) cnt[p][w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
     for (int dd = t; dd < NB; dd += RS_THREADS) {
       uint32_t s = gbase[dd];
 #pragma unroll
       for (int q = 0; q < RS_WAVES; ++q) {
-        wbase[q][dd] = s;
-        s += cnt[q][dd];
+        wbase[p][q][dd] = s;
+        s += cnt[p][q][dd];
+        cnt[p ^ 1][q][dd] = 0;
       }
       gbase[dd] = s;
     }
     __syncthreads();
     if (valid) {
-      const uint32_t pos = wbase[w][d] + rank;
+      const uint32_t pos = wbase[p][w][d] + rank;
       okeys[pos] = k;
       if (VALS) ovals[pos] = vals[i];
     }
