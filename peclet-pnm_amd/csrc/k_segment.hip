@@ -322,14 +322,17 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
                                                      int32_t* __restrict__ newlab,
                                                      uint32_t* __restrict__ stage,
                                                      uint32_t* __restrict__ cnt, int64_t seg_cap) {
-  __shared__ unsigned wc[4];
+  // wave counts of the staging, by round parity: a round writes the set the previous round did
+  // not read, so its one barrier (inside block_stage_pos) orders it with both neighbours
+  __shared__ unsigned wc[2][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t nband = (int64_t)*count;
   const int64_t share = (nband + gridDim.x - 1) / gridDim.x;
   const int64_t j_begin = (int64_t)blockIdx.x * share;
   const int64_t j_end = min(nband, j_begin + share);
   unsigned base = 0;
-  for (int64_t j0 = j_begin; j0 < j_end; j0 += 256) {  // block-uniform
+  int par = 0;
+  for (int64_t j0 = j_begin; j0 < j_end; j0 += 256, par ^= 1) {  // block-uniform
     const int64_t j = j0 + threadIdx.x;
     bool keep = false;
     uint32_t i = 0;
@@ -363,10 +366,9 @@ __global__ __launch_bounds__(256) void k_flood_sweep(const int32_t* __restrict__
       newlab[j] = keep ? -1 : (int)best;
     }
     unsigned tot;
-    const unsigned pos = block_stage_pos<4>(keep, wc, wave, lane, tot);
+    const unsigned pos = block_stage_pos<4>(keep, wc[par], wave, lane, tot);
     if (keep) stage[(int64_t)blockIdx.x * seg_cap + base + pos] = i;
     base += tot;
-    __syncthreads();  // wc is rewritten by the next round
   }
   if (threadIdx.x == 0) cnt[blockIdx.x] = base;
 }

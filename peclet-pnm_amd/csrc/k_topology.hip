@@ -57,14 +57,14 @@ __global__ __launch_bounds__(TOPO_THREADS) void k_topo_cand(const int32_t* __res
   __shared__ int S[FUSED ? TB_X * (TB_Y * TB_Z + 1) : 1];  // + a spare row (see below)
   // flat offset of each region row; bit 31: a row of the brick inside the volume (writes seg)
   __shared__ uint32_t RBASE[FUSED ? TB_Y * TB_Z : 1];
-  __shared__ unsigned wc[TOPO_BY];
-  __shared__ unsigned s_base;
+  // wave counts of the staging, by round parity (a round writes the set the previous one did
+  // not read, so block_stage_pos's one barrier orders it)
+  __shared__ unsigned wc[2][TOPO_BY];
   // per wave: the brick's differing faces not yet looked up (S index | direction << 14)
   __shared__ uint16_t PF[FUSED ? TOPO_BY * TOPO_FACES : 1];
   const int tid = threadIdx.y * 64 + threadIdx.x;
   const int lane = threadIdx.x, wave = threadIdx.y;
   for (int s = tid; s < TOPO_TABLE; s += TOPO_THREADS) table[s] = kEmpty;
-  if (tid == 0) s_base = 0;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TOPO_BY, z0 = blockIdx.z * TOPO_ZB;
   constexpr int NROW = TB_Y * TB_Z;
   if (FUSED && tid < NROW) {  // wrapped flat offset of each row of the region, once per block
@@ -253,17 +253,16 @@ __global__ __launch_bounds__(TOPO_THREADS) void k_topo_cand(const int32_t* __res
   }
   __syncthreads();
   const int64_t bid = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  for (int s0 = 0; s0 < TOPO_TABLE; s0 += TOPO_THREADS) {
+  unsigned sbase = 0;  // block-uniform
+  for (int s0 = 0, par = 0; s0 < TOPO_TABLE; s0 += TOPO_THREADS, par ^= 1) {
     const uint64_t key = table[s0 + tid];
     const bool has = key != kEmpty;
     unsigned tot;
-    const unsigned pos = block_stage_pos<TOPO_BY>(has, wc, wave, lane, tot);
-    if (has) stage[bid * TOPO_TABLE + s_base + pos] = key;
-    __syncthreads();
-    if (tid == 0) s_base += tot;
-    __syncthreads();
+    const unsigned pos = block_stage_pos<TOPO_BY>(has, wc[par], wave, lane, tot);
+    if (has) stage[bid * TOPO_TABLE + sbase + pos] = key;
+    sbase += tot;
   }
-  if (tid == 0) scnt[bid] = s_base;
+  if (tid == 0) scnt[bid] = sbase;
 }
 
 // Appends the overflow keys after the staged ones (both counts on the device).
