@@ -227,21 +227,17 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
         for (int rx = lane; rx < FR_X; rx += 64) sweep_one(row * FR_X + rx, k);
     }
     __syncthreads();
-    if (listed) {
-      for (unsigned j = lane; j < nw; j += 64) {
-        const int e = cand_list[wave * WL + j];
-        const int v = R[e];
-        if (v <= -3) R[e] = -3 - v;
-      }
-    } else {
+    // the last sweep's labels are decoded where the outputs read them (each wave reads its own
+    // row); the region scan applies every sweep, as its later sweeps read the region
+    if (!listed || k < H) {
       for (int row = wave; row < rows; row += TILE_Y)
         for (int rx = lane; rx < FR_X; rx += 64) {
           const int e = row * FR_X + rx;
           const int v = R[e];
           if (v <= -3) R[e] = -3 - v;
         }
+      __syncthreads();
     }
-    __syncthreads();
   }
   // tile outputs, staged per wave in (slice, lane) order
   const int x = x0 + lane, y = y0 + wave;
@@ -252,7 +248,8 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
     if (zl >= nzt) break;
     const int e = ((zl + H) * FR_Y + wave + H) * FR_X + lane + H;
     const int i = ((z0 + zl) * ny + y) * nx + x;
-    const int v = col_in ? R[e] : -2;
+    const int rv = col_in ? R[e] : -2;
+    const int v = rv <= -3 ? -3 - rv : rv;  // labelled by the last sweep
     bool root = false;
     if (col_in) {
       // -2: sdf >= 0 and unlabelled, a voxel whose seg source word the tile pass or the root
