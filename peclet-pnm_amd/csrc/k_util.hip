@@ -690,29 +690,58 @@ void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int
 }
 
 // ---------------- unique of a sorted list ----------------
-__global__ void k_unique_flags(const uint64_t* __restrict__ k, int64_t n,
-                               uint32_t* __restrict__ f) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) f[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
-}
-__global__ void k_unique_write(const uint64_t* __restrict__ k, int64_t n,
-                               const uint32_t* __restrict__ f, const uint32_t* __restrict__ pos,
-                               uint64_t* __restrict__ out, uint64_t* __restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && f[i]) out[pos[i]] = k[i];
-  if (i == n - 1) *count = (uint64_t)pos[i] + f[i];
+// One launch: each thread takes UQ_ITEMS consecutive keys and counts those that differ from
+// their predecessor, the look-back scan gives its output position, and it writes them in
+// order; the last block writes the count.
+constexpr int UQ_ITEMS = 8;
+__global__ __launch_bounds__(SCAN_THREADS) void k_unique(const uint64_t* __restrict__ k, int64_t n,
+                                                        uint64_t* __restrict__ out,
+                                                        uint64_t* __restrict__ total_out,
+                                                        uint32_t* __restrict__ ctl,
+                                                        uint64_t* __restrict__ state,
+                                                        unsigned nblocks,
+                                                        uint32_t* __restrict__ err) {
+  __shared__ uint32_t lds[SCAN_THREADS / 64];
+  __shared__ uint32_t s_bid, s_epoch, s_excl;
+  uint32_t bid, epoch;
+  lb_begin(ctl, bid, epoch, &s_bid, &s_epoch);
+  const int64_t base = ((int64_t)bid * SCAN_THREADS + threadIdx.x) * UQ_ITEMS;
+  uint64_t v[UQ_ITEMS];
+  load_run<UQ_ITEMS>(k, base, n, v);
+  const uint64_t before = (base > 0 && base <= n) ? k[base - 1] : 0ull;
+  uint32_t f = 0;  // bit q: item q starts a new key
+  uint32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < UQ_ITEMS; ++q) {
+    const int64_t i = base + q;
+    const bool nw = i < n && (i == 0 || v[q] != (q == 0 ? before : v[q - 1]));
+    f |= (uint32_t)nw << q;
+    s += nw;
+  }
+  uint32_t tot;
+  const uint32_t pre = block_excl_scan(s, lds, &tot);
+  if (threadIdx.x < 64) {
+    const uint32_t excl = lb_lookback(state, bid, epoch, tot, err);
+    if (threadIdx.x == 0) s_excl = excl;
+  }
+  __syncthreads();
+  uint32_t r = s_excl + pre;
+#pragma unroll
+  for (int q = 0; q < UQ_ITEMS; ++q)
+    if ((f >> q) & 1u) out[r++] = v[q];
+  if (threadIdx.x == 0) {
+    if (bid == nblocks - 1) *total_out = (uint64_t)s_excl + tot;
+    lb_end(ctl, epoch, nblocks);
+  }
 }
 
 int64_t unique_sorted_u64(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n) {
   if (n <= 0) return 0;
-  uint32_t* f = c->arena.get_n<uint32_t>((size_t)n);
-  uint32_t* pos = c->arena.get_n<uint32_t>((size_t)n);
   uint64_t* cnt = c->arena.get_n<uint64_t>(1);
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  PNM_LAUNCH(c, "k_unique_flags", k_unique_flags, dim3(nb), dim3(256), 0, keys, n, f);
-  scan_exclusive_u32(c, f, pos, n);
-  PNM_LAUNCH(c, "k_unique_write", k_unique_write, dim3(nb), dim3(256), 0, keys, n, f, pos, out,
-                     cnt);
+  const int64_t nb = cdiv(n, SCAN_THREADS * UQ_ITEMS);
+  lb_reserve(c, nb);
+  PNM_LAUNCH(c, "k_unique", k_unique, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, keys, n, out, cnt,
+             c->lb_ctl, c->lb_state, (unsigned)nb, c->err_dev);
   uint64_t h;
   readback(c, cnt, 1, &h);
   return (int64_t)h;
