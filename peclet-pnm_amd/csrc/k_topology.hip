@@ -310,7 +310,8 @@ static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol 
   uint64_t* cnt = c->arena.get_n<uint64_t>(2);
   fill_u64(c, cnt, 0, 2);
   // keys that find no slot in a brick's table go to `ovf`; the staged tables are compacted to
-  // the head of `cand` and the overflow appended after them, on the device
+  // the head of `cand`, and the overflow (rare) is appended after them once the counts are read
+  // back
   if (ids)
     PNM_LAUNCH(c, "k_seg_topo", k_topo_cand<true>, grid, dim3(64, TOPO_BY, 1), 0, seg, *ids, v.nx,
                v.ny, v.nz, lo, bits, stage, scnt, ovf,
@@ -320,10 +321,11 @@ static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol 
                v.nx, v.ny, v.nz, lo, bits, stage, scnt, ovf,
                reinterpret_cast<unsigned long long*>(cnt + 1));
   compact_staged64(c, stage, scnt, nblk, TOPO_TABLE, cand, cnt);
-  PNM_LAUNCH(c, "k_topo_append", k_topo_append, dim3(64), dim3(256), 0, cand, cnt, ovf);
   uint64_t h[2];
   readback(c, cnt, 2, h);
   const int64_t ncand = (int64_t)h[0] + (int64_t)h[1];
+  if (h[1] > 0)  // keys that found no slot in their brick's table (rare)
+    PNM_LAUNCH(c, "k_topo_append", k_topo_append, dim3(64), dim3(256), 0, cand, cnt, ovf);
   if (ncand == 0) {
     *needed = 0;
     return 0;
