@@ -92,7 +92,6 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
                                                             bool write_all) {
   __shared__ int R[FR_N];
   __shared__ uint16_t cand_list[FLOOD_LIST];
-  __shared__ unsigned s_wcnt[TILE_Y];
   const int lane = threadIdx.x, wave = threadIdx.y;
   const int tid = wave * 64 + lane;
   constexpr int NT = 64 * TILE_Y;
@@ -172,8 +171,9 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
   // Band candidates: the tile's voxels that start unlabelled (with one sweep the exact region is
   // the tile itself). cand0 marks this thread's (flooded voxels lower the minimum); they are also
   // compacted into an LDS list, one segment per wave holding the wave's own row slice by slice,
-  // filled by ballot without atomics, so that the sweep touches only them. If a wave holds more
-  // candidates than its segment, the tile scans the region instead.
+  // filled by ballot without atomics, so that the sweep touches only them. A wave with more
+  // candidates than its segment sweeps its row slice by slice instead (each wave sweeps and
+  // later reads only its own row, so no barrier separates the listing from the sweep).
   static_assert(FLOOD_SWEEPS == 1, "the candidate region is the tile");
   constexpr int WL = FLOOD_LIST / TILE_Y;
   uint32_t cand0 = 0;
@@ -188,11 +188,7 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
     if (cand && pos < (unsigned)WL) cand_list[wave * WL + pos] = (uint16_t)e;
     nw += (unsigned)__popcll(m);
   }
-  if (lane == 0) s_wcnt[wave] = nw;
-  __syncthreads();
-  bool listed = true;
-#pragma unroll
-  for (int w = 0; w < TILE_Y; ++w) listed &= s_wcnt[w] <= (unsigned)WL;
+  const bool listed = nw <= (unsigned)WL;  // wave-uniform
   // One sweep at region entry e: the minimum labelled neighbour. As unsigned numbers every
   // unlabelled state (negative) lies above every label (a flat index below 2^31), so the
   // minimum of the 26 words is the smallest label if there is one.
@@ -210,35 +206,14 @@ __global__ __launch_bounds__(64 * TILE_Y) void k_flood_tile(const int* __restric
         }
     if (best < 0x80000000u) R[e] = -3 - (int)best;  // labelled this sweep: reads as unlabelled
   };
-  auto sweep_one = [&](int e, int k) {  // region scan: entries outside the exact region skip
-    const int rz = e / (FR_X * FR_Y), rem = e - rz * (FR_X * FR_Y);
-    const int ry = rem / FR_X, rx = rem - ry * FR_X;
-    if (rx < k || rx >= FR_X - k || ry < k || ry >= FR_Y - k || rz < k ||
-        rz >= nzt + 2 * H - k)
-      return;
-    sweep_at(e);
-  };
-  // Jacobi sweeps on the shrinking region
-  for (int k = 1; k <= H; ++k) {
-    if (listed) {
-      for (unsigned j = lane; j < nw; j += 64) sweep_at(cand_list[wave * WL + j]);
-    } else {
-      for (int row = wave; row < rows; row += TILE_Y)
-        for (int rx = lane; rx < FR_X; rx += 64) sweep_one(row * FR_X + rx, k);
-    }
-    __syncthreads();
-    // the last sweep's labels are decoded where the outputs read them (each wave reads its own
-    // row); the region scan applies every sweep, as its later sweeps read the region
-    if (!listed || k < H) {
-      for (int row = wave; row < rows; row += TILE_Y)
-        for (int rx = lane; rx < FR_X; rx += 64) {
-          const int e = row * FR_X + rx;
-          const int v = R[e];
-          if (v <= -3) R[e] = -3 - v;
-        }
-      __syncthreads();
-    }
+  // the one sweep, exact on the tile (the region with its one-voxel halo), over the wave's row
+  if (listed) {
+    for (unsigned j = lane; j < nw; j += 64) sweep_at(cand_list[wave * WL + j]);
+  } else {
+    for (int zl = 0; zl < nzt; ++zl) sweep_at(((zl + H) * FR_Y + wave + H) * FR_X + lane + H);
   }
+  __syncthreads();
+  // the sweep's labels (-3 - label) are decoded where the outputs read them
   // tile outputs, staged per wave in (slice, lane) order
   const int x = x0 + lane, y = y0 + wave;
   const bool col_in = x < nx && y < ny;
