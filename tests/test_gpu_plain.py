@@ -140,8 +140,8 @@ def test_pore_heavy_rows(capi_ctx):
 
 def test_band_heavy_rows(capi_ctx):
     """Mostly unlabelled solid (-1.5 < sdf < 0) with sparse markers: tile rows with more band
-    candidates than the flood tile's per-wave list holds take its region scan, and the flood
-    needs many global sweeps."""
+    candidates than the flood tile's per-wave list holds sweep their row slice by slice, and the
+    flood needs many global sweeps."""
     rng = np.random.default_rng(78)
     sdf = np.round(rng.uniform(-1.25, -0.25, size=(24, 20, 130)) / 0.25) * 0.25
     sdf[rng.random(sdf.shape) < 0.03] = -3.0
