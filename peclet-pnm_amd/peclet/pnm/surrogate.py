@@ -2,7 +2,7 @@
 
 The network and its training run on the GPU through the C-ABI (pnm_sur_*: BF16-MFMA convolutions,
 FP32 dense head, Adam). The plain bookkeeping (extract_network_flow on solved fields) produces
-the targets and remains the fallback and the judge of the surrogate's error.
+the targets and remains the fallback and the yardstick of the surrogate's error.
 
 Per throat the inputs are
   - a 16^3 SDF patch centred on the interface centroid (throat_center), scaled by 1/8;
