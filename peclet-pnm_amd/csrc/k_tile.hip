@@ -944,19 +944,35 @@ __global__ __launch_bounds__(256) void k_ccl_seams(int* L, const int2* __restric
 }
 
 // ---- tile-local roots -> component roots ----
-// One wave per staging segment of k_tile: after the seams, every tile-local root points at its
-// component root, so a marker's component is L[L[i]].
+// After the seams, every tile-local root points at its component root, so a marker's component
+// is L[L[i]]. A wave takes LRF_SEGS consecutive staging segments of k_tile (a tile row has only a
+// few dozen local roots) and chases their entries as one list, 64 at a time.
+constexpr int LRF_SEGS = 4;
 __global__ __launch_bounds__(256) void k_lroot_flatten(int* L, const uint32_t* __restrict__ stage,
                                                        const uint32_t* __restrict__ cnt,
                                                        int64_t nseg) {
-  const int64_t seg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (seg >= nseg) return;
+  const int64_t seg0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * LRF_SEGS;
+  if (seg0 >= nseg) return;  // wave-uniform; the kernel has no block barrier
   const int lane = threadIdx.x & 63;
-  const unsigned c = cnt[seg];
-  for (unsigned j = lane; j < c; j += 64) {
+  unsigned c[LRF_SEGS], total = 0;
+#pragma unroll
+  for (int s = 0; s < LRF_SEGS; ++s) {
+    c[s] = seg0 + s < nseg ? cnt[seg0 + s] : 0u;
+    total += c[s];
+  }
+  for (unsigned j = lane; j < total; j += 64) {
+    // entry j of the concatenated segments
+    unsigned jj = j;
+    int64_t seg = seg0;
+#pragma unroll
+    for (int s = 0; s < LRF_SEGS - 1; ++s)
+      if (seg == seg0 + s && jj >= c[s]) {
+        jj -= c[s];
+        seg = seg0 + s + 1;
+      }
     // read-only chase, one final store: a path-halving store here could overwrite the root
     // another thread has just stored for its own local root with a lesser ancestor
-    const int r = (int)stage[seg * T_SEG + j];
+    const int r = (int)stage[seg * T_SEG + jj];
     int x = r, p = L[r];
     while (p != x) {
       x = p;
@@ -969,8 +985,8 @@ __global__ __launch_bounds__(256) void k_lroot_flatten(int* L, const uint32_t* _
 void launch_lroot_flatten(Ctx* c, Vol v, int32_t* labels, const uint32_t* stage,
                           const uint32_t* cnt) {
   const int64_t nseg = tile_segments(v);
-  PNM_LAUNCH(c, "k_lroot_flatten", k_lroot_flatten, dim3(cdiv(nseg, 4)), dim3(256), 0, labels,
-             stage, cnt, nseg);
+  PNM_LAUNCH(c, "k_lroot_flatten", k_lroot_flatten, dim3(cdiv(nseg, 4 * LRF_SEGS)), dim3(256), 0,
+             labels, stage, cnt, nseg);
 }
 
 // ---- walks that left their tile ----
