@@ -141,26 +141,31 @@ void launch_pre(Ctx* c, const float* sdf, Vol v, float thr, const PreOut& o) {
 }
 
 // Squared-positive-SDF weighted centroid, dz -> dy -> dx order, FP32
-// (pore_extraction.hpp:107-127, identical block at :543-562).
-__global__ void k_centroids(const float* __restrict__ sdf, int nx, int ny, int nz, float ox,
-                            float oy, float oz, float sx, float sy, float sz,
-                            const uint32_t* __restrict__ peaks, int64_t count,
-                            pnm_pore_t* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  const int ci = (int)peaks[k];
+// (pore_extraction.hpp:107-127, identical block at :543-562). A wave takes two pores: lanes
+// 0..26 and 32..58 load one of a pore's 27 values each (so the peaks' scattered loads are spread
+// over many waves), and the pore's first lane sums them from LDS in the reference's order.
+constexpr int CENT_PER_BLOCK = 8;  // 4 waves x 2 pores
+__global__ __launch_bounds__(256) void k_centroids(const float* __restrict__ sdf, int nx, int ny,
+                                                   int nz, float ox, float oy, float oz, float sx,
+                                                   float sy, float sz,
+                                                   const uint32_t* __restrict__ peaks,
+                                                   int64_t count, pnm_pore_t* __restrict__ out) {
+  __shared__ float V[CENT_PER_BLOCK][32];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = wave * 2 + (lane >> 5), q = lane & 31;
+  const int64_t k = (int64_t)blockIdx.x * CENT_PER_BLOCK + slot;
+  const bool valid = k < count;
+  const int ci = valid ? (int)peaks[k] : 0;
   const int ix = ci % nx, iy = (ci / nx) % ny, iz = ci / (nx * ny);
-  const int xs[3] = {wm1(ix, nx), ix, wp1(ix, nx)};
-  const int ys[3] = {wm1(iy, ny), iy, wp1(iy, ny)};
-  const int zs[3] = {wm1(iz, nz), iz, wp1(iz, nz)};
-  // the 27 values are loaded before any is used, so that all loads are in flight together
-  float vals[27];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) vals[(a * 3 + b) * 3 + cc] = sdf[(zs[a] * ny + ys[b]) * nx + xs[cc]];
+  if (valid && q < 27) {
+    const int a = q / 9, b = (q / 3) % 3, cc = q % 3;
+    const int zz = a == 0 ? wm1(iz, nz) : (a == 1 ? iz : wp1(iz, nz));
+    const int yy = b == 0 ? wm1(iy, ny) : (b == 1 ? iy : wp1(iy, ny));
+    const int xx = cc == 0 ? wm1(ix, nx) : (cc == 1 ? ix : wp1(ix, nx));
+    V[slot][q] = sdf[(zz * ny + yy) * nx + xx];
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (!valid || q != 0) return;
   float sw = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -168,7 +173,7 @@ __global__ void k_centroids(const float* __restrict__ sdf, int nx, int ny, int n
     for (int b = 0; b < 3; ++b)
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
-        const float val = vals[(a * 3 + b) * 3 + cc];
+        const float val = V[slot][(a * 3 + b) * 3 + cc];
         float w = val > 0.0f ? val : 0.0f;
         w = w * w;
         sw += w;
@@ -186,16 +191,15 @@ __global__ void k_centroids(const float* __restrict__ sdf, int nx, int ny, int n
   P.x = ox + ((float)ix + fx) * sx;
   P.y = oy + ((float)iy + fy) * sy;
   P.z = oz + ((float)iz + fz) * sz;
-  P.radius = sdf[ci];
+  P.radius = V[slot][13];  // the centre value
   out[k] = P;
 }
 
 void launch_centroids(Ctx* c, const float* sdf, Vol v, const float org[3], const float spc[3],
                       const uint32_t* peaks, int64_t count, pnm_pore_t* out) {
   if (count <= 0) return;
-  PNM_LAUNCH(c, "k_centroids", k_centroids, dim3(cdiv(count, 256)), dim3(256), 0, sdf, v.nx,
-                     v.ny, v.nz, org[0], org[1], org[2], spc[0], spc[1], spc[2], peaks, count,
-                     out);
+  PNM_LAUNCH(c, "k_centroids", k_centroids, dim3(cdiv(count, CENT_PER_BLOCK)), dim3(256), 0, sdf,
+             v.nx, v.ny, v.nz, org[0], org[1], org[2], spc[0], spc[1], spc[2], peaks, count, out);
 }
 
 // Successor of the gradient walk. Mode kWalkPores: ascent for sdf > 0 voxels, descent for the
