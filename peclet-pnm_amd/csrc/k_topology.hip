@@ -274,10 +274,14 @@ __global__ void k_topo_append(uint64_t* __restrict__ cand, const uint64_t* __res
     cand[base + j] = ovf[j];
 }
 
-__global__ void k_topo_decode(const uint64_t* __restrict__ keys, int64_t count, int32_t lo,
+// The unique count is read on the device (the grid covers every candidate), and nothing is
+// written when it exceeds the caller's capacity.
+__global__ void k_topo_decode(const uint64_t* __restrict__ keys,
+                              const uint64_t* __restrict__ count_dev, int64_t cap, int32_t lo,
                               int bits, int32_t* __restrict__ pairs) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= count) return;
+  const int64_t count = (int64_t)*count_dev;
+  if (count > cap || k >= count) return;
   const uint64_t key = keys[k];
   const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1);
   pairs[2 * k] = (int32_t)((uint32_t)(key >> bits) + (uint32_t)lo);
@@ -332,12 +336,16 @@ static int64_t topology_impl(Ctx* c, const int32_t* seg, const SegIds* ids, Vol 
   }
   radix_sort_u64(c, cand, nullptr, ncand, 0, 2 * bits);
   uint64_t* uniq = c->arena.get_n<uint64_t>((size_t)ncand);
-  const int64_t nu = unique_sorted_u64(c, cand, uniq, ncand);
+  uint64_t* nu_dev = c->arena.get_n<uint64_t>(1);
+  unique_sorted_u64_dev(c, cand, uniq, ncand, nu_dev);
+  // decoded before the count is read back: one host round trip for both
+  PNM_LAUNCH(c, "k_topo_decode", k_topo_decode, dim3(cdiv(ncand, 256)), dim3(256), 0, uniq,
+             nu_dev, cap, lo, bits, pairs_d);
+  uint64_t hn;
+  readback(c, nu_dev, 1, &hn);
+  const int64_t nu = (int64_t)hn;
   *needed = nu;
-  if (nu > cap) return -1;
-  PNM_LAUNCH(c, "k_topo_decode", k_topo_decode, dim3(cdiv(nu, 256)), dim3(256), 0, uniq, nu, lo,
-             bits, pairs_d);
-  return nu;
+  return nu > cap ? -1 : nu;
 }
 
 int64_t topology(Ctx* c, const int32_t* seg, Vol v, int32_t lo_label, int32_t hi_label,

@@ -735,16 +735,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_unique(const uint64_t* __restr
   }
 }
 
-int64_t unique_sorted_u64(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n) {
-  if (n <= 0) return 0;
-  uint64_t* cnt = c->arena.get_n<uint64_t>(1);
+void unique_sorted_u64_dev(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n,
+                           uint64_t* count_dev) {
+  if (n <= 0) {
+    fill_u64(c, count_dev, 0, 1);
+    return;
+  }
   const int64_t nb = cdiv(n, SCAN_THREADS * UQ_ITEMS);
   lb_reserve(c, nb);
-  PNM_LAUNCH(c, "k_unique", k_unique, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, keys, n, out, cnt,
-             c->lb_ctl, c->lb_state, (unsigned)nb, c->err_dev);
-  uint64_t h;
-  readback(c, cnt, 1, &h);
-  return (int64_t)h;
+  PNM_LAUNCH(c, "k_unique", k_unique, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, keys, n, out,
+             count_dev, c->lb_ctl, c->lb_state, (unsigned)nb, c->err_dev);
 }
+
 
 }  // namespace pnm

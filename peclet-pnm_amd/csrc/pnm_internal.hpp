@@ -205,7 +205,10 @@ void compact_staged_multi(Ctx* c, int K, const uint32_t* const* stage, const uin
 void compact_staged64(Ctx* c, const uint64_t* stage, const uint32_t* counts, int64_t nblocks,
                       int64_t cap, uint64_t* out, uint64_t* total_dev);
 // Writes keys[i] with keys[i] != keys[i-1] to out; returns the count (synchronises).
-int64_t unique_sorted_u64(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n);
+// Unique of a sorted list into out; the count is left on the device (*count_dev), no host
+// synchronisation.
+void unique_sorted_u64_dev(Ctx* c, const uint64_t* keys, uint64_t* out, int64_t n,
+                           uint64_t* count_dev);
 
 // ---------- plain pipeline kernels (k_extract.hip, k_segment.hip, k_topology.hip) ----------
 struct PreOut {

@@ -261,6 +261,19 @@ def test_capacity_errors(capi_ctx):
         capi_ctx.extract_pore_network_d(s, (8, 8, 8), (0, 0, 0), (1, 1, 1), small, seg, pairs)
 
 
+def test_pairs_capacity_writes_nothing(capi_ctx):
+    """A pairs buffer below the unique pair count: PNM_ECAPACITY, and the buffer is not written
+    (the decode reads the count on the device and skips when it exceeds the capacity)."""
+    sdf = quantized((8, 8, 8), 5)
+    s = dev(sdf)
+    seg = torch.empty(sdf.size, dtype=torch.int32, device="cuda")
+    pores = torch.empty(4 * sdf.size, dtype=torch.float32, device="cuda")
+    pairs = torch.full((2,), -7, dtype=torch.int32, device="cuda")
+    with pytest.raises(RuntimeError, match="too small"):
+        capi_ctx.extract_pore_network_d(s, (8, 8, 8), (0, 0, 0), (1, 1, 1), pores, seg, pairs)
+    assert pairs.cpu().tolist() == [-7, -7]
+
+
 def _torch_topology(seg3):
     """Independent full-size topology on the GPU with torch ops."""
     keys = []
