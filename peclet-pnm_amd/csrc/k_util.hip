@@ -17,8 +17,24 @@ static unsigned fill_blocks(int64_t count) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
+// 16-byte stores for an aligned body, the tail (fewer than 4 words) by the first threads.
+__global__ void k_fill_u32x4(uint32_t* __restrict__ p, uint32_t v, int64_t count) {
+  const int64_t n4 = count >> 2;
+  uint4* __restrict__ p4 = reinterpret_cast<uint4*>(p);
+  const uint4 vv = make_uint4(v, v, v, v);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t; i < n4; i += stride) p4[i] = vv;
+  if (t < (count & 3)) p[(n4 << 2) + t] = v;
+}
+
 void fill_u32(Ctx* c, uint32_t* p, uint32_t v, int64_t count) {
   if (count <= 0) return;
+  if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0 && count >= 1024) {
+    PNM_LAUNCH(c, "k_fill", k_fill_u32x4, dim3(fill_blocks((count + 3) / 4)), dim3(256), 0, p, v,
+               count);
+    return;
+  }
   PNM_LAUNCH(c, "k_fill", k_fill<uint32_t>, dim3(fill_blocks(count)), dim3(256), 0, p, v,
                      count);
 }
